@@ -1,0 +1,668 @@
+"""CPU oracle for the nssDMRG superblock path — TEST INFRASTRUCTURE ONLY.
+
+This module is the checker, never the product: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it.  The product path (``nssdmrg_amd``) never imports anything from here and
+fails loudly if its HIP library is missing.
+
+It restates, in plain numpy and in the reference's own basis ordering, the
+algorithm of aamaricci/nssDMRG for one DMRG step and the infinite/finite
+drivers around it.  Each function cites the reference file:line it follows.
+
+Third-party arithmetic the reference delegates to SciFortran (not vendored,
+version unpinned: CI clones HEAD, ``.github/workflows/PushWorkflow.yml:38-41``):
+  * ``sp_eigh`` (ARPACK implicitly restarted Lanczos) is restated as a plain
+    Lanczos with full re-orthogonalisation (``lanczos_lowest``); tests cross
+    check it against scipy's ARPACK (``eigsh``) and LAPACK (``eigh``);
+  * ``eigh`` (LAPACK dsyev) is ``numpy.linalg.eigh`` (LAPACK dsyevd);
+  * ``sort_quicksort`` tie order is not reproducible (parity unpinned for the
+    order of exactly tied rho eigenvalues; kept counts are tie independent
+    because the degeneracy window always keeps every tied value).
+
+Pinning (tests/test_oracle_*.py): the reference's golden ``.check`` values
+(spin-1/2 Heisenberg L40 M20 and Hubbard U=0 L40 M40 trajectories), closed
+forms (4-site Heisenberg, free fermions), brute-force exact diagonalisation of
+full chains, and invariants (symmetry of H_sb, trace of rho, lambda_L=lambda_R).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+TINY = np.finfo(np.float64).tiny   # Fortran tiny(1d0)
+EPS_LOG = 1e-32                    # src/DMRG_MAIN.f90:509 (epsi)
+
+# ---------------------------------------------------------------------------
+# Seeded RNG: SplitMix64 -> uniform[-1,1).  Same stream as the HIP library's
+# host-side start vector (nssdmrg_amd/csrc/nss_rng.h).
+# ---------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+
+
+def splitmix64_uniform(n: int, seed: int) -> np.ndarray:
+    """n doubles uniform in [-1,1) from SplitMix64(seed) (53-bit mantissa)."""
+    out = np.empty(n, dtype=np.float64)
+    s = seed & _M64
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & _M64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (2.0 ** -53) * 2.0 - 1.0
+    return out
+
+
+def splitmix64_uniform_fast(n: int, seed: int) -> np.ndarray:
+    """Vectorised SplitMix64 (identical stream to splitmix64_uniform)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        s = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53) * 2.0 - 1.0
+
+
+# ---------------------------------------------------------------------------
+# Sites (src/BLOCKS/SITES.f90, src/FOCK/HLOCAL.f90)
+# ---------------------------------------------------------------------------
+@dataclass
+class Site:
+    kind: str                       # "spin" | "fermion"
+    dim: int
+    qn: np.ndarray                  # (dim, qdim) float
+    ops: dict                       # name -> dense matrix
+    optype: dict                    # name -> "bosonic" | "fermionic" | "psign"
+
+
+def spin_site(sun: int = 2, hvec=(0.0, 0.0)) -> Site:
+    """spin_site, src/BLOCKS/SITES.f90:281-344 (SU(2) and SU(3) presets)."""
+    if sun == 2:
+        sz = np.diag([0.5, -0.5])
+        sp = np.array([[0.0, 1.0], [0.0, 0.0]])           # Sp(1,2)=1, :318
+        px = np.array([[0.0, 1.0], [1.0, 0.0]])
+        pz = np.array([[1.0, 0.0], [0.0, -1.0]])
+        h = hvec[0] * px + hvec[1] * pz
+        qn = np.array([[0.5], [-0.5]])
+    elif sun == 3:
+        sz = np.diag([1.0, 0.0, -1.0])
+        sp = np.zeros((3, 3))
+        sp[0, 1] = math.sqrt(2.0)
+        sp[1, 2] = math.sqrt(2.0)
+        sx = np.array([[0, 1, 0], [1, 0, 1], [0, 1, 0]], float) / math.sqrt(2.0)
+        h = hvec[0] * sx + hvec[1] * sz
+        qn = np.array([[1.0], [0.0], [-1.0]])
+    else:
+        raise ValueError("spin_site: SU(N) value not supported")
+    return Site("spin", sz.shape[0], qn,
+                {"H": h, "Sz": sz, "Sp": sp},
+                {"H": "bosonic", "Sz": "bosonic", "Sp": "bosonic"})
+
+
+def electron_site(uloc: float = 0.0, hfmode: bool = True, eloc=(0.0, 0.0)) -> Site:
+    """electron_site for Norb=1, src/BLOCKS/SITES.f90:358-434.
+
+    Fock basis m = iup + 2*idw, |up,dw> with up fastest
+    (src/FOCK/HLOCAL.f90:735-758); QN [Nup,Ndw].  Local H from
+    build_Hlocal_operator :312-581 (crystal field, U n_up n_dw, Hartree shift
+    -U/2 (n_up+n_dw) when HFMODE=T :423-435).  C operators carry the
+    Jordan-Wigner sign of src/FOCK/HLOCAL.f90:884-896.
+    """
+    dim = 4
+    qn = np.zeros((dim, 2))
+    h = np.zeros((dim, dim))
+    cup = np.zeros((dim, dim))
+    cdn = np.zeros((dim, dim))
+    p = np.zeros((dim, dim))
+    for m in range(dim):
+        nup, ndw = m & 1, (m >> 1) & 1
+        qn[m] = (nup, ndw)
+        e = eloc[0] * nup + eloc[1] * ndw + uloc * nup * ndw
+        if hfmode:
+            e -= 0.5 * uloc * (nup + ndw)
+        h[m, m] = e
+        p[m, m] = (-1.0) ** (nup + ndw)
+        if nup:                                  # c_up: no bit below -> +1
+            cup[m & ~1, m] = 1.0
+        if ndw:                                  # c_dw: sign (-1)^n_up
+            cdn[m & ~2, m] = -1.0 if nup else 1.0
+    return Site("fermion", dim, qn,
+                {"H": h, "Cup": cup, "Cdn": cdn, "P": p},
+                {"H": "bosonic", "Cup": "fermionic", "Cdn": "fermionic", "P": "psign"})
+
+
+# ---------------------------------------------------------------------------
+# Blocks (src/BLOCKS/BLOCKS.f90) with dense operators in the reference basis
+# ---------------------------------------------------------------------------
+@dataclass
+class Block:
+    kind: str
+    length: int
+    qn: np.ndarray                  # (dim, qdim), basis-state QNs in basis order
+    ops: dict
+    optype: dict
+
+    @property
+    def dim(self) -> int:
+        return self.qn.shape[0]
+
+    def copy(self) -> "Block":
+        return Block(self.kind, self.length, self.qn.copy(),
+                     {k: v.copy() for k, v in self.ops.items()}, dict(self.optype))
+
+
+def block_from_site(s: Site) -> Block:
+    return Block(s.kind, 1, s.qn.copy(), {k: v.copy() for k, v in s.ops.items()},
+                 dict(s.optype))
+
+
+def sectors(qn: np.ndarray):
+    """QN -> increasing state list, in first-appearance order.
+
+    sectors_list(tbasis) / appends, src/LIST/LIST_SECTORS.f90:286-325.
+    """
+    order, maps = [], {}
+    for i, q in enumerate(map(tuple, qn)):
+        if q not in maps:
+            maps[q] = []
+            order.append(q)
+        maps[q].append(i)
+    return order, {q: np.array(v, dtype=np.int64) for q, v in maps.items()}
+
+
+def _connect(left_ops, right_ops, kind, hij):
+    """connect_{spin,fermion}_blocks, src/DMRG_CONNECT.f90:296-454 (link 'n')."""
+    if kind == "spin":
+        return (hij[0][0] * np.kron(left_ops["Sz"], right_ops["Sz"])
+                + hij[1][1] * np.kron(left_ops["Sp"], right_ops["Sp"].T)
+                + hij[1][1] * np.kron(left_ops["Sp"].T, right_ops["Sp"]))
+    pl = left_ops["P"]
+    out = 0.0
+    names = ["Cup", "Cdn"]
+    for io in range(2):
+        for jo in range(2):
+            t = hij[io][jo]
+            if t == 0.0:
+                continue
+            cl, cr = left_ops[names[io]], right_ops[names[jo]]
+            out = out + t * np.kron(cl.T @ pl, cr) + t * np.kron(pl @ cl, cr.T)
+    return out
+
+
+def enlarge_block(b: Block, dot: Site, side: str, hij) -> Block:
+    """enlarge_block, src/DMRG_CONNECT.f90:18-291 (OBC, link 'n').
+
+    left : H = H_b x 1_d + 1_b x H_d + H_{b,d};  O -> (1_b | P_b) x O_d
+    right: H = 1_d x H_b + H_d x 1_b + H_{d,b};  O -> O_d x 1_b,  P -> P_d x P_b
+    basis: left = self .o. dot, right = dot .o. self (QN_TUPLE_BASIS.f90:371-398).
+    """
+    db, dd = b.dim, dot.dim
+    ib, idd = np.eye(db), np.eye(dd)
+    ops, optype = {}, dict(b.optype)
+    if side == "left":
+        ops["H"] = (np.kron(b.ops["H"], idd) + np.kron(ib, dot.ops["H"])
+                    + _connect(b.ops, dot.ops, b.kind, hij))
+        for k, t in b.optype.items():
+            if k == "H":
+                continue
+            eo = ib if t == "bosonic" else b.ops["P"]
+            ops[k] = np.kron(eo, dot.ops[k])
+        qn = (b.qn[:, None, :] + dot.qn[None, :, :]).reshape(db * dd, -1)
+    else:
+        ops["H"] = (np.kron(idd, b.ops["H"]) + np.kron(dot.ops["H"], ib)
+                    + _connect(dot.ops, b.ops, b.kind, hij))
+        for k, t in b.optype.items():
+            if k == "H":
+                continue
+            eo = b.ops["P"] if t == "psign" else ib
+            ops[k] = np.kron(dot.ops[k], eo)
+        qn = (dot.qn[:, None, :] + b.qn[None, :, :]).reshape(db * dd, -1)
+    return Block(b.kind, b.length + 1, qn, ops, optype)
+
+
+# ---------------------------------------------------------------------------
+# Superblock: states, direct term table and H*v
+# ---------------------------------------------------------------------------
+@dataclass
+class Superblock:
+    target: tuple
+    qns: list                       # left QN per SB sector
+    Dl: np.ndarray
+    Dr: np.ndarray
+    offset: np.ndarray
+    lmap: list                      # left block states per sector
+    rmap: list                      # right block states per sector
+    HL: list = field(default_factory=list)
+    HR: list = field(default_factory=list)
+    terms: list = field(default_factory=list)   # (dst, src, A, B, ishc)
+
+    @property
+    def n(self) -> int:
+        return int((self.Dl * self.Dr).sum())
+
+
+def current_target(length: int, target_qn, norb: int = 1) -> tuple:
+    """sb_set_current_qn, QNtype=local: int(target*L*Norb), src/DMRG_SUPERBLOCK.f90:717-726."""
+    return tuple(float(int(t * length * norb)) for t in target_qn)
+
+
+def sb_get_states(left: Block, right: Block, target: tuple) -> Superblock:
+    """sb_get_states + sb_build_dims, src/DMRG_SUPERBLOCK.f90:36-121,167-283."""
+    lord, lmaps = sectors(left.qn)
+    _, rmaps = sectors(right.qn)
+    qns, dls, drs, lm, rm = [], [], [], [], []
+    for q in lord:
+        rq = tuple(t - a for t, a in zip(target, q))
+        if rq not in rmaps:
+            continue
+        qns.append(q)
+        lm.append(lmaps[q])
+        rm.append(rmaps[rq])
+        dls.append(len(lmaps[q]))
+        drs.append(len(rmaps[rq]))
+    if not qns:
+        raise RuntimeError("sb_get_states ERROR: total_states=0. There are no SB states.")
+    dl, dr = np.array(dls), np.array(drs)
+    off = np.concatenate([[0], np.cumsum(dl * dr)[:-1]]).astype(np.int64)
+    return Superblock(target, qns, dl, dr, off, lm, rm)
+
+
+def setup_direct(sb: Superblock, left: Block, right: Block, hij) -> Superblock:
+    """Setup_SuperBlock_{Spin,Fermion}_Direct, src/DMRG_SUPERBLOCK_SETUP.f90:152-693.
+
+    Terms are (dst, src, A, B, is_hconjg): Y_dst += A . Psi_src . B^T.
+    """
+    index = {q: i for i, q in enumerate(sb.qns)}
+    sb.HL = [left.ops["H"][np.ix_(m, m)] for m in sb.lmap]
+    sb.HR = [right.ops["H"][np.ix_(m, m)] for m in sb.rmap]
+    sb.terms = []
+    for isb, q in enumerate(sb.qns):
+        ai, bi = sb.lmap[isb], sb.rmap[isb]
+        if left.kind == "spin":                       # :289-321
+            a = hij[0][0] * left.ops["Sz"][np.ix_(ai, ai)]
+            b = right.ops["Sz"][np.ix_(bi, bi)]
+            sb.terms.append((isb, isb, a, b, 0))
+            qm = (q[0] - 1.0,)
+            if qm not in index:
+                continue
+            jsb = index[qm]
+            aj, bj = sb.lmap[jsb], sb.rmap[jsb]
+            a2 = hij[1][1] * left.ops["Sp"][np.ix_(ai, aj)]
+            b2 = right.ops["Sp"].T[np.ix_(bi, bj)]
+            sb.terms.append((isb, jsb, a2, b2, 0))
+            sb.terms.append((jsb, isb, a2.T.copy(), b2.T.copy(), 1))
+        else:                                          # :574-641
+            names = ["Cup", "Cdn"]
+            pl = left.ops["P"]
+            for ispin, dq in enumerate([(1.0, 0.0), (0.0, 1.0)]):
+                qm = (q[0] - dq[0], q[1] - dq[1])
+                if qm not in index:
+                    continue
+                jsb = index[qm]
+                aj, bj = sb.lmap[jsb], sb.rmap[jsb]
+                t = hij[ispin][ispin]
+                if t == 0.0:
+                    continue
+                a1 = t * (left.ops[names[ispin]].T @ pl)[np.ix_(ai, aj)]
+                b1 = right.ops[names[ispin]][np.ix_(bi, bj)]
+                sb.terms.append((isb, jsb, a1, b1, 0))
+                sb.terms.append((jsb, isb, a1.T.copy(), b1.T.copy(), 1))
+    return sb
+
+
+def hv_direct(sb: Superblock, v: np.ndarray) -> np.ndarray:
+    """spMatVec_direct_main, src/DMRG_SUPERBLOCK_SETUP.f90:740-865, matrix form.
+
+    Psi_q = v[off_q : off_q + Dl*Dr] as C row-major (Dl, Dr): element
+    i = ir + (il-1)*Dr + off (Fortran) -> [il, ir].  Order of evaluation is the
+    reference's: 1xH_R, H_Lx1, then for each term C = B.V (B first), Hv += C.A^T.
+    """
+    hv = np.zeros_like(v)
+    psi = [v[o:o + dl * dr].reshape(dl, dr) for o, dl, dr in zip(sb.offset, sb.Dl, sb.Dr)]
+    out = [hv[o:o + dl * dr].reshape(dl, dr) for o, dl, dr in zip(sb.offset, sb.Dl, sb.Dr)]
+    for k in range(len(sb.qns)):
+        out[k] += psi[k] @ sb.HR[k].T               # :767-779
+        out[k] += sb.HL[k] @ psi[k]                 # :784-796
+    for dst, src, a, b, _ in sb.terms:               # :808-857
+        w = psi[src] @ b.T                           # C = B.V   (Dl_src x Dr_dst)
+        out[dst] += a @ w                            # Hv += A.C^T
+    return hv
+
+
+def hv_direct_loops(sb: Superblock, v: np.ndarray) -> np.ndarray:
+    """Scalar-loop twin of hv_direct for tiny superblocks (reference loop order)."""
+    hv = np.zeros_like(v)
+    for k in range(len(sb.qns)):
+        dl, dr, off = int(sb.Dl[k]), int(sb.Dr[k]), int(sb.offset[k])
+        for il in range(dl):
+            for ir in range(dr):
+                i = ir + il * dr + off
+                for jr in range(dr):
+                    hv[i] += sb.HR[k][ir, jr] * v[jr + il * dr + off]
+        for ir in range(dr):
+            for il in range(dl):
+                i = ir + il * dr + off
+                for jl in range(dl):
+                    hv[i] += sb.HL[k][il, jl] * v[ir + jl * dr + off]
+    for dst, src, a, b, _ in sb.terms:
+        dld, drd = a.shape[0], b.shape[0]
+        dls, drs = a.shape[1], b.shape[1]
+        c = np.zeros((drd, dls))
+        for aj in range(dls):
+            for bi in range(drd):
+                for bj in range(drs):
+                    c[bi, aj] += b[bi, bj] * v[bj + aj * drs + sb.offset[src]]
+        for bi in range(drd):
+            for ai in range(dld):
+                i = bi + ai * drd + sb.offset[dst]
+                for aj in range(dls):
+                    hv[i] += a[ai, aj] * c[bi, aj]
+    return hv
+
+
+def hv_flops(sb: Superblock) -> float:
+    """Algorithmic FLOPs of one direct H*v with dense sector blocks."""
+    f = float(np.sum(2.0 * sb.Dl * sb.Dr * (sb.Dl + sb.Dr)))
+    for dst, src, _, _, _ in sb.terms:
+        f += 2.0 * sb.Dr[dst] * (sb.Dl[src] * sb.Dr[src] + sb.Dl[dst] * sb.Dl[src])
+    return f
+
+
+def sb_dense(sb: Superblock) -> np.ndarray:
+    """Dense H_sb (same matrix as Setup_SuperBlock_Sparse + dump, SETUP:46-88)."""
+    n = sb.n
+    h = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        h[:, j] = hv_direct(sb, e)
+    return h
+
+
+# ---------------------------------------------------------------------------
+# Eigen-solvers (restating SciFortran sp_eigh / eigh)
+# ---------------------------------------------------------------------------
+def lanczos_lowest(matvec, n: int, neigen: int, v0: np.ndarray, tol: float = 1e-12,
+                   kmax: int = 600):
+    """Lowest `neigen` eigenpairs by Lanczos with full re-orthogonalisation.
+
+    Restates the role of SciFortran sp_eigh (src/DMRG_SUPERBLOCK.f90:412-425):
+    converged eigenvalues/vectors to residual ||Hx - e x|| <= tol*max(1,|e|).
+    Returns (evals ascending, evecs (n, neigen), number of H*v).
+    """
+    kmax = min(kmax, n)
+    vecs = np.zeros((kmax + 1, n))
+    q = v0 / np.linalg.norm(v0)
+    vecs[0] = q
+    alpha, beta = [], []
+    nop = 0
+    for j in range(kmax):
+        w = matvec(vecs[j])
+        nop += 1
+        a = float(np.dot(vecs[j], w))
+        w -= a * vecs[j]
+        if j > 0:
+            w -= beta[-1] * vecs[j - 1]
+        for _ in range(2):
+            w -= vecs[:j + 1].T @ (vecs[:j + 1] @ w)
+        b = float(np.linalg.norm(w))
+        alpha.append(a)
+        beta.append(b)
+        m = j + 1
+        if m >= neigen:
+            t = np.diag(alpha) + np.diag(beta[:-1], 1) + np.diag(beta[:-1], -1)
+            th, s = np.linalg.eigh(t)
+            res = np.abs(b * s[-1, :neigen])
+            if np.all(res <= tol * np.maximum(1.0, np.abs(th[:neigen]))) or m == kmax or b < 1e-300:
+                x = vecs[:m].T @ s[:, :neigen]
+                x /= np.linalg.norm(x, axis=0)
+                return th[:neigen], x, nop
+        vecs[j + 1] = w / b
+    raise RuntimeError("lanczos_lowest: not converged")
+
+
+# ---------------------------------------------------------------------------
+# RDM + truncation (src/DMRG_RDM.f90, src/MATRIX/MATRIX_BLOCKS.f90)
+# ---------------------------------------------------------------------------
+@dataclass
+class RhoBlocks:
+    qns: list
+    maps: list
+    evals_blk: list
+    evecs_blk: list
+    evalues: np.ndarray = None      # all evals, sorted descending
+    eorder: np.ndarray = None       # (block, position) per sorted eval
+
+
+def build_rho(sb: Superblock, psi: np.ndarray, side: str) -> RhoBlocks:
+    """sb_get_rdm + build_density_matrix, src/DMRG_RDM.f90:18-162."""
+    qns, maps, blocks = [], [], []
+    for k, q in enumerate(sb.qns):
+        dl, dr, off = int(sb.Dl[k]), int(sb.Dr[k]), int(sb.offset[k])
+        p = psi[off:off + dl * dr].reshape(dl, dr)
+        if side == "left":
+            rho = p @ p.T
+            qns.append(q)
+            maps.append(sb.lmap[k])
+        else:
+            rho = p.T @ p
+            qns.append(tuple(t - a for t, a in zip(sb.target, q)))
+            maps.append(sb.rmap[k])
+        if np.any(np.abs(rho - rho.T) > 1e-12):
+            raise RuntimeError("build_density_matrix error: rho not Symmetric")
+        blocks.append(rho)
+    return eigh_blocks(qns, maps, blocks)
+
+
+def eigh_blocks(qns, maps, blocks) -> RhoBlocks:
+    """eigh_blocks_matrix(sort=T, reverse=T), src/MATRIX/MATRIX_BLOCKS.f90:589-657."""
+    ev_blk, vec_blk, allv, order = [], [], [], []
+    for ib, rho in enumerate(blocks):
+        e, u = np.linalg.eigh(rho)
+        e = np.where(e < 0.0, TINY, e)
+        ev_blk.append(e)
+        vec_blk.append(u)
+        allv.extend(e.tolist())
+        order.extend((ib, p) for p in range(len(e)))
+    allv = np.array(allv)
+    asc = np.argsort(allv, kind="stable")
+    desc = asc[::-1]
+    r = RhoBlocks(qns, maps, ev_blk, vec_blk)
+    r.evalues = allv[desc]
+    r.eorder = [order[i] for i in desc]
+    return r
+
+
+def kept_count(evals: np.ndarray, mstates: int, mdim: int, deg_thr: float,
+               estates: float = 0.0) -> int:
+    """m_s selection + degeneracy extension, src/DMRG_RDM.f90:204-218."""
+    if mstates != 0:
+        ms = min(mstates, mdim, len(evals))
+    elif estates != 0.0:
+        ms = int(np.argmin(np.abs(1.0 - np.cumsum(evals) - estates))) + 1
+    else:
+        raise RuntimeError("Mdmrg=Edmrg=0 can not fix a threshold for the RDM")
+    e_ = evals[ms - 1]
+    j_ = ms
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i in range(j_, len(evals)):
+            err = abs(e_ - evals[i]) / e_
+            if err <= deg_thr:
+                ms += 1
+    return ms
+
+
+def renormalize(b: Block, rho: RhoBlocks, mstates: int, deg_thr: float,
+                estates: float = 0.0):
+    """renormalize_block + operators_renormalization, src/DMRG_RDM.f90:172-404.
+
+    Returns (new block, m_s, truncation error, kept-state QN list).
+    """
+    ms = kept_count(rho.evalues, mstates, b.dim, deg_thr, estates)
+    trunc = 1.0 - float(np.sum(rho.evalues[:ms]))
+    u = np.zeros((b.dim, ms))
+    newqn = np.zeros((ms, b.qn.shape[1]))
+    for im in range(ms):                       # rho%sparse(m_left,m_s), MATRIX_BLOCKS:556-587
+        ib, pos = rho.eorder[im]
+        u[rho.maps[ib], im] = rho.evecs_blk[ib][:, pos]
+        newqn[im] = rho.qns[ib]
+    ops = {k: u.T @ o @ u for k, o in b.ops.items()}
+    return Block(b.kind, b.length, newqn, ops, dict(b.optype)), ms, trunc, newqn
+
+
+def entropy(evals: np.ndarray) -> float:
+    """write_entanglement S = -sum l log(l+1e-32), src/DMRG_MAIN.f90:501-521."""
+    ev = np.where(evals < 0.0, 0.0, evals)
+    return float(-np.sum(ev * np.log(ev + EPS_LOG)))
+
+
+# ---------------------------------------------------------------------------
+# DMRG driver (src/DMRG_MAIN.f90)
+# ---------------------------------------------------------------------------
+@dataclass
+class Params:
+    ldmrg: int
+    mdmrg: int
+    target_qn: tuple
+    hij: list
+    msweep: list = field(default_factory=list)
+    neigen: int = 1
+    lanc_dim_threshold: int = 128
+    lanc_tol: float = 1e-12
+    deg_thr: float = 0.1
+    seed: int = 0
+    norb: int = 1
+    ncv_max: int = 600
+
+
+@dataclass
+class StepRecord:
+    kind: str
+    left_len: int
+    right_len: int
+    L: int
+    energies: np.ndarray
+    e_per_site: np.ndarray
+    entropy_left: float
+    entropy_right: float
+    m_left: int
+    m_right: int
+    trunc_left: float
+    trunc_right: float
+    kept_left: dict
+    kept_right: dict
+    m_sb: int
+    nsect: int
+    nop: int
+    hv_flops: float
+
+
+class DMRG:
+    """init_dmrg / run_DMRG / step_dmrg, src/DMRG_MAIN.f90:22-433 (OBC)."""
+
+    def __init__(self, dot: Site, p: Params):
+        self.dot, self.p = dot, p
+        self.left = block_from_site(dot)
+        self.right = block_from_site(dot)
+        self.records: list[StepRecord] = []
+        self.last_sb = None
+        self.last_psi = None
+
+    def step(self, kind: str, label: int = 0, mstates: int = None) -> StepRecord:
+        p = self.p
+        ms = p.mdmrg if mstates is None else mstates
+        self.left = enlarge_block(self.left, self.dot, "left", p.hij)
+        self.right = enlarge_block(self.right, self.dot, "right", p.hij)
+        L = self.left.length + self.right.length
+        target = current_target(L, p.target_qn, p.norb)
+        sb = sb_get_states(self.left, self.right, target)
+        setup_direct(sb, self.left, self.right, p.hij)
+        msb = sb.n
+        neig = min(msb, p.neigen)
+        nop = 0
+        if neig == msb or msb <= p.lanc_dim_threshold:     # sb_diag, SUPERBLOCK.f90:367-369
+            e, u = np.linalg.eigh(sb_dense(sb))
+            evals, evecs = e[:neig], u[:, :neig]
+        else:
+            v0 = splitmix64_uniform_fast(msb, p.seed)
+            evals, evecs, nop = lanczos_lowest(lambda x: hv_direct(sb, x), msb, neig, v0,
+                                               p.lanc_tol, p.ncv_max)
+        psi = evecs[:, 0]
+        self.last_sb, self.last_psi = sb, psi
+        rl = build_rho(sb, psi, "left")
+        rr = build_rho(sb, psi, "right")
+        if kind == "i":
+            do_ren = self.left.length != p.ldmrg
+        else:
+            do_ren = not (self.left.length == p.ldmrg + 1 and label == 1)
+        ml, mr, tl, tr = self.left.dim, self.right.dim, 0.0, 0.0
+        kl, kr = {}, {}
+        if do_ren:
+            self.left, ml, tl, ql = renormalize(self.left, rl, ms, p.deg_thr)
+            self.right, mr, tr, qr = renormalize(self.right, rr, ms, p.deg_thr)
+            for q in map(tuple, ql):
+                kl[q] = kl.get(q, 0) + 1
+            for q in map(tuple, qr):
+                kr[q] = kr.get(q, 0) + 1
+        rec = StepRecord(kind, self.left.length, self.right.length, L, evals,
+                         evals / L / p.norb, entropy(rl.evalues), entropy(rr.evalues),
+                         ml, mr, tl, tr, kl, kr, msb, len(sb.qns), nop, hv_flops(sb))
+        self.records.append(rec)
+        return rec
+
+    def infinite(self):
+        """infinite_DMRG, src/DMRG_MAIN.f90:140-159 (OBC)."""
+        while self.left.length < self.p.ldmrg:
+            self.step("i")
+
+    def finite(self):
+        """finite_DMRG, src/DMRG_MAIN.f90:168-241 (OBC only; PBC refused :179-180)."""
+        p = self.p
+        fn = 2 * p.ldmrg
+        bl = {1: {1: self.left.copy()}, 2: {1: self.right.copy()}}
+        while self.left.length < p.ldmrg:
+            self.step("i")
+            bl[1][self.left.length] = self.left
+            bl[2][self.right.length] = self.right
+        for im, m in enumerate(p.msweep):
+            while self.left.length < fn:
+                self.right = bl[2][fn - self.left.length]
+                self.step("f", 1, m)
+                bl[1][self.left.length] = self.left
+            while self.right.length < fn:
+                self.left = bl[1][fn - self.right.length]
+                self.step("f", 2, m)
+                bl[2][self.right.length] = self.right
+            while self.left.length <= p.ldmrg:
+                self.right = bl[2][fn - self.left.length]
+                self.step("f", 1, m)
+                bl[1][self.left.length] = self.left
+
+
+# ---------------------------------------------------------------------------
+# Brute-force exact diagonalisation of full chains (pinning only)
+# ---------------------------------------------------------------------------
+def chain_hamiltonian(dot: Site, L: int, hij) -> tuple[np.ndarray, np.ndarray]:
+    """Full-chain H (OBC) and per-state QN, built site by site with Kronecker
+    products in the left-block order; independent of the superblock code."""
+    b = block_from_site(dot)
+    for _ in range(L - 1):
+        b = enlarge_block(b, dot, "left", hij)
+    return b.ops["H"], b.qn
+
+
+def ed_ground(dot: Site, L: int, hij, target: tuple, k: int = 1) -> np.ndarray:
+    h, qn = chain_hamiltonian(dot, L, hij)
+    sel = np.all(np.isclose(qn, np.array(target)[None, :]), axis=1)
+    hs = h[np.ix_(sel, sel)]
+    return np.linalg.eigvalsh(hs)[:k]
+
+
+def free_fermion_obc(L: int, nup: int, ndw: int, t: float = 1.0) -> float:
+    """OBC tight-binding ground energy: sum of the lowest single-particle levels."""
+    eps = np.sort(-2.0 * t * np.cos(np.pi * np.arange(1, L + 1) / (L + 1)))
+    return float(eps[:nup].sum() + eps[:ndw].sum())

@@ -1,0 +1,234 @@
+"""Pins the CPU oracle (oracle/dmrg_oracle.py) before anything is checked against it.
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign, wrong
+transpose, wrong sector bookkeeping) fails at least one of them:
+  * the reference's golden .check trajectories (tests/golden, copied data);
+  * closed forms: 4-site Heisenberg, free fermions (U=0), 2-site Hubbard;
+  * brute-force exact diagonalisation of full chains;
+  * an independent dense Kronecker construction of H_sb;
+  * library eigen-solvers (LAPACK eigh, ARPACK eigsh) for the Lanczos restatement;
+  * invariants of rho.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from oracle import dmrg_oracle as O
+
+SPIN_HIJ = [[1.0, 0.0], [0.0, 0.5]]          # diag([Jp, Jx/2]), spinSU2_1d.f90:35
+HUB_HIJ = [[-1.0, 0.0], [0.0, -1.0]]         # diag(ts), hm_1d.f90 (TS=-1)
+
+
+def _load(golden_dir, name):
+    return np.loadtxt(os.path.join(golden_dir, name))
+
+
+# --------------------------------------------------------------------------- RNG
+def test_splitmix64_known_answer():
+    # SplitMix64(seed=0) first raw output is 0xE220A8397B1DCDAF (published KAT).
+    raw = 0xE220A8397B1DCDAF
+    expect = (raw >> 11) * 2.0 ** -53 * 2.0 - 1.0
+    assert O.splitmix64_uniform(1, 0)[0] == expect
+    a = O.splitmix64_uniform(257, 12345)
+    b = O.splitmix64_uniform_fast(257, 12345)
+    assert np.array_equal(a, b)
+    assert a.min() >= -1.0 and a.max() < 1.0
+
+
+# ------------------------------------------------------------------ golden files
+@pytest.fixture(scope="module")
+def spin_run():
+    p = O.Params(ldmrg=40, mdmrg=20, target_qn=(0.0,), hij=SPIN_HIJ, neigen=2,
+                 lanc_dim_threshold=128)
+    d = O.DMRG(O.spin_site(2), p)
+    d.infinite()
+    return d
+
+
+def test_spin_golden_energy_trajectory(spin_run, golden_dir):
+    ref = _load(golden_dir, "spin12_L40_M20_energy.check")
+    assert len(spin_run.records) == len(ref) == 39
+    for r, row in zip(spin_run.records, ref):
+        assert r.left_len == int(row[0])
+        assert abs(r.e_per_site[0] - row[1]) < 1e-12
+        assert abs(r.e_per_site[1] - row[2]) < 1e-12
+    fin = _load(golden_dir, "spin12_L40_M20_final_energy.check")
+    assert abs(spin_run.records[-1].e_per_site[0] - fin[1]) < 1e-12
+
+
+def test_spin_golden_entropy_trajectory(spin_run, golden_dir):
+    ref = _load(golden_dir, "spin12_L40_M20_Sentropy.check")
+    # Rows 2..39.  The file's last row repeats row 39 (written by an older code
+    # version that reused the previous step's rho spectrum); the current code
+    # recomputes rho every step (src/DMRG_RDM.f90:73-88), so row 40 is skipped.
+    assert ref[-1] == ref[-2]
+    for r, s in zip(spin_run.records[:-1], ref[:-1]):
+        assert abs(r.entropy_left - s) < 1e-10
+
+
+def test_spin_closed_form_4site(spin_run):
+    e4 = -(0.75 + math.sqrt(3.0) / 2.0) / 4.0
+    assert abs(spin_run.records[0].e_per_site[0] - e4) < 1e-14
+
+
+def test_hubbard_golden_trajectory(golden_dir):
+    ref = _load(golden_dir, "hubbard_U0_L40_M40_energy.check")
+    refs = _load(golden_dir, "hubbard_U0_L40_M40_Sentropy.check")
+    p = O.Params(ldmrg=40, mdmrg=40, target_qn=(0.5, 0.5), hij=HUB_HIJ, neigen=1,
+                 lanc_dim_threshold=128)
+    d = O.DMRG(O.electron_site(0.0, True), p)
+    d.infinite()
+    assert len(d.records) == 39
+    for r, e in zip(d.records, ref):
+        assert abs(r.e_per_site[0] - e) < 1e-12
+    for r, s in zip(d.records[:-1], refs[:-1]):
+        assert abs(r.entropy_left - s) < 1e-9
+    # first row: 4-site free fermions at half filling, -sqrt(5)/2 per site
+    assert abs(d.records[0].e_per_site[0] + math.sqrt(5.0) / 2.0) < 1e-14
+    # U=0: the untruncated steps (L=4, 6) equal the tight-binding value
+    for r in d.records[:2]:
+        ex = O.free_fermion_obc(r.L, r.L // 2, r.L // 2) / r.L
+        assert abs(r.e_per_site[0] - ex) < 1e-12
+
+
+# ------------------------------------------------------------------ exact checks
+@pytest.mark.parametrize("sun,L", [(2, 10), (3, 6)])
+def test_spin_untruncated_equals_ed(sun, L):
+    p = O.Params(ldmrg=L // 2, mdmrg=4096, target_qn=(0.0,), hij=SPIN_HIJ, neigen=2,
+                 lanc_dim_threshold=64)
+    d = O.DMRG(O.spin_site(sun), p)
+    d.infinite()
+    for r in d.records:
+        ed = O.ed_ground(O.spin_site(sun), r.L, SPIN_HIJ, (0.0,), k=2)
+        assert np.allclose(r.energies, ed, atol=1e-10, rtol=0)
+
+
+@pytest.mark.parametrize("U,hf", [(4.0, True), (2.0, False)])
+def test_hubbard_untruncated_equals_ed(U, hf):
+    site = O.electron_site(U, hf)
+    p = O.Params(ldmrg=3, mdmrg=4096, target_qn=(0.5, 0.5), hij=HUB_HIJ, neigen=1,
+                 lanc_dim_threshold=64)
+    d = O.DMRG(site, p)
+    d.infinite()
+    for r in d.records:
+        ed = O.ed_ground(site, r.L, HUB_HIJ, (r.L // 2, r.L // 2))
+        assert abs(r.energies[0] - ed[0]) < 1e-10
+
+
+def test_hubbard_two_site_closed_form():
+    # E = (U - sqrt(U^2+16 t^2))/2 for 2 sites, 1 up + 1 down; HF shift -U/2*N.
+    U = 3.0
+    ed = O.ed_ground(O.electron_site(U, True), 2, HUB_HIJ, (1.0, 1.0))[0]
+    assert abs(ed - ((U - math.sqrt(U * U + 16.0)) / 2.0 - U)) < 1e-12
+
+
+def test_finite_sweeps_reach_ed_when_untruncated():
+    # Ldmrg=4 -> sweeps act on 2*Ldmrg+2 = 10 sites; M large -> exact.
+    p = O.Params(ldmrg=4, mdmrg=4096, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1,
+                 lanc_dim_threshold=64, msweep=[4096])
+    d = O.DMRG(O.spin_site(2), p)
+    d.finite()
+    ed10 = O.ed_ground(O.spin_site(2), 10, SPIN_HIJ, (0.0,))[0]
+    fin = [r for r in d.records if r.kind == "f"]
+    assert fin and all(r.L == 10 for r in fin)
+    for r in fin:
+        assert abs(r.energies[0] - ed10) < 1e-10
+
+
+# ------------------------------------------------------- superblock H, H*v
+def _sb_after(dot, hij, steps, M, target_qn):
+    p = O.Params(ldmrg=steps + 1, mdmrg=M, target_qn=target_qn, hij=hij, neigen=1,
+                 lanc_dim_threshold=64)
+    d = O.DMRG(dot, p)
+    for _ in range(steps):
+        d.step("i")
+    left = O.enlarge_block(d.left, dot, "left", hij)
+    right = O.enlarge_block(d.right, dot, "right", hij)
+    tgt = O.current_target(left.length + right.length, target_qn)
+    sb = O.sb_get_states(left, right, tgt)
+    O.setup_direct(sb, left, right, hij)
+    return sb, left, right
+
+
+@pytest.mark.parametrize("model", ["spin", "spin1", "hubbard"])
+def test_direct_hv_equals_kronecker_restriction(model):
+    if model == "spin":
+        dot, hij, tq = O.spin_site(2), SPIN_HIJ, (0.0,)
+    elif model == "spin1":
+        dot, hij, tq = O.spin_site(3), SPIN_HIJ, (0.0,)
+    else:
+        dot, hij, tq = O.electron_site(2.5, True), HUB_HIJ, (0.5, 0.5)
+    sb, left, right = _sb_after(dot, hij, 2, 6, tq)
+    # independent construction: full left x right Hamiltonian, then restrict
+    hfull = (np.kron(left.ops["H"], np.eye(right.dim)) + np.kron(np.eye(left.dim), right.ops["H"])
+             + O._connect(left.ops, right.ops, left.kind, hij))
+    states = np.concatenate([(lm[:, None] * right.dim + rm[None, :]).ravel()
+                             for lm, rm in zip(sb.lmap, sb.rmap)])
+    href = hfull[np.ix_(states, states)]
+    hd = O.sb_dense(sb)
+    assert np.allclose(hd, href, atol=1e-13, rtol=0)
+    assert np.allclose(hd, hd.T, atol=1e-13)
+    # loop twin follows the reference loops exactly
+    v = O.splitmix64_uniform_fast(sb.n, 7)
+    assert np.allclose(O.hv_direct_loops(sb, v), O.hv_direct(sb, v), atol=1e-12)
+    # no leakage: the restricted H equals the full H on SB states
+    x = np.zeros(left.dim * right.dim)
+    x[states] = v
+    assert np.allclose((hfull @ x)[states], O.hv_direct(sb, v), atol=1e-12)
+
+
+def test_lanczos_matches_library_solvers():
+    sb, _, _ = _sb_after(O.spin_site(2), SPIN_HIJ, 4, 16, (0.0,))
+    n = sb.n
+    v0 = O.splitmix64_uniform_fast(n, 0)
+    ev, x, nop = O.lanczos_lowest(lambda y: O.hv_direct(sb, y), n, 2, v0)
+    hd = O.sb_dense(sb)
+    ex = np.linalg.eigvalsh(hd)[:2]
+    assert np.allclose(ev, ex, atol=1e-11)
+    op = spla.LinearOperator((n, n), matvec=lambda y: O.hv_direct(sb, y), dtype=float)
+    ea = np.sort(spla.eigsh(op, k=2, which="SA", tol=1e-12, v0=v0)[0])
+    assert np.allclose(ev, ea, atol=1e-11)
+    res = np.linalg.norm(hd @ x[:, 0] - ev[0] * x[:, 0])
+    assert res < 1e-9
+
+
+# ------------------------------------------------------------------ rho / truncation
+def test_rho_invariants():
+    sb, _, _ = _sb_after(O.electron_site(4.0, True), HUB_HIJ, 2, 16, (0.5, 0.5))
+    hd = O.sb_dense(sb)
+    e, u = np.linalg.eigh(hd)
+    psi = u[:, 0]
+    rl = O.build_rho(sb, psi, "left")
+    rr = O.build_rho(sb, psi, "right")
+    assert abs(rl.evalues.sum() - 1.0) < 1e-12 and abs(rr.evalues.sum() - 1.0) < 1e-12
+    nz = min(len(rl.evalues), len(rr.evalues))
+    assert np.allclose(rl.evalues[:nz], rr.evalues[:nz], atol=1e-12)
+    # S from Schmidt values of each sector block equals S from rho
+    sv = np.concatenate([np.linalg.svd(psi[o:o + a * b].reshape(a, b), compute_uv=False)
+                         for o, a, b in zip(sb.offset, sb.Dl, sb.Dr)])
+    lam = sv ** 2
+    s_svd = float(-np.sum(lam * np.log(lam + 1e-32)))
+    assert abs(O.entropy(rl.evalues) - s_svd) < 1e-12
+    assert np.all(np.diff(rl.evalues) <= 0.0)
+
+
+def test_product_state_has_zero_entropy():
+    ev = np.array([1.0, 0.0, 0.0])
+    assert O.entropy(ev) == 0.0
+    bell = np.array([0.5, 0.5])
+    assert abs(O.entropy(bell) - math.log(2.0)) < 1e-15
+
+
+def test_kept_count_rule():
+    ev = np.array([0.4, 0.3, 0.1, 0.095, 0.091, 0.05, 0.0])
+    assert O.kept_count(ev, 3, 10, 0.1) == 5          # 0.095,0.091 within 10% of 0.1
+    assert O.kept_count(ev, 3, 10, 0.0) == 3
+    assert O.kept_count(ev, 2, 10, 0.1) == 2          # 0.1 is 66% below 0.3
+    assert O.kept_count(ev, 20, 4, 0.1) == 5          # m_s=min(M,mdim)=4, 0.091 kept
+    tied = np.array([0.5, 0.25, 0.25, 0.0])
+    assert O.kept_count(tied, 2, 4, 0.1) == 3         # exact ties are never split
+    # truncation error threshold mode (Estates): argmin |1-cumsum-E|
+    assert O.kept_count(np.array([0.9, 0.09, 0.01]), 0, 3, 0.0, estates=0.01) == 2
