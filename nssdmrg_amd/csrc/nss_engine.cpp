@@ -1,0 +1,961 @@
+// Device-resident DMRG engine (init_dmrg / run_DMRG / step_dmrg,
+// src/DMRG_MAIN.f90:22-433).  Every block operator lives in HBM as a list of
+// dense QN blocks; enlargement, the superblock product, rho and the
+// truncation rotation are GPU launches.  The host keeps only QN
+// bookkeeping, the Lanczos projected matrix and the rho spectra.
+//
+// Basis order.  A block's basis is sector-contiguous.  The enlarged basis of
+// block (x) site groups the pairs (block sector a, site state s) by total QN
+// in first-appearance order over (a outer, s inner); each pair contributes
+// the D_a states of sector a in their order.  This is a permutation of the
+// reference's Kronecker order within each QN sector
+// (src/LIST/QN_TUPLE_BASIS.f90:371-398), so spectra, energies, entropies and
+// kept-state counts per QN are unchanged.
+//
+// QNs are stored as integers scaled by 2 (spin Sz = +-1/2 -> +-1).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <numeric>
+
+#include "nss_internal.h"
+
+namespace nss {
+
+struct QN {
+  int a = 0, b = 0;
+  bool operator==(const QN& o) const { return a == o.a && b == o.b; }
+  bool operator<(const QN& o) const { return a < o.a || (a == o.a && b < o.b); }
+  QN operator+(const QN& o) const { return {a + o.a, b + o.b}; }
+  QN operator-(const QN& o) const { return {a - o.a, b - o.b}; }
+};
+
+struct Sector {
+  QN q;
+  int dim = 0;
+};
+
+struct OpBlock {
+  int rs, cs;
+  size_t off;
+};
+
+struct BlockOp {
+  QN dq;  // row QN - col QN
+  std::vector<OpBlock> blks;
+  std::vector<int> by_row;  // row sector -> block index or -1
+  DevArr<double> data;
+  DevArr<float> data32;
+  const OpBlock* row(int rs) const {
+    if (rs < 0 || rs >= (int)by_row.size() || by_row[rs] < 0) return nullptr;
+    return &blks[by_row[rs]];
+  }
+};
+using OpPtr = std::shared_ptr<BlockOp>;
+
+struct Block {
+  int length = 0;
+  std::vector<Sector> sec;
+  std::map<std::string, OpPtr> ops;
+  int dim() const {
+    int d = 0;
+    for (auto& s : sec) d += s.dim;
+    return d;
+  }
+  int find(QN q) const {
+    for (size_t i = 0; i < sec.size(); ++i)
+      if (sec[i].q == q) return (int)i;
+    return -1;
+  }
+};
+
+struct SiteOp {
+  QN dq;
+  std::vector<double> m;  // dim x dim row-major
+};
+
+struct Site {
+  bool fermion = false;
+  int dim = 0;
+  std::vector<QN> qn;
+  std::map<std::string, SiteOp> ops;
+};
+
+static int fsign(const QN& q) { return (((q.a + q.b) / 2) & 1) ? -1 : 1; }  // (-1)^(Nup+Ndw)
+
+// -------------------------------------------------------------- sites
+// spin_site, src/BLOCKS/SITES.f90:281-344; electron_site :358-434 with the
+// local Fock space of src/FOCK/HLOCAL.f90 (Norb=1).
+static Site make_site(const nss_model& m) {
+  Site s;
+  if (m.model == NSS_MODEL_SPIN12 || m.model == NSS_MODEL_SPIN1) {
+    const bool one = m.model == NSS_MODEL_SPIN1;
+    s.dim = one ? 3 : 2;
+    const int d = s.dim;
+    std::vector<double> H(d * d, 0.0), Sz(d * d, 0.0), Sp(d * d, 0.0);
+    if (!one) {
+      Sz[0] = 0.5; Sz[3] = -0.5;
+      Sp[0 * 2 + 1] = 1.0;
+      // H = hx*pauli_x + hz*pauli_z
+      H[0 * 2 + 1] = H[1 * 2 + 0] = m.hvec[0];
+      H[0] = m.hvec[1]; H[3] = -m.hvec[1];
+      s.qn = {{1, 0}, {-1, 0}};
+    } else {
+      Sz[0] = 1.0; Sz[4] = 0.0; Sz[8] = -1.0;
+      Sp[0 * 3 + 1] = std::sqrt(2.0);
+      Sp[1 * 3 + 2] = std::sqrt(2.0);
+      const double r = 1.0 / std::sqrt(2.0);
+      // H = hx*Sx + hz*Sz, Sx = [[0,1,0],[1,0,1],[0,1,0]]/sqrt(2)
+      H[0 * 3 + 1] = H[1 * 3 + 0] = H[1 * 3 + 2] = H[2 * 3 + 1] = m.hvec[0] * r;
+      H[0] += m.hvec[1]; H[8] -= m.hvec[1];
+      s.qn = {{2, 0}, {0, 0}, {-2, 0}};
+    }
+    s.ops["H"] = {{0, 0}, H};
+    s.ops["Sz"] = {{0, 0}, Sz};
+    s.ops["Sp"] = {{2, 0}, Sp};
+  } else if (m.model == NSS_MODEL_HUBBARD) {
+    s.fermion = true;
+    s.dim = 4;
+    std::vector<double> H(16, 0.0), Cu(16, 0.0), Cd(16, 0.0);
+    for (int st = 0; st < 4; ++st) {
+      const int nup = st & 1, ndw = (st >> 1) & 1;
+      s.qn.push_back({2 * nup, 2 * ndw});
+      double e = m.eloc[0] * nup + m.eloc[1] * ndw + m.uloc * nup * ndw;
+      if (m.hfmode) e -= 0.5 * m.uloc * (nup + ndw);
+      H[st * 4 + st] = e;
+      if (nup) Cu[(st & ~1) * 4 + st] = 1.0;
+      if (ndw) Cd[(st & ~2) * 4 + st] = nup ? -1.0 : 1.0;
+    }
+    s.ops["H"] = {{0, 0}, H};
+    s.ops["Cup"] = {{-2, 0}, Cu};
+    s.ops["Cdn"] = {{0, -2}, Cd};
+  } else {
+    throw Error(NSS_E_ARG, "unknown model");
+  }
+  return s;
+}
+
+static std::vector<double> transpose(const std::vector<double>& a, int d) {
+  std::vector<double> t(d * d);
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) t[j * d + i] = a[i * d + j];
+  return t;
+}
+
+// -------------------------------------------------------------- engine
+struct Step {
+  nss_step_info info{};
+  // kept sectors: QN and the rho eigenvalues of their kept states
+  std::vector<std::pair<QN, std::vector<double>>> kept_left, kept_right;
+};
+
+struct Engine {
+  nss_ctx* ctx;
+  nss_model model;
+  nss_params par;
+  std::vector<int32_t> msweep;
+  std::vector<double> esweep;
+  Site site;
+  Block left, right;
+  std::vector<Step> steps;
+  std::vector<double> hij;  // 2x2
+  // last superblock (for export)
+  Block lastL, lastR;
+  std::vector<int> sbL, sbR;
+  std::vector<SbTerm> lastTerms;
+  DevArr<double> psi;
+  bool want32 = false;
+
+  // ------------------------------------------------------------ blocks
+  OpPtr new_op(QN dq, const std::vector<Sector>& rsec, const std::vector<Sector>& csec) {
+    auto op = std::make_shared<BlockOp>();
+    op->dq = dq;
+    op->by_row.assign(rsec.size(), -1);
+    size_t off = 0;
+    for (size_t r = 0; r < rsec.size(); ++r) {
+      const QN qc = rsec[r].q - dq;
+      int c = -1;
+      for (size_t k = 0; k < csec.size(); ++k)
+        if (csec[k].q == qc) { c = (int)k; break; }
+      if (c < 0) continue;
+      op->by_row[r] = (int)op->blks.size();
+      op->blks.push_back({(int)r, c, off});
+      off += (size_t)rsec[r].dim * csec[c].dim;
+    }
+    op->data.resize(ctx->pool, std::max<size_t>(1, off));
+    return op;
+  }
+
+  void make_fp32(BlockOp& op) {
+    if (!want32) return;
+    op.data32.resize(ctx->pool, op.data.n);
+    launch_d2f(op.data.p, op.data32.p, (int64_t)op.data.n, ctx->stream);
+  }
+
+  Block block_from_site() {
+    Block b;
+    b.length = 1;
+    std::vector<int> state_sec(site.dim), state_pos(site.dim);
+    for (int s = 0; s < site.dim; ++s) {
+      int k = b.find(site.qn[s]);
+      if (k < 0) {
+        b.sec.push_back({site.qn[s], 0});
+        k = (int)b.sec.size() - 1;
+      }
+      state_sec[s] = k;
+      state_pos[s] = b.sec[k].dim++;
+    }
+    for (auto& kv : site.ops) {
+      OpPtr op = new_op(kv.second.dq, b.sec, b.sec);
+      std::vector<double> h(op->data.n, 0.0);
+      for (int i = 0; i < site.dim; ++i)
+        for (int j = 0; j < site.dim; ++j) {
+          const double v = kv.second.m[i * site.dim + j];
+          if (v == 0.0) continue;
+          const OpBlock* ob = op->row(state_sec[i]);
+          NSS_REQUIRE(ob && ob->cs == state_sec[j], NSS_E_STATE, "site operator breaks its QN shift");
+          h[ob->off + (size_t)state_pos[i] * b.sec[ob->cs].dim + state_pos[j]] = v;
+        }
+      NSS_HIP(hipMemcpyAsync(op->data.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+      NSS_HIP(hipStreamSynchronize(ctx->stream));
+      make_fp32(*op);
+      b.ops[kv.first] = op;
+    }
+    return b;
+  }
+
+  // X (x) Y term of an enlarged operator
+  struct KTerm {
+    const BlockOp* x;  // nullptr -> identity
+    int xtrans;
+    int sign_row, sign_col;  // multiply by P_b sign of the row / col block sector
+    std::vector<double> y;   // site matrix (dim x dim)
+    QN dq;                   // shift of the whole term
+    double coef;
+  };
+
+  // enlarge_block, src/DMRG_CONNECT.f90:18-291 (OBC link 'n').
+  Block enlarge(const Block& b, bool is_left) {
+    Block e;
+    e.length = b.length + 1;
+    struct Piece { int a, s, off; };
+    std::vector<std::vector<Piece>> pieces;
+    for (size_t a = 0; a < b.sec.size(); ++a)
+      for (int s = 0; s < site.dim; ++s) {
+        const QN q = b.sec[a].q + site.qn[s];
+        int k = e.find(q);
+        if (k < 0) {
+          e.sec.push_back({q, 0});
+          pieces.emplace_back();
+          k = (int)e.sec.size() - 1;
+        }
+        pieces[k].push_back({(int)a, s, e.sec[k].dim});
+        e.sec[k].dim += b.sec[a].dim;
+      }
+    const int d = site.dim;
+    auto sop = [&](const char* k) -> const SiteOp& { return site.ops.at(k); };
+    std::vector<double> I(d * d, 0.0);
+    for (int i = 0; i < d; ++i) I[i * d + i] = 1.0;
+    std::vector<double> Pd(d * d, 0.0);
+    for (int i = 0; i < d; ++i) Pd[i * d + i] = fsign(site.qn[i]);
+    auto mul = [&](const std::vector<double>& A, const std::vector<double>& B) {
+      std::vector<double> C(d * d, 0.0);
+      for (int i = 0; i < d; ++i)
+        for (int k = 0; k < d; ++k)
+          for (int j = 0; j < d; ++j) C[i * d + j] += A[i * d + k] * B[k * d + j];
+      return C;
+    };
+    const QN z{0, 0};
+    std::map<std::string, std::vector<KTerm>> defs;
+    std::vector<KTerm>& H = defs["H"];
+    H.push_back({b.ops.at("H").get(), 0, 0, 0, I, z, 1.0});
+    H.push_back({nullptr, 0, 0, 0, sop("H").m, z, 1.0});
+    if (!site.fermion) {
+      const BlockOp* Sz = b.ops.at("Sz").get();
+      const BlockOp* Sp = b.ops.at("Sp").get();
+      const double jz = hij[0], jxy = hij[3];
+      const QN up{2, 0}, dn{-2, 0};
+      // connect_spin_blocks :396-454 (symmetric under left <-> right)
+      H.push_back({Sz, 0, 0, 0, sop("Sz").m, z, jz});
+      H.push_back({Sp, 0, 0, 0, transpose(sop("Sp").m, d), z, jxy});
+      H.push_back({Sp, 1, 0, 0, sop("Sp").m, z, jxy});
+      (void)up; (void)dn;
+      defs["Sz"].push_back({nullptr, 0, 0, 0, sop("Sz").m, z, 1.0});
+      defs["Sp"].push_back({nullptr, 0, 0, 0, sop("Sp").m, sop("Sp").dq, 1.0});
+    } else {
+      const char* names[2] = {"Cup", "Cdn"};
+      for (int sp = 0; sp < 2; ++sp) {
+        const double t = hij[sp * 2 + sp];
+        const BlockOp* C = b.ops.at(names[sp]).get();
+        const std::vector<double>& Cs = sop(names[sp]).m;
+        if (t != 0.0) {
+          if (is_left) {
+            // (C_b^T P_b) x C_s  +  (P_b C_b) x C_s^T      (connect(self,dot))
+            H.push_back({C, 1, 0, 1, Cs, z, t});
+            H.push_back({C, 0, 1, 0, transpose(Cs, d), z, t});
+          } else {
+            // C_b x (C_s^T P_s)  +  C_b^T x (P_s C_s)      (connect(dot,self))
+            H.push_back({C, 0, 0, 0, mul(transpose(Cs, d), Pd), z, t});
+            H.push_back({C, 1, 0, 0, mul(Pd, Cs), z, t});
+          }
+        }
+        // fermionic edge operator: left P_b x C_s, right C_s x 1_b
+        defs[names[sp]].push_back({nullptr, 0, is_left ? 1 : 0, 0, Cs, sop(names[sp]).dq, 1.0});
+      }
+    }
+    // tasks
+    std::vector<KronTask> tasks;
+    std::vector<KronContrib> contribs;
+    for (auto& kv : defs) {
+      QN dq = kv.first == "H" ? z : site.ops.at(kv.first).dq;
+      OpPtr op = new_op(dq, e.sec, e.sec);
+      for (const OpBlock& ob : op->blks) {
+        const int ldd = e.sec[ob.cs].dim;
+        for (const Piece& p : pieces[ob.rs])
+          for (const Piece& pc : pieces[ob.cs]) {
+            const int c0 = (int)contribs.size();
+            const int Da = b.sec[p.a].dim, Dc = b.sec[pc.a].dim;
+            for (const KTerm& t : kv.second) {
+              const double yv = t.y[p.s * d + pc.s];
+              if (yv == 0.0) continue;
+              KronContrib c{};
+              c.scale = t.coef * yv;
+              if (t.sign_row) c.scale *= fsign(b.sec[p.a].q);
+              if (t.sign_col) c.scale *= fsign(b.sec[pc.a].q);
+              if (t.x == nullptr) {
+                if (p.a != pc.a) continue;
+                c.x = nullptr;
+              } else if (!t.xtrans) {
+                const OpBlock* xb = t.x->row(p.a);
+                if (!xb || xb->cs != pc.a) continue;
+                c.x = t.x->data.p + xb->off;
+                c.ldx = Dc;
+                c.trans = 0;
+              } else {
+                const OpBlock* xb = t.x->row(pc.a);
+                if (!xb || xb->cs != p.a) continue;
+                c.x = t.x->data.p + xb->off;
+                c.ldx = Da;  // stored (pc.a x p.a) row-major: element (r,c) at [c*Da + r]
+                c.trans = 1;
+              }
+              contribs.push_back(c);
+            }
+            const int nc = (int)contribs.size() - c0;
+            if (nc == 0) continue;  // stays zero
+            double* dst = op->data.p + ob.off + (size_t)p.off * ldd + pc.off;
+            for (int r = 0; r < Da; r += KRON_TILE)
+              for (int cc = 0; cc < Dc; cc += KRON_TILE)
+                tasks.push_back({dst, ldd, Da, Dc, c0, nc, r, cc});
+          }
+      }
+      NSS_HIP(hipMemsetAsync(op->data.p, 0, op->data.n * sizeof(double), ctx->stream));
+      e.ops[kv.first] = op;
+    }
+    DevArr<KronTask> dt;
+    DevArr<KronContrib> dc;
+    dt.resize(ctx->pool, std::max<size_t>(1, tasks.size()));
+    dc.resize(ctx->pool, std::max<size_t>(1, contribs.size()));
+    if (!tasks.empty()) {
+      NSS_HIP(hipMemcpyAsync(dt.p, tasks.data(), tasks.size() * sizeof(KronTask), hipMemcpyHostToDevice,
+                             ctx->stream));
+      NSS_HIP(hipMemcpyAsync(dc.p, contribs.data(), contribs.size() * sizeof(KronContrib),
+                             hipMemcpyHostToDevice, ctx->stream));
+      launch_kron(dt.p, (int)tasks.size(), dc.p, ctx->stream);
+    }
+    for (auto& kv : e.ops) make_fp32(*kv.second);
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return e;
+  }
+
+  // ------------------------------------------------------------ superblock
+  // sb_get_states + Setup_SuperBlock_{Spin,Fermion}_Direct
+  void setup_superblock(const Block& L, const Block& R, QN target) {
+    sbL.clear();
+    sbR.clear();
+    std::vector<int> Dl, Dr;
+    for (size_t k = 0; k < L.sec.size(); ++k) {
+      const int r = R.find(target - L.sec[k].q);
+      if (r < 0) continue;
+      sbL.push_back((int)k);
+      sbR.push_back(r);
+      Dl.push_back(L.sec[k].dim);
+      Dr.push_back(R.sec[r].dim);
+    }
+    NSS_REQUIRE(!sbL.empty(), NSS_E_STATE, "sb_get_states ERROR: total_states=0. There are no SB states.");
+    const int nq = (int)sbL.size();
+    sb_set_layout(ctx->sb, Dl, Dr, nullptr);
+    std::map<QN, int> index;
+    for (int k = 0; k < nq; ++k) index[L.sec[sbL[k]].q] = k;
+    auto ref = [&](const BlockOp& op, const OpBlock& ob, int ld, int trans, double alpha) {
+      OpRef r;
+      r.p = op.data.p + ob.off;
+      r.p32 = want32 ? (const void*)(op.data32.p + ob.off) : nullptr;
+      r.ld = ld;
+      r.trans = trans;
+      r.alpha = alpha;
+      return r;
+    };
+    std::vector<OpRef> HL(nq), HR(nq);
+    const BlockOp& hl = *L.ops.at("H");
+    const BlockOp& hr = *R.ops.at("H");
+    for (int k = 0; k < nq; ++k) {
+      const OpBlock* a = hl.row(sbL[k]);
+      const OpBlock* b = hr.row(sbR[k]);
+      NSS_REQUIRE(a && b, NSS_E_STATE, "missing H block");
+      HL[k] = ref(hl, *a, Dl[k], 0, 1.0);
+      HR[k] = ref(hr, *b, Dr[k], 0, 1.0);
+    }
+    std::vector<SbTerm> terms;
+    if (!site.fermion) {
+      const BlockOp& SzL = *L.ops.at("Sz");
+      const BlockOp& SzR = *R.ops.at("Sz");
+      const BlockOp& SpL = *L.ops.at("Sp");
+      const BlockOp& SpR = *R.ops.at("Sp");
+      const double jz = hij[0], jxy = hij[3];
+      for (int k = 0; k < nq; ++k) {
+        const int lk = sbL[k], rk = sbR[k];
+        if (jz != 0.0) {  // it=1, SETUP:289-299
+          const OpBlock* a = SzL.row(lk);
+          const OpBlock* b = SzR.row(rk);
+          if (a && b) terms.push_back({k, k, ref(SzL, *a, Dl[k], 0, jz), ref(SzR, *b, Dr[k], 1, 1.0)});
+        }
+        if (jxy == 0.0) continue;
+        auto it = index.find(L.sec[lk].q - QN{2, 0});
+        if (it == index.end()) continue;
+        const int j = it->second, lj = sbL[j], rj = sbR[j];
+        const OpBlock* a = SpL.row(lk);   // (lk, lj)
+        const OpBlock* b = SpR.row(rj);   // (rj, rk)
+        if (!a || a->cs != lj || !b || b->cs != rk) continue;
+        // it=2: Y_k += Jxy Sp_L(lk,lj) Psi_j Sp_R(rj,rk)        SETUP:306-312
+        terms.push_back({k, j, ref(SpL, *a, Dl[j], 0, jxy), ref(SpR, *b, Dr[k], 0, 1.0)});
+        // it=3: hconjg partner, Y_j += A2^T Psi_k B2^T^T        SETUP:315-321
+        terms.push_back({j, k, ref(SpL, *a, Dl[j], 1, jxy), ref(SpR, *b, Dr[k], 1, 1.0)});
+      }
+    } else {
+      const char* names[2] = {"Cup", "Cdn"};
+      const QN dqs[2] = {{2, 0}, {0, 2}};
+      for (int k = 0; k < nq; ++k) {
+        const int lk = sbL[k], rk = sbR[k];
+        for (int sp = 0; sp < 2; ++sp) {
+          const double t = hij[sp * 2 + sp];
+          if (t == 0.0) continue;
+          auto it = index.find(L.sec[lk].q - dqs[sp]);
+          if (it == index.end()) continue;
+          const int j = it->second, lj = sbL[j], rj = sbR[j];
+          const BlockOp& CL = *L.ops.at(names[sp]);
+          const BlockOp& CR = *R.ops.at(names[sp]);
+          const OpBlock* a = CL.row(lj);  // C_L(lj, lk)
+          const OpBlock* b = CR.row(rk);  // C_R(rk, rj)
+          if (!a || a->cs != lk || !b || b->cs != rj) continue;
+          const double alpha = t * fsign(L.sec[lj].q);
+          // it=1: A = t (C_L^T P_L)(lk,lj), B = C_R(rk,rj)       SETUP:601-610
+          terms.push_back({k, j, ref(CL, *a, Dl[k], 1, alpha), ref(CR, *b, Dr[j], 1, 1.0)});
+          // it=2: hconjg partner                                  SETUP:612-619
+          terms.push_back({j, k, ref(CL, *a, Dl[k], 0, alpha), ref(CR, *b, Dr[j], 0, 1.0)});
+        }
+      }
+    }
+    sb_set_terms(ctx, ctx->sb, HL, HR, terms, want32);
+    lastTerms = terms;
+  }
+
+  // ------------------------------------------------------------ renormalize
+  struct Spectrum {
+    std::vector<std::vector<double>> ev, vec;  // per SB sector (ascending)
+    std::vector<double> all_desc;
+    std::vector<std::pair<int, int>> order;    // (block, position) in descending order
+  };
+
+  static Spectrum sort_spectrum(std::vector<std::vector<double>>&& ev, std::vector<std::vector<double>>&& vec) {
+    Spectrum s;
+    s.ev = std::move(ev);
+    s.vec = std::move(vec);
+    std::vector<double> all;
+    std::vector<std::pair<int, int>> pos;
+    for (size_t b = 0; b < s.ev.size(); ++b)
+      for (size_t i = 0; i < s.ev[b].size(); ++i) {
+        all.push_back(s.ev[b][i]);
+        pos.push_back({(int)b, (int)i});
+      }
+    std::vector<int> idx(all.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return all[a] < all[b]; });
+    std::reverse(idx.begin(), idx.end());  // sort_quicksort + reverse, MATRIX_BLOCKS.f90:642-648
+    for (int i : idx) {
+      s.all_desc.push_back(all[i]);
+      s.order.push_back(pos[i]);
+    }
+    return s;
+  }
+
+  static double entropy(const std::vector<double>& ev) {
+    double S = 0.0;
+    for (double l : ev) {
+      const double x = l < 0.0 ? 0.0 : l;
+      S -= x * std::log(x + 1e-32);  // write_entanglement, DMRG_MAIN.f90:511-518
+    }
+    return S;
+  }
+
+  // renormalize_block + operators_renormalization (src/DMRG_RDM.f90:172-404)
+  Block renormalize(const Block& E, const std::vector<int>& sbsec, const Spectrum& sp, int mstates,
+                    double estates, int& ms_out, double& trunc, double& cut,
+                    std::vector<std::pair<QN, std::vector<double>>>& kept) {
+    int ms = 0;
+    double te = 0.0;
+    NSS_REQUIRE(!sp.all_desc.empty(), NSS_E_STATE, "renormalize: empty rho spectrum");
+    const int rc = nss_kept_count(sp.all_desc.data(), (int)sp.all_desc.size(), mstates, estates, E.dim(),
+                                  par.deg_thr, &ms, &te);
+    if (rc != NSS_OK) throw Error(rc, nss_last_error());
+    ms = std::min<int>(ms, (int)sp.all_desc.size());
+    // new sectors: first appearance over the descending spectrum
+    std::vector<int> newsec_of_blk(sp.ev.size(), -1);
+    std::vector<int> cnt(sp.ev.size(), 0);
+    Block N;
+    N.length = E.length;
+    std::vector<int> src_sector;  // enlarged sector of each new sector
+    std::vector<int> blk_of_new;
+    for (int im = 0; im < ms; ++im) {
+      const int b = sp.order[im].first;
+      if (newsec_of_blk[b] < 0) {
+        newsec_of_blk[b] = (int)N.sec.size();
+        N.sec.push_back({E.sec[sbsec[b]].q, 0});
+        src_sector.push_back(sbsec[b]);
+        blk_of_new.push_back(b);
+      }
+      N.sec[newsec_of_blk[b]].dim++;
+      cnt[b]++;
+    }
+    kept.clear();
+    for (auto& s : N.sec) kept.push_back({s.q, {}});
+    for (int im = 0; im < ms; ++im) {
+      const int b = sp.order[im].first;
+      kept[newsec_of_blk[b]].second.push_back(sp.all_desc[im]);
+    }
+    cut = sp.all_desc[ms - 1];
+    // U blocks (D x m, row-major): kept eigenvectors, descending eigenvalue
+    std::vector<int64_t> uoff(N.sec.size());
+    int64_t utot = 0;
+    for (size_t k = 0; k < N.sec.size(); ++k) {
+      uoff[k] = utot;
+      utot += (int64_t)E.sec[src_sector[k]].dim * N.sec[k].dim;
+    }
+    std::vector<double> hu(std::max<int64_t>(1, utot), 0.0);
+    for (size_t k = 0; k < N.sec.size(); ++k) {
+      const int b = blk_of_new[k];
+      const int D = E.sec[src_sector[k]].dim, m = N.sec[k].dim;
+      const std::vector<double>& V = sp.vec[b];
+      for (int c = 0; c < m; ++c) {
+        const int col = D - 1 - c;  // largest first
+        for (int r = 0; r < D; ++r) hu[uoff[k] + (size_t)r * m + c] = V[(size_t)r * D + col];
+      }
+    }
+    DevArr<double> du;
+    du.resize(ctx->pool, hu.size());
+    NSS_HIP(hipMemcpyAsync(du.p, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // map enlarged sector -> new sector
+    std::vector<int> new_of_enl(E.sec.size(), -1);
+    for (size_t k = 0; k < N.sec.size(); ++k) new_of_enl[src_sector[k]] = (int)k;
+    // rotate every operator: T = O(rs,cs) U_cs ; O' = U_rs^T T
+    GemmPlan p1, p2;
+    std::vector<OpPtr> outs;
+    std::vector<std::string> names;
+    std::vector<DevArr<double>> mids;
+    for (auto& kv : E.ops) {
+      const BlockOp& O = *kv.second;
+      OpPtr R = new_op(O.dq, N.sec, N.sec);
+      int64_t mt = 0;
+      std::vector<int64_t> moff;
+      for (const OpBlock& nb : R->blks) {
+        moff.push_back(mt);
+        mt += (int64_t)E.sec[src_sector[nb.rs]].dim * N.sec[nb.cs].dim;
+      }
+      DevArr<double> mid;
+      mid.resize(ctx->pool, std::max<int64_t>(1, mt));
+      for (size_t i = 0; i < R->blks.size(); ++i) {
+        const OpBlock& nb = R->blks[i];
+        const int ers = src_sector[nb.rs], ecs = src_sector[nb.cs];
+        const OpBlock* ob = O.row(ers);
+        const int Dr = E.sec[ers].dim, Dc = E.sec[ecs].dim, mr = N.sec[nb.rs].dim, mc = N.sec[nb.cs].dim;
+        NSS_REQUIRE(ob && ob->cs == ecs, NSS_E_STATE, "renormalize: operator block missing");
+        GemmSeg g1{};
+        g1.a = O.data.p + ob->off; g1.lda = Dc;
+        g1.b = du.p + uoff[nb.cs]; g1.ldb = mc;
+        g1.K = Dc; g1.alpha = 1.0; g1.abase = g1.bbase = BASE_ABS;
+        p1.add_output(BASE_ABS, mid.p + moff[i], 0, mc, Dr, mc, {g1});
+        GemmSeg g2{};
+        g2.a = du.p + uoff[nb.rs]; g2.lda = mr; g2.flags = SEG_TA;
+        g2.b = mid.p + moff[i]; g2.ldb = mc;
+        g2.K = Dr; g2.alpha = 1.0; g2.abase = g2.bbase = BASE_ABS;
+        p2.add_output(BASE_ABS, R->data.p + nb.off, 0, mc, mr, mc, {g2});
+      }
+      mids.push_back(std::move(mid));
+      outs.push_back(R);
+      names.push_back(kv.first);
+    }
+    p1.upload(ctx->pool, ctx->stream);
+    p2.upload(ctx->pool, ctx->stream);
+    launch_gemm_f64(p1, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
+    launch_gemm_f64(p2, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
+    for (size_t i = 0; i < outs.size(); ++i) N.ops[names[i]] = outs[i];
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    ms_out = ms;
+    trunc = te;
+    return N;
+  }
+
+  // ------------------------------------------------------------ step
+  QN target_of(int L) const {
+    // QNtype=local: int(target_qn*L*Norb), src/DMRG_SUPERBLOCK.f90:722
+    QN t;
+    t.a = 2 * (int)(par.target_qn[0] * L);
+    t.b = site.fermion ? 2 * (int)(par.target_qn[1] * L) : 0;
+    return t;
+  }
+
+  void step(char type, int label, int sweep) {
+    using clk = std::chrono::steady_clock;
+    auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    const auto t0 = clk::now();
+    int mstates = par.mdmrg;
+    double estates = par.edmrg;
+    if (type == 'f') {
+      mstates = msweep.at(sweep);
+      estates = esweep.empty() ? 0.0 : esweep.at(sweep);
+    }
+    Step st;
+    nss_step_info& I = st.info;
+    I.type = type;
+    I.label = label;
+    I.sweep = sweep + (type == 'f' ? 1 : 0);
+    left = enlarge(left, true);
+    right = enlarge(right, false);
+    const auto t1 = clk::now();
+    I.t_enlarge = sec(t0, t1);
+    I.left_len = left.length;
+    I.right_len = right.length;
+    I.L = left.length + right.length;
+    setup_superblock(left, right, target_of(I.L));
+    const int64_t msb = sb_n(ctx->sb);
+    I.m_sb = msb;
+    I.nsect = sb_nq(ctx->sb);
+    I.nterms = (int)lastTerms.size();
+    I.hv_flops = sb_flops(ctx->sb);
+    const int neig = (int)std::min<int64_t>(msb, par.neigen);
+    I.neigen = neig;
+    psi.resize(ctx->pool, (size_t)neig * msb);
+    LanczosOut lo;
+    const auto t2 = clk::now();
+    if (neig == msb || msb <= par.lanc_dim_threshold) {
+      lo = dense_diag(ctx, ctx->sb, neig, psi.p);
+      I.numop = 0;
+    } else {
+      lo = lanczos_device(ctx, ctx->sb, neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed,
+                          par.v0_mode, par.precision, psi.p);
+      I.numop = lo.numop + lo.numop_fp32;
+      I.numop_fp32 = lo.numop_fp32;
+    }
+    const auto t3 = clk::now();
+    I.t_diag = sec(t2, t3);
+    for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = lo.evals[i];
+    // sb_get_rdm
+    std::vector<std::vector<double>> evl, vl, evr, vr;
+    sb_rdm_eigh(ctx, ctx->sb, psi.p, 0, evl, vl);
+    sb_rdm_eigh(ctx, ctx->sb, psi.p, 1, evr, vr);
+    Spectrum sl = sort_spectrum(std::move(evl), std::move(vl));
+    Spectrum sr = sort_spectrum(std::move(evr), std::move(vr));
+    const auto t4 = clk::now();
+    I.t_rdm = sec(t3, t4);
+    I.entropy_left = entropy(sl.all_desc);
+    I.entropy_right = entropy(sr.all_desc);
+    bool ren;
+    if (type == 'i') ren = left.length != par.ldmrg;
+    else ren = !((left.length == par.ldmrg + 1) && (label == 1));
+    I.renormalized = ren;
+    I.m_left = left.dim();
+    I.m_right = right.dim();
+    lastL = left;
+    lastR = right;
+    if (ren) {
+      int ml = 0, mr = 0;
+      double tl = 0.0, tr = 0.0;
+      Block nl = renormalize(left, sbL, sl, mstates, estates, ml, tl, I.cut_left, st.kept_left);
+      Block nr = renormalize(right, sbR, sr, mstates, estates, mr, tr, I.cut_right, st.kept_right);
+      left = std::move(nl);
+      right = std::move(nr);
+      I.m_left = ml;
+      I.m_right = mr;
+      I.trunc_left = tl;
+      I.trunc_right = tr;
+    }
+    const auto t5 = clk::now();
+    I.t_renorm = sec(t4, t5);
+    I.t_hv = nss_hv_timer_flush(ctx);
+    I.t_step = sec(t0, t5);
+    if (par.verbose > 0) {
+      std::printf("step %c%d len %d+%d  m_sb %lld nsect %d  E/L %.15f  S %.12f  m %d,%d  nop %d  t %.3fs\n", type,
+                  label, I.left_len, I.right_len, (long long)msb, I.nsect, I.energies[0] / I.L, I.entropy_left,
+                  I.m_left, I.m_right, I.numop, I.t_step);
+      std::fflush(stdout);
+    }
+    steps.push_back(std::move(st));
+  }
+
+  void infinite() {
+    while (left.length < par.ldmrg) step('i', 0, 0);
+  }
+
+  // finite_DMRG, src/DMRG_MAIN.f90:168-241, split into the iDMRG growth and
+  // one call per sweep.
+  std::map<int, Block> bl1, bl2;
+  bool finite_ready = false;
+
+  void finite_begin() {
+    bl1.clear();
+    bl2.clear();
+    bl1[1] = left;
+    bl2[1] = right;
+    while (left.length < par.ldmrg) {
+      step('i', 0, 0);
+      bl1[left.length] = left;
+      bl2[right.length] = right;
+    }
+    finite_ready = true;
+  }
+
+  void sweep(int im) {
+    NSS_REQUIRE(finite_ready, NSS_E_STATE, "finite sweep before nss_dmrg_finite_begin");
+    NSS_REQUIRE(im >= 0 && im < (int)msweep.size(), NSS_E_ARG, "sweep index outside MSWEEP");
+    const int fn = 2 * par.ldmrg;
+    while (left.length < fn) {
+      right = bl2.at(fn - left.length);
+      step('f', 1, im);
+      bl1[left.length] = left;
+    }
+    while (right.length < fn) {
+      left = bl1.at(fn - right.length);
+      step('f', 2, im);
+      bl2[right.length] = right;
+    }
+    while (left.length <= par.ldmrg) {
+      right = bl2.at(fn - left.length);
+      step('f', 1, im);
+      bl1[left.length] = left;
+    }
+  }
+
+  void finite() {
+    finite_begin();
+    for (int im = 0; im < (int)msweep.size(); ++im) sweep(im);
+  }
+};
+
+}  // namespace nss
+
+struct nss_dmrg {
+  nss::Engine* e = nullptr;
+};
+
+using namespace nss;
+
+#define NSS_API_BEGIN try {
+#define NSS_API_END                  \
+  return NSS_OK;                     \
+  }                                  \
+  catch (const nss::Error& e) {      \
+    nss::set_last_error(e.what());   \
+    return e.code;                   \
+  }                                  \
+  catch (const std::exception& e) {  \
+    nss::set_last_error(e.what());   \
+    return NSS_E_ARG;                \
+  }
+
+extern "C" int nss_dmrg_create(nss_ctx* ctx, const nss_model* model, const nss_params* params, nss_dmrg** out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && model && params && out, NSS_E_ARG, "nss_dmrg_create: NULL arguments");
+  NSS_REQUIRE(params->ldmrg >= 1, NSS_E_ARG, "LDMRG must be >= 1");
+  NSS_REQUIRE(params->neigen >= 1 && params->neigen <= 4, NSS_E_ARG, "LANC_NEIGEN must be in 1..4");
+  NSS_REQUIRE(params->mdmrg > 0 || params->edmrg != 0.0, NSS_E_ARG,
+              "Mdmrg=Edmrg=0 can not fix a threshold for the RDM");
+  NSS_REQUIRE(params->precision == NSS_FP64 || params->precision == NSS_FP32_MIXED, NSS_E_ARG, "bad precision");
+  NSS_HIP(hipSetDevice(ctx->device));
+  auto* E = new Engine();
+  E->ctx = ctx;
+  E->model = *model;
+  E->par = *params;
+  if (params->nsweep > 0) {
+    NSS_REQUIRE(params->msweep != nullptr, NSS_E_ARG, "MSWEEP required when NSWEEP > 0");
+    E->msweep.assign(params->msweep, params->msweep + params->nsweep);
+    if (params->esweep) E->esweep.assign(params->esweep, params->esweep + params->nsweep);
+  }
+  E->hij.assign(model->hij, model->hij + 4);
+  E->want32 = params->precision == NSS_FP32_MIXED;
+  try {
+    E->site = make_site(*model);
+    E->left = E->block_from_site();
+    E->right = E->block_from_site();
+  } catch (...) {
+    delete E;
+    throw;
+  }
+  auto* d = new nss_dmrg();
+  d->e = E;
+  ctx->engines.push_back(d);
+  *out = d;
+  NSS_API_END
+}
+
+// Called by nss_ctx_destroy: engines still alive lose their state (their
+// handles stay valid shells that nss_dmrg_destroy frees).
+void nss::engines_release(nss_ctx* ctx) {
+  for (nss_dmrg* d : ctx->engines) {
+    delete d->e;
+    d->e = nullptr;
+  }
+  ctx->engines.clear();
+}
+
+extern "C" int nss_dmrg_run(nss_dmrg* d, int type) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e, NSS_E_ARG, "nss_dmrg_run: NULL handle");
+  NSS_HIP(hipSetDevice(d->e->ctx->device));
+  if (type == 'i' || type == 'I') d->e->infinite();
+  else if (type == 'f' || type == 'F') d->e->finite();
+  else throw Error(NSS_E_ARG, "ERROR DMRG: unsupported DMRGtype: DMRGtype !=['i','f']");
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_finite_begin(nss_dmrg* d) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e, NSS_E_ARG, "nss_dmrg_finite_begin: NULL handle");
+  NSS_HIP(hipSetDevice(d->e->ctx->device));
+  d->e->finite_begin();
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_sweep(nss_dmrg* d, int im) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e, NSS_E_ARG, "nss_dmrg_sweep: NULL handle");
+  NSS_HIP(hipSetDevice(d->e->ctx->device));
+  d->e->sweep(im);
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_nsteps(nss_dmrg* d) { return (d && d->e) ? (int)d->e->steps.size() : -1; }
+
+extern "C" int nss_dmrg_step_info(nss_dmrg* d, int i, nss_step_info* out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && out, NSS_E_ARG, "nss_dmrg_step_info: NULL arguments");
+  NSS_REQUIRE(i >= 0 && i < (int)d->e->steps.size(), NSS_E_ARG, "nss_dmrg_step_info: step out of range");
+  *out = d->e->steps[i].info;
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_kept_qn(nss_dmrg* d, int i, int side, double min_weight, int max, double* qn,
+                                int32_t* count, int32_t* nout) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && nout, NSS_E_ARG, "nss_dmrg_kept_qn: NULL arguments");
+  NSS_REQUIRE(i >= 0 && i < (int)d->e->steps.size(), NSS_E_ARG, "nss_dmrg_kept_qn: step out of range");
+  const auto& k = side == 0 ? d->e->steps[i].kept_left : d->e->steps[i].kept_right;
+  *nout = (int32_t)k.size();
+  for (int j = 0; j < (int)k.size() && j < max; ++j) {
+    if (qn) {
+      qn[2 * j] = 0.5 * k[j].first.a;
+      qn[2 * j + 1] = 0.5 * k[j].first.b;
+    }
+    if (count) {
+      int c = 0;
+      for (double l : k[j].second) c += (l > min_weight) ? 1 : 0;
+      count[j] = c;
+    }
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_last_sb_dims(nss_dmrg* d, int* nq, int* nterms, int64_t* n) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e, NSS_E_ARG, "NULL handle");
+  if (nq) *nq = sb_nq(d->e->ctx->sb);
+  if (nterms) *nterms = (int)d->e->lastTerms.size();
+  if (n) *n = sb_n(d->e->ctx->sb);
+  NSS_API_END
+}
+
+// Dense row-major copies: HL, HR concatenated per sector; A, B per term
+// (A_t: Dl_dst x Dl_src, B_t: Dr_dst x Dr_src with Hv_dst += A Psi_src B^T).
+extern "C" int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, int32_t* dst, int32_t* src,
+                                       double* HL, double* HR, double* A, double* B, double* psi) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e, NSS_E_ARG, "NULL handle");
+  Engine* E = d->e;
+  SbDevice* sb = E->ctx->sb;
+  const auto& dl = sb_Dl(sb);
+  const auto& dr = sb_Dr(sb);
+  const int nq = (int)dl.size();
+  hipStream_t s = E->ctx->stream;
+  auto fetch = [&](const OpRef& r, int rows, int cols, double* outp) {
+    // r describes an operand (rows x cols) with ld/trans; alpha applied
+    std::vector<double> raw((size_t)rows * cols);
+    const double* p = static_cast<const double*>(r.p);
+    const int st_rows = r.trans ? cols : rows, st_cols = r.ld;
+    std::vector<double> stor((size_t)st_rows * st_cols);
+    NSS_HIP(hipMemcpyAsync(stor.data(), p, stor.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < cols; ++j)
+        outp[(size_t)i * cols + j] = r.alpha * (r.trans ? stor[(size_t)j * r.ld + i] : stor[(size_t)i * r.ld + j]);
+  };
+  if (Dl) for (int q = 0; q < nq; ++q) Dl[q] = dl[q];
+  if (Dr) for (int q = 0; q < nq; ++q) Dr[q] = dr[q];
+  size_t oh = 0, orr = 0;
+  for (int q = 0; q < nq; ++q) {
+    const BlockOp& hl = *E->lastL.ops.at("H");
+    const BlockOp& hr = *E->lastR.ops.at("H");
+    OpRef a, b;
+    a.p = hl.data.p + hl.row(E->sbL[q])->off; a.ld = dl[q];
+    b.p = hr.data.p + hr.row(E->sbR[q])->off; b.ld = dr[q];
+    if (HL) fetch(a, dl[q], dl[q], HL + oh);
+    if (HR) fetch(b, dr[q], dr[q], HR + orr);
+    oh += (size_t)dl[q] * dl[q];
+    orr += (size_t)dr[q] * dr[q];
+  }
+  size_t oa = 0, ob = 0;
+  for (size_t t = 0; t < E->lastTerms.size(); ++t) {
+    const SbTerm& T = E->lastTerms[t];
+    if (dst) dst[t] = T.dst;
+    if (src) src[t] = T.src;
+    if (A) fetch(T.A, dl[T.dst], dl[T.src], A + oa);
+    if (B) {  // B = (B^T)^T: fetch B^T (Dr_src x Dr_dst) then transpose
+      std::vector<double> bt((size_t)dr[T.src] * dr[T.dst]);
+      fetch(T.Bt, dr[T.src], dr[T.dst], bt.data());
+      for (int i = 0; i < dr[T.dst]; ++i)
+        for (int j = 0; j < dr[T.src]; ++j) B[ob + (size_t)i * dr[T.src] + j] = bt[(size_t)j * dr[T.dst] + i];
+    }
+    oa += (size_t)dl[T.dst] * dl[T.src];
+    ob += (size_t)dr[T.dst] * dr[T.src];
+  }
+  if (psi) {
+    NSS_HIP(hipMemcpyAsync(psi, E->psi.p, sb_n(sb) * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipStreamSynchronize(s));
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_destroy(nss_dmrg* d) {
+  NSS_API_BEGIN
+  if (!d) return NSS_OK;
+  if (d->e) {
+    nss_ctx* c = d->e->ctx;
+    // the context's superblock plan points into this engine's blocks
+    sb_reset(c->sb);
+    c->engines.erase(std::remove(c->engines.begin(), c->engines.end(), d), c->engines.end());
+    delete d->e;
+  }
+  delete d;
+  NSS_API_END
+}

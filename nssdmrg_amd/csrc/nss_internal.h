@@ -1,0 +1,273 @@
+// Internal declarations of the nssDMRG MI355X library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nssdmrg.h"
+
+namespace nss {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void set_last_error(const std::string& msg);
+
+#define NSS_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::nss::Error(NSS_E_HIP, std::string(#call " failed: ") + hipGetErrorString(e_) + \
+                                        " at " __FILE__ ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define NSS_REQUIRE(cond, code, msg)                 \
+  do {                                               \
+    if (!(cond)) throw ::nss::Error((code), (msg));  \
+  } while (0)
+
+// ------------------------------------------------------------- allocator
+// Caching device allocator: power-of-two size classes, never returns memory
+// to the driver until the context is destroyed (hipMalloc per DMRG step
+// would dominate small steps).
+class DevicePool {
+ public:
+  explicit DevicePool(int device) : device_(device) {}
+  ~DevicePool();
+  void* alloc(size_t bytes);
+  void free(void* p);
+  int device() const { return device_; }
+  size_t bytes_in_use() const { return in_use_; }
+  size_t bytes_reserved() const { return reserved_; }
+
+ private:
+  int device_;
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> live_;
+  size_t in_use_ = 0, reserved_ = 0;
+};
+
+template <typename T>
+struct DevArr {  // owning device array backed by a pool
+  DevicePool* pool = nullptr;
+  T* p = nullptr;
+  size_t n = 0;
+  DevArr() = default;
+  DevArr(const DevArr&) = delete;
+  DevArr& operator=(const DevArr&) = delete;
+  DevArr(DevArr&& o) noexcept { *this = std::move(o); }
+  DevArr& operator=(DevArr&& o) noexcept {
+    if (this != &o) {
+      release();
+      pool = o.pool; p = o.p; n = o.n;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
+  ~DevArr() { release(); }
+  void release() {
+    if (p && pool) pool->free(p);
+    p = nullptr; n = 0;
+  }
+  void resize(DevicePool* pl, size_t count) {
+    if (count <= n && p) return;
+    release();
+    pool = pl;
+    n = count;
+    p = count ? static_cast<T*>(pool->alloc(count * sizeof(T))) : nullptr;
+  }
+};
+
+// ------------------------------------------------------- grouped GEMM
+// acc(tile) = sum_seg alpha_seg * opA(A_seg) . opB(B_seg); C(tile) = acc (+ C if beta).
+// Pointers are (base selector, element offset): base 0 absolute address
+// (offset ignored), 1 = X (input vector), 2 = W (scratch), 3 = Y (output).
+enum : int { BASE_ABS = 0, BASE_X = 1, BASE_W = 2, BASE_Y = 3 };
+enum : int { SEG_TA = 1, SEG_TB = 2 };
+
+struct GemmSeg {
+  const void* a;       // used when abase == BASE_ABS
+  const void* b;
+  int64_t aoff, boff;  // element offsets for relative bases
+  int32_t lda, ldb, K, flags;
+  int32_t abase, bbase;
+  double alpha;
+};
+
+struct GemmTile {
+  void* c;
+  int64_t coff;
+  int32_t ldc, M, N, row0, col0;
+  int32_t seg0, nseg;
+  int32_t cbase, beta;
+};
+
+struct Bases {
+  const void* x;
+  void* w;
+  void* y;
+};
+
+// One grouped launch: tiles over outputs, segments over K-contributions.
+struct GemmPlan {
+  std::vector<GemmTile> htiles;
+  std::vector<GemmSeg> hsegs;
+  DevArr<GemmTile> tiles;
+  DevArr<GemmSeg> segs;
+  double flops = 0.0;
+  void clear() { htiles.clear(); hsegs.clear(); flops = 0.0; }
+  // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
+  // the same list of segments.
+  void add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
+                  const std::vector<GemmSeg>& segs_for_c, int beta = 0);
+  void upload(DevicePool* pool, hipStream_t s);
+};
+
+constexpr int GEMM_BM = 64, GEMM_BN = 64;
+
+enum : int { GEMM_GENERIC = 0, GEMM_HV_W = 1, GEMM_HV_Y = 2 };
+void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind = GEMM_GENERIC);
+void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind);
+
+// ------------------------------------------------------- vector kernels
+void launch_fill_splitmix(double* v, int64_t n, uint64_t seed, hipStream_t s);
+void launch_fill_const(double* v, int64_t n, double c, hipStream_t s);
+void launch_d2f(const double* x, float* y, int64_t n, hipStream_t s);
+void launch_f2d(const float* x, double* y, int64_t n, hipStream_t s);
+void launch_scale(double* x, int64_t n, const double* num, const double* den, hipStream_t s);
+// h[j] = <V_j, w> for j < ncols (V vectors of length n, stride ldv);
+// partial scratch >= ncols * nchunks(n); result to h (device).
+int64_t gemv_t_chunks(int64_t n);
+void launch_gemv_t(const double* V, int64_t ldv, int ncols, const double* w, int64_t n,
+                   double* partial, double* h, hipStream_t s);
+// w -= V h  (or y = V h when overwrite)
+void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, double* w, int64_t n,
+                   int overwrite, hipStream_t s);
+// out[0] = sqrt(<x,x>)
+void launch_norm(const double* x, int64_t n, double* partial, double* out, hipStream_t s);
+// y[i] = x[i] * (1/den[0])
+void launch_div(const double* x, double* y, int64_t n, const double* den, hipStream_t s);
+
+// Kronecker-scatter for block enlargement: dst sub-block (rows x cols, ld)
+// = sum of contributions scale * X (or X^T, or identity).
+struct KronContrib {
+  const double* x;   // nullptr -> identity
+  int32_t ldx, trans;
+  double scale;
+};
+struct KronTask {
+  double* dst;
+  int32_t ldd, rows, cols;
+  int32_t c0, nc;
+  int32_t row_t, col_t;  // tile origin within the task
+};
+void launch_kron(const KronTask* tasks, int ntasks, const KronContrib* contribs, hipStream_t s);
+
+// ------------------------------------------------------------- context
+struct HostMat {  // small host-side dense matrix (row-major)
+  int r = 0, c = 0;
+  std::vector<double> a;
+};
+
+struct SbDevice;  // superblock plan (nss_superblock.cpp)
+constexpr int KRON_TILE = 32;  // k_kron task tile edge
+
+}  // namespace nss
+
+
+struct nss_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  nss::DevicePool* pool = nullptr;
+  void* rocblas = nullptr;  // rocblas_handle, created lazily
+  nss::SbDevice* sb = nullptr;
+  // H*v timing with HIP events on the context stream: 3 events per call
+  // (start, after the W launch, after the Y launch).
+  bool time_hv = true;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  // per recorded call: flops of the W and Y launches
+  std::vector<double> ev_fw, ev_fy;
+  // accumulated seconds / flops, [0] FP64, [1] FP32
+  double st_tw[2] = {0, 0}, st_ty[2] = {0, 0}, st_fw[2] = {0, 0}, st_fy[2] = {0, 0};
+  int64_t st_calls[2] = {0, 0};
+  std::vector<int> ev_kind;  // 0 fp64, 1 fp32 per recorded call
+  // engines created on this context: destroyed (their device memory
+  // returned) before the context's pool goes away
+  std::vector<struct nss_dmrg*> engines;
+};
+
+namespace nss {
+void nss_release_rocblas(::nss_ctx* ctx);
+// Frees the state of every engine still alive on ctx (nss_engine.cpp).
+void engines_release(::nss_ctx* ctx);
+// Synchronises the context stream, folds the recorded H*v events into the
+// context statistics and returns the seconds spent in H*v since the last flush.
+double nss_hv_timer_flush(::nss_ctx* ctx);
+
+// Host symmetric eigen-solvers (nss_eig.cpp).
+// Dense symmetric a (n x n row-major, overwritten by eigenvectors as columns),
+// eigenvalues ascending in w.  Householder tridiagonalisation + implicit QL.
+void sym_eig(int n, double* a, double* w);
+// Symmetric tridiagonal (d[0..n), e[0..n-1)): eigenvalues ascending to w and
+// the requested rows of the eigenvector matrix Z (T = Z diag(w) Z^T):
+// zrows (nr x n, row-major) must enter holding the same rows of the identity.
+void tridiag_eig(int n, const double* d, const double* e, double* w, int nr, const int* rows,
+                 double* zrows);
+// Full eigenvectors of the tridiagonal (z n x n row-major, columns = vectors).
+void tridiag_eig_full(int n, const double* d, const double* e, double* w, double* z);
+
+// Lanczos on a context's current superblock (nss_lanczos.cpp).
+struct LanczosOut {
+  std::vector<double> evals;
+  int numop = 0, numop_fp32 = 0;
+};
+// Eigenvectors stay on the device in xdev (neigen x n).
+LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int maxiter, double tol,
+                          uint64_t seed, int v0_mode, int precision, double* xdev);
+// Dense path: H_sb assembled on the device from unit vectors, diagonalised on the host.
+LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev);
+
+// Superblock plan API (nss_superblock.cpp).
+struct OpRef {  // a dense operand: device pointer, leading dim, transpose flag, scale
+  const void* p = nullptr;
+  const void* p32 = nullptr;  // FP32 mirror (mixed precision), may be null
+  int32_t ld = 0;
+  int32_t trans = 0;
+  double alpha = 1.0;
+};
+struct SbTerm {
+  int dst, src;
+  OpRef A;   // Dl_dst x Dl_src
+  OpRef Bt;  // B^T : Dr_src x Dr_dst
+};
+SbDevice* sb_create(nss_ctx* ctx);
+void sb_destroy(SbDevice* sb);
+void sb_reset(SbDevice* sb);  // forget layout and plans (operands owned elsewhere may be gone)
+void sb_set_layout(SbDevice* sb, const std::vector<int>& Dl, const std::vector<int>& Dr,
+                   const std::vector<int64_t>* offset);
+// HL, HR per sector (square, row-major refs), terms; builds W/Y plans.
+void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL,
+                  const std::vector<OpRef>& HR, const std::vector<SbTerm>& terms, bool want_fp32);
+void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);          // device, async
+void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);     // device, async
+int64_t sb_n(const SbDevice* sb);
+double sb_flops(const SbDevice* sb);
+int sb_nq(const SbDevice* sb);
+const std::vector<int>& sb_Dl(const SbDevice* sb);
+const std::vector<int>& sb_Dr(const SbDevice* sb);
+const std::vector<int64_t>& sb_off(const SbDevice* sb);
+// rho blocks of a device vector: side 0 left, 1 right -> host eigen-decomposition.
+void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
+                 std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs);
+void sb_take_fp32_ops(nss_ctx* ctx, SbDevice* sb);
+
+}  // namespace nss

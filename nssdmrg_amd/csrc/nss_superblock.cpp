@@ -1,0 +1,751 @@
+// Context, allocator, grouped-GEMM plans and the superblock H*v of the
+// nssDMRG MI355X library.  H*v mirrors spMatVec_direct_main
+// (src/DMRG_SUPERBLOCK_SETUP.f90:740-865) as two grouped launches:
+//   W phase: W_t = Psi_src(t) . B_t^T              (one tile set per term)
+//   Y phase: Y_q = H_L,q Psi_q + Psi_q H_R,q^T + sum_{t->q} A_t W_t
+// which is the reference's evaluation order (B first, then A).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <mutex>
+
+#include "nss_internal.h"
+
+namespace nss {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ---------------------------------------------------------------- pool
+static size_t size_class(size_t b) {
+  size_t c = 256;
+  while (c < b) c <<= 1;
+  return c;
+}
+
+void* DevicePool::alloc(size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  const size_t c = size_class(bytes);
+  void* p = nullptr;
+  auto it = free_.find(c);
+  if (it != free_.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+  } else {
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      // release cached blocks and retry once
+      for (auto& kv : free_)
+        for (void* q : kv.second) {
+          (void)hipFree(q);
+          reserved_ -= kv.first;
+        }
+      free_.clear();
+      e = hipMalloc(&p, c);
+      if (e != hipSuccess)
+        throw Error(NSS_E_ALLOC, "device allocation of " + std::to_string(c) + " bytes failed: " +
+                                     hipGetErrorString(e));
+    }
+    reserved_ += c;
+  }
+  live_[p] = c;
+  in_use_ += c;
+  return p;
+}
+
+void DevicePool::free(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_[it->second].push_back(p);
+  in_use_ -= it->second;
+  live_.erase(it);
+}
+
+DevicePool::~DevicePool() {
+  for (auto& kv : free_)
+    for (void* q : kv.second) (void)hipFree(q);
+  for (auto& kv : live_) (void)hipFree(kv.first);
+}
+
+// ---------------------------------------------------------------- plans
+void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
+                          const std::vector<GemmSeg>& sg, int beta) {
+  if (M <= 0 || N <= 0) return;
+  const int seg0 = (int)hsegs.size();
+  int nseg = 0;
+  for (const GemmSeg& g : sg) {
+    if (g.K <= 0) continue;
+    hsegs.push_back(g);
+    ++nseg;
+    flops += 2.0 * M * (double)N * g.K;
+  }
+  for (int r = 0; r < M; r += GEMM_BM)
+    for (int cc = 0; cc < N; cc += GEMM_BN) {
+      GemmTile t{};
+      t.c = c;
+      t.coff = coff;
+      t.ldc = ldc;
+      t.M = M;
+      t.N = N;
+      t.row0 = r;
+      t.col0 = cc;
+      t.seg0 = seg0;
+      t.nseg = nseg;
+      t.cbase = cbase;
+      t.beta = beta;
+      htiles.push_back(t);
+    }
+}
+
+void GemmPlan::upload(DevicePool* pool, hipStream_t s) {
+  tiles.resize(pool, std::max<size_t>(1, htiles.size()));
+  segs.resize(pool, std::max<size_t>(1, hsegs.size()));
+  if (!htiles.empty())
+    NSS_HIP(hipMemcpyAsync(tiles.p, htiles.data(), htiles.size() * sizeof(GemmTile),
+                           hipMemcpyHostToDevice, s));
+  if (!hsegs.empty())
+    NSS_HIP(hipMemcpyAsync(segs.p, hsegs.data(), hsegs.size() * sizeof(GemmSeg),
+                           hipMemcpyHostToDevice, s));
+  // host vectors must outlive the async copies: callers keep the plan alive
+  // and synchronize before mutating it.
+  NSS_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- superblock
+struct SbDevice {
+  std::vector<int> Dl, Dr;
+  std::vector<int64_t> off;
+  int64_t n = 0;
+  GemmPlan W64, Y64, W32, Y32;
+  DevArr<double> W;
+  DevArr<float> Wf, Xf, Yf;
+  int64_t wsize = 0;
+  double flops = 0.0;
+  bool has32 = false;
+  // storage of the low-level upload path
+  DevArr<double> owned;
+  DevArr<float> owned32;
+  // rho scratch
+  DevArr<double> rho;
+  GemmPlan R;
+};
+
+SbDevice* sb_create(nss_ctx*) { return new SbDevice(); }
+void sb_reset(SbDevice* sb) {
+  sb->Dl.clear();
+  sb->Dr.clear();
+  sb->off.clear();
+  sb->n = 0;
+  sb->W64.clear();
+  sb->Y64.clear();
+  sb->W32.clear();
+  sb->Y32.clear();
+  sb->has32 = false;
+  sb->flops = 0.0;
+}
+void sb_destroy(SbDevice* sb) { delete sb; }
+int64_t sb_n(const SbDevice* sb) { return sb->n; }
+double sb_flops(const SbDevice* sb) { return sb->flops; }
+int sb_nq(const SbDevice* sb) { return (int)sb->Dl.size(); }
+const std::vector<int>& sb_Dl(const SbDevice* sb) { return sb->Dl; }
+const std::vector<int>& sb_Dr(const SbDevice* sb) { return sb->Dr; }
+const std::vector<int64_t>& sb_off(const SbDevice* sb) { return sb->off; }
+
+void sb_set_layout(SbDevice* sb, const std::vector<int>& Dl, const std::vector<int>& Dr,
+                   const std::vector<int64_t>* offset) {
+  NSS_REQUIRE(Dl.size() == Dr.size() && !Dl.empty(), NSS_E_ARG, "sb_set_layout: nq must be > 0");
+  std::vector<int64_t> off(Dl.size(), 0);
+  int64_t acc = 0;
+  for (size_t q = 0; q < Dl.size(); ++q) {  // validate before touching the current state
+    NSS_REQUIRE(Dl[q] >= 0 && Dr[q] >= 0, NSS_E_ARG, "sb_set_layout: negative sector dimension");
+    if (offset) {
+      NSS_REQUIRE((*offset)[q] == acc, NSS_E_ARG,
+                  "sb_set_layout: offsets must be the prefix sums of Dl*Dr (Offset, "
+                  "DMRG_SUPERBLOCK.f90:72)");
+    }
+    off[q] = acc;
+    acc += (int64_t)Dl[q] * Dr[q];
+  }
+  sb->Dl = Dl;
+  sb->Dr = Dr;
+  sb->off = off;
+  sb->n = acc;
+  sb->W64.clear();
+  sb->Y64.clear();
+  sb->W32.clear();
+  sb->Y32.clear();
+  sb->has32 = false;
+  sb->flops = 0.0;
+}
+
+static GemmSeg seg_abs(const void* a, int lda, int ta, const void* b, int ldb, int tb, int K,
+                       double alpha) {
+  GemmSeg g{};
+  g.a = a; g.b = b; g.aoff = 0; g.boff = 0;
+  g.lda = lda; g.ldb = ldb; g.K = K;
+  g.flags = (ta ? SEG_TA : 0) | (tb ? SEG_TB : 0);
+  g.abase = BASE_ABS; g.bbase = BASE_ABS;
+  g.alpha = alpha;
+  return g;
+}
+
+static void build_hv_plans(SbDevice* sb, const std::vector<OpRef>& HL, const std::vector<OpRef>& HR,
+                           const std::vector<SbTerm>& terms, bool f32, GemmPlan& Wp, GemmPlan& Yp) {
+  Wp.clear();
+  Yp.clear();
+  const int nq = (int)sb->Dl.size();
+  std::vector<int64_t> woff(terms.size(), 0);
+  int64_t w = 0;
+  for (size_t t = 0; t < terms.size(); ++t) {
+    woff[t] = w;
+    w += (int64_t)sb->Dl[terms[t].src] * sb->Dr[terms[t].dst];
+  }
+  sb->wsize = w;
+  auto P = [&](const OpRef& r) { return f32 ? r.p32 : r.p; };
+  // W phase: W_t = Psi_src . B_t^T   (Dl_src x Dr_dst, K = Dr_src)
+  for (size_t t = 0; t < terms.size(); ++t) {
+    const SbTerm& T = terms[t];
+    const int dls = sb->Dl[T.src], drs = sb->Dr[T.src], drd = sb->Dr[T.dst];
+    if (dls == 0 || drs == 0 || drd == 0 || sb->Dl[T.dst] == 0) continue;
+    GemmSeg g = seg_abs(nullptr, drs, 0, P(T.Bt), T.Bt.ld, T.Bt.trans, drs, 1.0);
+    g.abase = BASE_X;
+    g.aoff = sb->off[T.src];
+    Wp.add_output(BASE_W, nullptr, woff[t], drd, dls, drd, {g});
+  }
+  // Y phase
+  std::vector<std::vector<size_t>> into(nq);
+  for (size_t t = 0; t < terms.size(); ++t) into[terms[t].dst].push_back(t);
+  for (int q = 0; q < nq; ++q) {
+    const int dl = sb->Dl[q], dr = sb->Dr[q];
+    if (dl == 0 || dr == 0) continue;
+    std::vector<GemmSeg> sg;
+    GemmSeg a = seg_abs(P(HL[q]), HL[q].ld, HL[q].trans, nullptr, dr, 0, dl, HL[q].alpha);
+    a.bbase = BASE_X;
+    a.boff = sb->off[q];
+    sg.push_back(a);
+    GemmSeg b = seg_abs(nullptr, dr, 0, P(HR[q]), HR[q].ld, !HR[q].trans, dr, HR[q].alpha);
+    b.abase = BASE_X;
+    b.aoff = sb->off[q];
+    sg.push_back(b);
+    for (size_t t : into[q]) {
+      const SbTerm& T = terms[t];
+      const int dls = sb->Dl[T.src];
+      if (dls == 0 || sb->Dr[T.src] == 0) continue;
+      GemmSeg c = seg_abs(P(T.A), T.A.ld, T.A.trans, nullptr, dr, 0, dls, T.A.alpha);
+      c.bbase = BASE_W;
+      c.boff = woff[t];
+      sg.push_back(c);
+    }
+    Yp.add_output(BASE_Y, nullptr, sb->off[q], dr, dl, dr, sg);
+  }
+}
+
+void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, const std::vector<OpRef>& HR,
+                  const std::vector<SbTerm>& terms, bool want_fp32) {
+  const int nq = (int)sb->Dl.size();
+  NSS_REQUIRE((int)HL.size() == nq && (int)HR.size() == nq, NSS_E_ARG, "sb_set_terms: HL/HR count");
+  for (const SbTerm& t : terms)
+    NSS_REQUIRE(t.dst >= 0 && t.dst < nq && t.src >= 0 && t.src < nq, NSS_E_ARG,
+                "sb_set_terms: term sector index out of range");
+  build_hv_plans(sb, HL, HR, terms, false, sb->W64, sb->Y64);
+  sb->flops = sb->W64.flops + sb->Y64.flops;
+  sb->W.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
+  sb->W64.upload(ctx->pool, ctx->stream);
+  sb->Y64.upload(ctx->pool, ctx->stream);
+  sb->has32 = false;
+  if (want_fp32) {
+    build_hv_plans(sb, HL, HR, terms, true, sb->W32, sb->Y32);
+    sb->Wf.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
+    sb->Xf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
+    sb->Yf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
+    sb->W32.upload(ctx->pool, ctx->stream);
+    sb->Y32.upload(ctx->pool, ctx->stream);
+    sb->has32 = true;
+  }
+}
+
+static hipEvent_t next_event(nss_ctx* ctx) {
+  if (ctx->ev_used == ctx->ev.size()) {
+    hipEvent_t e;
+    NSS_HIP(hipEventCreate(&e));
+    ctx->ev.push_back(e);
+  }
+  return ctx->ev[ctx->ev_used++];
+}
+
+static void record(nss_ctx* ctx) { NSS_HIP(hipEventRecord(next_event(ctx), ctx->stream)); }
+
+void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
+  Bases b{x, sb->W.p, y};
+  const bool t = ctx->time_hv;
+  if (t) record(ctx);
+  launch_gemm_f64(sb->W64, b, ctx->stream, GEMM_HV_W);
+  if (t) record(ctx);
+  launch_gemm_f64(sb->Y64, b, ctx->stream, GEMM_HV_Y);
+  if (t) {
+    record(ctx);
+    ctx->ev_kind.push_back(0);
+    ctx->ev_fw.push_back(sb->W64.flops);
+    ctx->ev_fy.push_back(sb->Y64.flops);
+  }
+}
+
+void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
+  NSS_REQUIRE(sb->has32, NSS_E_STATE, "FP32 H*v requested but blocks were uploaded for FP64 only");
+  launch_d2f(x, sb->Xf.p, sb->n, ctx->stream);
+  Bases b{sb->Xf.p, sb->Wf.p, sb->Yf.p};
+  const bool t = ctx->time_hv;
+  if (t) record(ctx);
+  launch_gemm_f32(sb->W32, b, ctx->stream, GEMM_HV_W);
+  if (t) record(ctx);
+  launch_gemm_f32(sb->Y32, b, ctx->stream, GEMM_HV_Y);
+  if (t) {
+    record(ctx);
+    ctx->ev_kind.push_back(1);
+    ctx->ev_fw.push_back(sb->W32.flops);
+    ctx->ev_fy.push_back(sb->Y32.flops);
+  }
+  launch_f2d(sb->Yf.p, y, sb->n, ctx->stream);
+}
+
+double nss_hv_timer_flush(nss_ctx* ctx) {
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  for (size_t c = 0; c < ctx->ev_kind.size(); ++c) {
+    float a = 0.f, b = 0.f;
+    NSS_HIP(hipEventElapsedTime(&a, ctx->ev[3 * c], ctx->ev[3 * c + 1]));
+    NSS_HIP(hipEventElapsedTime(&b, ctx->ev[3 * c + 1], ctx->ev[3 * c + 2]));
+    const double w = a * 1e-3, y = b * 1e-3;
+    const int k = ctx->ev_kind[c];
+    tot += w + y;
+    ctx->st_tw[k] += w;
+    ctx->st_ty[k] += y;
+    ctx->st_fw[k] += ctx->ev_fw[c];
+    ctx->st_fy[k] += ctx->ev_fy[c];
+    ctx->st_calls[k]++;
+  }
+  ctx->ev_used = 0;
+  ctx->ev_kind.clear();
+  ctx->ev_fw.clear();
+  ctx->ev_fy.clear();
+  return tot;
+}
+
+void nss_release_rocblas(nss_ctx* ctx) {
+  for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
+  ctx->ev.clear();
+}
+
+void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
+                 std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs) {
+  const int nq = (int)sb->Dl.size();
+  std::vector<int64_t> roff(nq, 0);
+  int64_t tot = 0;
+  for (int q = 0; q < nq; ++q) {
+    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+    roff[q] = tot;
+    tot += (int64_t)d * d;
+  }
+  sb->rho.resize(ctx->pool, std::max<int64_t>(1, tot));
+  sb->R.clear();
+  for (int q = 0; q < nq; ++q) {
+    const int dl = sb->Dl[q], dr = sb->Dr[q];
+    if (dl == 0 || dr == 0) continue;
+    GemmSeg g{};
+    g.alpha = 1.0;
+    g.abase = BASE_X;
+    g.bbase = BASE_X;
+    g.aoff = g.boff = sb->off[q];
+    g.lda = g.ldb = dr;
+    if (side == 0) {  // rho_L = Psi Psi^T : K = Dr
+      g.K = dr;
+      g.flags = SEG_TB;
+      sb->R.add_output(BASE_Y, nullptr, roff[q], dl, dl, dl, {g});
+    } else {  // rho_R = Psi^T Psi : K = Dl
+      g.K = dl;
+      g.flags = SEG_TA;
+      sb->R.add_output(BASE_Y, nullptr, roff[q], dr, dr, dr, {g});
+    }
+  }
+  sb->R.upload(ctx->pool, ctx->stream);
+  Bases b{psi_dev, nullptr, sb->rho.p};
+  launch_gemm_f64(sb->R, b, ctx->stream, GEMM_GENERIC);
+  std::vector<double> h(tot);
+  if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  evals.assign(nq, {});
+  evecs.assign(nq, {});
+  for (int q = 0; q < nq; ++q) {
+    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+    if (d == 0 || (sb->Dl[q] * (int64_t)sb->Dr[q]) == 0) {
+      evals[q].assign(0, 0.0);
+      continue;
+    }
+    double* a = h.data() + roff[q];
+    for (int i = 0; i < d; ++i)
+      for (int j = i + 1; j < d; ++j)
+        if (std::fabs(a[(size_t)i * d + j] - a[(size_t)j * d + i]) > 1e-12)
+          throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
+    evals[q].resize(d);
+    evecs[q].assign(a, a + (size_t)d * d);
+    sym_eig(d, evecs[q].data(), evals[q].data());
+    for (double& e : evals[q])
+      if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
+  }
+}
+
+}  // namespace nss
+
+using namespace nss;
+
+// ------------------------------------------------------------------ C ABI
+#define NSS_API_BEGIN try {
+#define NSS_API_END                                  \
+  return NSS_OK;                                     \
+  }                                                  \
+  catch (const nss::Error& e) {                      \
+    nss::set_last_error(e.what());                   \
+    return e.code;                                   \
+  }                                                  \
+  catch (const std::exception& e) {                  \
+    nss::set_last_error(e.what());                   \
+    return NSS_E_ARG;                                \
+  }
+
+extern "C" const char* nss_last_error(void) { return nss::g_last_error.c_str(); }
+
+extern "C" int nss_device_ok(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return 0;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return 0;
+  return std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+extern "C" int nss_ctx_create(int device, nss_ctx** out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(out != nullptr, NSS_E_ARG, "nss_ctx_create: out is NULL");
+  int n = 0;
+  NSS_HIP(hipGetDeviceCount(&n));
+  NSS_REQUIRE(device >= 0 && device < n, NSS_E_ARG, "nss_ctx_create: no such device");
+  hipDeviceProp_t prop;
+  NSS_HIP(hipGetDeviceProperties(&prop, device));
+  NSS_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, NSS_E_HIP,
+              std::string("nss_ctx_create: device is ") + prop.gcnArchName +
+                  ", this library is built for gfx950 only");
+  NSS_HIP(hipSetDevice(device));
+  nss_ctx* c = new nss_ctx();
+  c->device = device;
+  NSS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->pool = new DevicePool(device);
+  c->sb = sb_create(c);
+  *out = c;
+  NSS_API_END
+}
+
+extern "C" int nss_ctx_destroy(nss_ctx* ctx) {
+  NSS_API_BEGIN
+  if (!ctx) return NSS_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  nss::engines_release(ctx);
+  sb_destroy(ctx->sb);
+  nss::nss_release_rocblas(ctx);
+  delete ctx->pool;
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  NSS_API_END
+}
+
+extern "C" int nss_sb_set_layout(nss_ctx* ctx, int nq, const int32_t* Dl, const int32_t* Dr,
+                                 const int64_t* offset) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && Dl && Dr && nq > 0, NSS_E_ARG, "nss_sb_set_layout: bad arguments");
+  std::vector<int> dl(Dl, Dl + nq), dr(Dr, Dr + nq);
+  std::vector<int64_t> off;
+  if (offset) off.assign(offset, offset + nq);
+  NSS_HIP(hipSetDevice(ctx->device));
+  sb_set_layout(ctx->sb, dl, dr, offset ? &off : nullptr);
+  NSS_API_END
+}
+
+extern "C" int nss_sb_upload_blocks(nss_ctx* ctx, const double* const* HL, const double* const* HR,
+                                    int nterms, const int32_t* dst, const int32_t* src,
+                                    const double* const* A, const double* const* B, int layout,
+                                    int precision) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && HL && HR, NSS_E_ARG, "nss_sb_upload_blocks: NULL arguments");
+  SbDevice* sb = ctx->sb;
+  const int nq = (int)sb->Dl.size();
+  NSS_REQUIRE(nq > 0, NSS_E_STATE, "nss_sb_upload_blocks: call nss_sb_set_layout first");
+  NSS_REQUIRE(nterms >= 0 && (nterms == 0 || (dst && src && A && B)), NSS_E_ARG,
+              "nss_sb_upload_blocks: bad term arrays");
+  NSS_REQUIRE(layout == NSS_ROW_MAJOR || layout == NSS_COL_MAJOR, NSS_E_ARG, "bad layout");
+  NSS_REQUIRE(precision == NSS_FP64 || precision == NSS_FP32_MIXED, NSS_E_ARG, "bad precision");
+  NSS_HIP(hipSetDevice(ctx->device));
+  const bool cm = layout == NSS_COL_MAJOR;
+  // pack every block into one staging buffer, one upload
+  std::vector<int64_t> oHL(nq), oHR(nq), oA(nterms), oB(nterms);
+  int64_t tot = 0;
+  for (int q = 0; q < nq; ++q) {
+    oHL[q] = tot; tot += (int64_t)sb->Dl[q] * sb->Dl[q];
+    oHR[q] = tot; tot += (int64_t)sb->Dr[q] * sb->Dr[q];
+  }
+  for (int t = 0; t < nterms; ++t) {
+    NSS_REQUIRE(dst[t] >= 0 && dst[t] < nq && src[t] >= 0 && src[t] < nq, NSS_E_ARG,
+                "nss_sb_upload_blocks: term sector out of range");
+    oA[t] = tot; tot += (int64_t)sb->Dl[dst[t]] * sb->Dl[src[t]];
+    oB[t] = tot; tot += (int64_t)sb->Dr[dst[t]] * sb->Dr[src[t]];
+  }
+  std::vector<double> stage((size_t)std::max<int64_t>(tot, 1));
+  auto put = [&](int64_t o, const double* p, int64_t cnt) {
+    if (cnt) {
+      NSS_REQUIRE(p != nullptr, NSS_E_ARG, "nss_sb_upload_blocks: NULL block with nonzero size");
+      std::memcpy(stage.data() + o, p, cnt * sizeof(double));
+    }
+  };
+  for (int q = 0; q < nq; ++q) {
+    put(oHL[q], HL[q], (int64_t)sb->Dl[q] * sb->Dl[q]);
+    put(oHR[q], HR[q], (int64_t)sb->Dr[q] * sb->Dr[q]);
+  }
+  for (int t = 0; t < nterms; ++t) {
+    put(oA[t], A[t], (int64_t)sb->Dl[dst[t]] * sb->Dl[src[t]]);
+    put(oB[t], B[t], (int64_t)sb->Dr[dst[t]] * sb->Dr[src[t]]);
+  }
+  sb->owned.resize(ctx->pool, stage.size());
+  NSS_HIP(hipMemcpyAsync(sb->owned.p, stage.data(), stage.size() * sizeof(double),
+                         hipMemcpyHostToDevice, ctx->stream));
+  const bool f32 = precision == NSS_FP32_MIXED;
+  if (f32) {
+    sb->owned32.resize(ctx->pool, stage.size());
+    launch_d2f(sb->owned.p, sb->owned32.p, (int64_t)stage.size(), ctx->stream);
+  }
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  auto ref = [&](int64_t o, int ld, int trans) {
+    OpRef r;
+    r.p = sb->owned.p + o;
+    r.p32 = f32 ? (const void*)(sb->owned32.p + o) : nullptr;
+    r.ld = ld;
+    r.trans = trans;
+    r.alpha = 1.0;
+    return r;
+  };
+  std::vector<OpRef> hl(nq), hr(nq);
+  for (int q = 0; q < nq; ++q) {
+    hl[q] = ref(oHL[q], sb->Dl[q], cm ? 1 : 0);
+    hr[q] = ref(oHR[q], sb->Dr[q], cm ? 1 : 0);
+  }
+  std::vector<SbTerm> terms(nterms);
+  for (int t = 0; t < nterms; ++t) {
+    const int dd = dst[t], ss = src[t];
+    terms[t].dst = dd;
+    terms[t].src = ss;
+    // A (Dl_dst x Dl_src): row-major ld = Dl_src; column-major = row-major A^T
+    terms[t].A = cm ? ref(oA[t], sb->Dl[dd], 1) : ref(oA[t], sb->Dl[ss], 0);
+    // B^T (Dr_src x Dr_dst): from row-major B (ld Dr_src) transposed, or
+    // column-major B read directly with ld Dr_dst
+    terms[t].Bt = cm ? ref(oB[t], sb->Dr[dd], 0) : ref(oB[t], sb->Dr[ss], 1);
+  }
+  sb_set_terms(ctx, sb, hl, hr, terms, f32);
+  NSS_API_END
+}
+
+static void hv_host(nss_ctx* ctx, const double* v, double* hv, int64_t n, bool f32) {
+  SbDevice* sb = ctx->sb;
+  NSS_REQUIRE(sb->n > 0 && !sb->Y64.htiles.empty(), NSS_E_STATE,
+              "nss_sb_hv: upload blocks before applying H");
+  NSS_REQUIRE(n == sb->n, NSS_E_ARG,
+              "nss_sb_hv: n=" + std::to_string(n) + " but the layout has " + std::to_string(sb->n));
+  NSS_HIP(hipSetDevice(ctx->device));
+  DevArr<double> x, y;
+  x.resize(ctx->pool, n);
+  y.resize(ctx->pool, n);
+  NSS_HIP(hipMemcpyAsync(x.p, v, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (f32) sb_hv_fp32(ctx, sb, x.p, y.p);
+  else sb_hv(ctx, sb, x.p, y.p);
+  NSS_HIP(hipMemcpyAsync(hv, y.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  nss_hv_timer_flush(ctx);
+}
+
+extern "C" int nss_sb_hv(nss_ctx* ctx, const double* v, double* hv, int64_t n) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && v && hv, NSS_E_ARG, "nss_sb_hv: NULL arguments");
+  hv_host(ctx, v, hv, n, false);
+  NSS_API_END
+}
+
+extern "C" int nss_sb_hv_fp32(nss_ctx* ctx, const double* v, double* hv, int64_t n) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && v && hv, NSS_E_ARG, "nss_sb_hv_fp32: NULL arguments");
+  hv_host(ctx, v, hv, n, true);
+  NSS_API_END
+}
+
+extern "C" int nss_sb_hv_device(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && d_v && d_hv, NSS_E_ARG, "nss_sb_hv_device: NULL arguments");
+  NSS_REQUIRE(n == ctx->sb->n && n > 0, NSS_E_ARG, "nss_sb_hv_device: n does not match the layout");
+  NSS_HIP(hipSetDevice(ctx->device));
+  sb_hv(ctx, ctx->sb, d_v, d_hv);
+  NSS_API_END
+}
+
+extern "C" int nss_ctx_hv_stats(nss_ctx* ctx, int reset, double* out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && out, NSS_E_ARG, "nss_ctx_hv_stats: NULL arguments");
+  nss_hv_timer_flush(ctx);
+  for (int k = 0; k < 2; ++k) {
+    out[6 * k + 0] = (double)ctx->st_calls[k];
+    out[6 * k + 1] = ctx->st_tw[k];
+    out[6 * k + 2] = ctx->st_ty[k];
+    out[6 * k + 3] = ctx->st_fw[k];
+    out[6 * k + 4] = ctx->st_fy[k];
+    out[6 * k + 5] = 0.0;
+    if (reset) {
+      ctx->st_calls[k] = 0;
+      ctx->st_tw[k] = ctx->st_ty[k] = ctx->st_fw[k] = ctx->st_fy[k] = 0.0;
+    }
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_sb_flops(nss_ctx* ctx, double* flops) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && flops, NSS_E_ARG, "nss_sb_flops: NULL arguments");
+  *flops = ctx->sb->flops;
+  NSS_API_END
+}
+
+extern "C" int nss_lanczos(nss_ctx* ctx, int neigen, int kmax, int maxiter, double tol, uint64_t seed,
+                           int v0_mode, int precision, double* evals, double* evecs, int* numop) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && evals, NSS_E_ARG, "nss_lanczos: NULL arguments");
+  SbDevice* sb = ctx->sb;
+  NSS_REQUIRE(sb->n > 0 && !sb->Y64.htiles.empty(), NSS_E_STATE, "nss_lanczos: no superblock uploaded");
+  NSS_REQUIRE(neigen >= 1 && neigen <= sb->n, NSS_E_ARG, "nss_lanczos: neigen out of range");
+  NSS_HIP(hipSetDevice(ctx->device));
+  DevArr<double> x;
+  x.resize(ctx->pool, (size_t)neigen * sb->n);
+  LanczosOut r = lanczos_device(ctx, sb, neigen, kmax, maxiter, tol, seed, v0_mode, precision, x.p);
+  for (int i = 0; i < neigen; ++i) evals[i] = r.evals[i];
+  if (evecs)
+    NSS_HIP(hipMemcpyAsync(evecs, x.p, (size_t)neigen * sb->n * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  if (numop) *numop = r.numop + r.numop_fp32;
+  NSS_API_END
+}
+
+extern "C" int nss_rdm_eigh(nss_ctx* ctx, const double* psi, int side, double* evals, double* evecs) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && psi && evals && evecs, NSS_E_ARG, "nss_rdm_eigh: NULL arguments");
+  NSS_REQUIRE(side == 0 || side == 1, NSS_E_ARG, "nss_rdm_eigh: side must be 0 (left) or 1 (right)");
+  SbDevice* sb = ctx->sb;
+  NSS_REQUIRE(sb->n > 0, NSS_E_STATE, "nss_rdm_eigh: set a layout first");
+  NSS_HIP(hipSetDevice(ctx->device));
+  DevArr<double> x;
+  x.resize(ctx->pool, sb->n);
+  NSS_HIP(hipMemcpyAsync(x.p, psi, sb->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<std::vector<double>> ev, vec;
+  sb_rdm_eigh(ctx, sb, x.p, side, ev, vec);
+  size_t oe = 0, ov = 0;
+  for (size_t q = 0; q < ev.size(); ++q) {
+    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+    if (ev[q].empty()) {  // empty sector partner: nothing (the reference skips it)
+      for (int i = 0; i < d; ++i) evals[oe++] = 0.0;
+      ov += (size_t)d * d;
+      continue;
+    }
+    std::memcpy(evals + oe, ev[q].data(), d * sizeof(double));
+    std::memcpy(evecs + ov, vec[q].data(), (size_t)d * d * sizeof(double));
+    oe += d;
+    ov += (size_t)d * d;
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_kept_count(const double* ev, int ne, int mstates, double estates, int mdim,
+                              double deg_thr, int* m_s, double* trunc_err) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ev && m_s && ne > 0, NSS_E_ARG, "nss_kept_count: bad arguments");
+  int ms;
+  if (mstates != 0) {
+    ms = std::min(mstates, std::min(mdim, ne));
+  } else if (estates != 0.0) {  // minloc(abs(1-cumulate(evals)-Estates)), DMRG_RDM.f90:208-209
+    double cum = 0.0, best = INFINITY;
+    ms = 1;
+    for (int i = 0; i < ne; ++i) {
+      cum += ev[i];
+      const double d = std::fabs(1.0 - cum - estates);
+      if (d < best) { best = d; ms = i + 1; }
+    }
+  } else {
+    throw Error(NSS_E_ARG, "Mdmrg=Edmrg=0 can not fix a threshold for the RDM");
+  }
+  NSS_REQUIRE(ms >= 1, NSS_E_ARG, "nss_kept_count: m_s < 1");
+  const double e_ = ev[ms - 1];
+  const int j_ = ms;
+  for (int i = j_; i < ne; ++i) {  // DMRG_RDM.f90:213-218
+    const double err = std::fabs(e_ - ev[i]) / e_;
+    if (err <= deg_thr) ++ms;
+  }
+  double s = 0.0;
+  for (int i = 0; i < ms; ++i) s += ev[i];
+  *m_s = ms;
+  if (trunc_err) *trunc_err = 1.0 - s;
+  NSS_API_END
+}
+
+extern "C" int nss_rotate_blocks(nss_ctx* ctx, int nblk, const int32_t* rows, const int32_t* cols,
+                                 const int32_t* mr, const int32_t* mc, const double* const* O,
+                                 const double* const* Urow, const double* const* Ucol,
+                                 double* const* out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && nblk >= 0, NSS_E_ARG, "nss_rotate_blocks: bad arguments");
+  if (nblk == 0) return NSS_OK;
+  NSS_REQUIRE(rows && cols && mr && mc && O && Urow && Ucol && out, NSS_E_ARG,
+              "nss_rotate_blocks: NULL arrays");
+  NSS_HIP(hipSetDevice(ctx->device));
+  std::vector<int64_t> oO(nblk), oR(nblk), oC(nblk), oT(nblk), oOut(nblk);
+  int64_t tin = 0, tmid = 0, tout = 0;
+  for (int b = 0; b < nblk; ++b) {
+    NSS_REQUIRE(rows[b] >= 0 && cols[b] >= 0 && mr[b] >= 0 && mc[b] >= 0, NSS_E_ARG,
+                "nss_rotate_blocks: negative dims");
+    oO[b] = tin; tin += (int64_t)rows[b] * cols[b];
+    oR[b] = tin; tin += (int64_t)rows[b] * mr[b];
+    oC[b] = tin; tin += (int64_t)cols[b] * mc[b];
+    oT[b] = tmid; tmid += (int64_t)rows[b] * mc[b];
+    oOut[b] = tout; tout += (int64_t)mr[b] * mc[b];
+  }
+  std::vector<double> stage(std::max<int64_t>(1, tin));
+  for (int b = 0; b < nblk; ++b) {
+    std::memcpy(stage.data() + oO[b], O[b], sizeof(double) * rows[b] * (size_t)cols[b]);
+    std::memcpy(stage.data() + oR[b], Urow[b], sizeof(double) * rows[b] * (size_t)mr[b]);
+    std::memcpy(stage.data() + oC[b], Ucol[b], sizeof(double) * cols[b] * (size_t)mc[b]);
+  }
+  DevArr<double> din, dmid, dout;
+  din.resize(ctx->pool, stage.size());
+  dmid.resize(ctx->pool, std::max<int64_t>(1, tmid));
+  dout.resize(ctx->pool, std::max<int64_t>(1, tout));
+  NSS_HIP(hipMemcpyAsync(din.p, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  GemmPlan p1, p2;
+  for (int b = 0; b < nblk; ++b) {  // T = O . Ucol ; out = Urow^T . T
+    p1.add_output(BASE_ABS, dmid.p + oT[b], 0, mc[b], rows[b], mc[b],
+                  {seg_abs(din.p + oO[b], cols[b], 0, din.p + oC[b], mc[b], 0, cols[b], 1.0)});
+    p2.add_output(BASE_ABS, dout.p + oOut[b], 0, mc[b], mr[b], mc[b],
+                  {seg_abs(din.p + oR[b], mr[b], 1, dmid.p + oT[b], mc[b], 0, rows[b], 1.0)});
+  }
+  p1.upload(ctx->pool, ctx->stream);
+  p2.upload(ctx->pool, ctx->stream);
+  launch_gemm_f64(p1, Bases{nullptr, nullptr, nullptr}, ctx->stream);
+  launch_gemm_f64(p2, Bases{nullptr, nullptr, nullptr}, ctx->stream);
+  std::vector<double> h(std::max<int64_t>(1, tout));
+  NSS_HIP(hipMemcpyAsync(h.data(), dout.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < nblk; ++b)
+    std::memcpy(out[b], h.data() + oOut[b], sizeof(double) * mr[b] * (size_t)mc[b]);
+  NSS_API_END
+}

@@ -36,7 +36,7 @@ EPS_LOG = 1e-32                    # src/DMRG_MAIN.f90:509 (epsi)
 
 # ---------------------------------------------------------------------------
 # Seeded RNG: SplitMix64 -> uniform[-1,1).  Same stream as the HIP library's
-# host-side start vector (nssdmrg_amd/csrc/nss_rng.h).
+# start vector (k_fill_splitmix in nssdmrg_amd/csrc/nss_kernels.hip).
 # ---------------------------------------------------------------------------
 _M64 = (1 << 64) - 1
 
@@ -500,7 +500,7 @@ def renormalize(b: Block, rho: RhoBlocks, mstates: int, deg_thr: float,
                 estates: float = 0.0):
     """renormalize_block + operators_renormalization, src/DMRG_RDM.f90:172-404.
 
-    Returns (new block, m_s, truncation error, kept-state QN list).
+    Returns (new block, m_s, truncation error, kept-state QN list, kept eigenvalues).
     """
     ms = kept_count(rho.evalues, mstates, b.dim, deg_thr, estates)
     trunc = 1.0 - float(np.sum(rho.evalues[:ms]))
@@ -511,7 +511,7 @@ def renormalize(b: Block, rho: RhoBlocks, mstates: int, deg_thr: float,
         u[rho.maps[ib], im] = rho.evecs_blk[ib][:, pos]
         newqn[im] = rho.qns[ib]
     ops = {k: u.T @ o @ u for k, o in b.ops.items()}
-    return Block(b.kind, b.length, newqn, ops, dict(b.optype)), ms, trunc, newqn
+    return Block(b.kind, b.length, newqn, ops, dict(b.optype)), ms, trunc, newqn, rho.evalues[:ms]
 
 
 def entropy(evals: np.ndarray) -> float:
@@ -559,6 +559,10 @@ class StepRecord:
     nsect: int
     nop: int
     hv_flops: float
+    kept_w_left: dict = None     # QN -> kept rho eigenvalues
+    kept_w_right: dict = None
+    cut_left: float = 0.0        # lambda_{m_s}
+    cut_right: float = 0.0
 
 
 class DMRG:
@@ -600,17 +604,20 @@ class DMRG:
         else:
             do_ren = not (self.left.length == p.ldmrg + 1 and label == 1)
         ml, mr, tl, tr = self.left.dim, self.right.dim, 0.0, 0.0
-        kl, kr = {}, {}
+        kl, kr, wl, wr, cl, cr = {}, {}, {}, {}, 0.0, 0.0
         if do_ren:
-            self.left, ml, tl, ql = renormalize(self.left, rl, ms, p.deg_thr)
-            self.right, mr, tr, qr = renormalize(self.right, rr, ms, p.deg_thr)
-            for q in map(tuple, ql):
+            self.left, ml, tl, ql, lw = renormalize(self.left, rl, ms, p.deg_thr)
+            self.right, mr, tr, qr, rw = renormalize(self.right, rr, ms, p.deg_thr)
+            for q, w in zip(map(tuple, ql), lw):
                 kl[q] = kl.get(q, 0) + 1
-            for q in map(tuple, qr):
+                wl.setdefault(q, []).append(float(w))
+            for q, w in zip(map(tuple, qr), rw):
                 kr[q] = kr.get(q, 0) + 1
+                wr.setdefault(q, []).append(float(w))
+            cl, cr = float(lw[-1]), float(rw[-1])
         rec = StepRecord(kind, self.left.length, self.right.length, L, evals,
                          evals / L / p.norb, entropy(rl.evalues), entropy(rr.evalues),
-                         ml, mr, tl, tr, kl, kr, msb, len(sb.qns), nop, hv_flops(sb))
+                         ml, mr, tl, tr, kl, kr, msb, len(sb.qns), nop, hv_flops(sb), wl, wr, cl, cr)
         self.records.append(rec)
         return rec
 

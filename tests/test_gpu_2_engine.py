@@ -1,0 +1,152 @@
+"""End-to-end GPU parity of the device-resident DMRG engine: the reference's
+golden trajectories, the oracle's kept-state counts per QN, finite sweeps,
+and the FP32-MFMA path with FP64 refinement (|dE0/L| <= 1e-9)."""
+import os
+
+import numpy as np
+import pytest
+
+import nssdmrg_amd as N
+from oracle import dmrg_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SPIN_HIJ = [[1.0, 0.0], [0.0, 0.5]]
+HUB_HIJ = [[-1.0, 0.0], [0.0, -1.0]]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, kind="i", **kw):
+    cfg = N.DMRGConfig(**kw)
+    d = N.DMRG(ctx, cfg)
+    steps = d.run(kind)
+    return d, steps
+
+
+# rho eigenvalues below this are rounding noise of rho's null space, whose
+# basis (and so which null states the kept-state rule picks) is arbitrary in
+# the reference too; parity of kept counts is asserted above it.
+W_MIN = 1e-10
+
+
+def _kept_oracle(rec, min_weight=0.0):
+    out = {}
+    for q, ws in rec.kept_w_left.items():
+        c = sum(1 for w in ws if w > min_weight)
+        if c:
+            out[(float(q[0]), float(q[1]) if len(q) > 1 else 0.0)] = c
+    return out
+
+
+def test_spin_golden_L40_M20(ctx, golden_dir):
+    ref = np.loadtxt(os.path.join(golden_dir, "spin12_L40_M20_energy.check"))
+    refS = np.loadtxt(os.path.join(golden_dir, "spin12_L40_M20_Sentropy.check"))
+    d, st = run(ctx, "i", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=40, mdmrg=20,
+                target_qn=(0.0, 0.0), neigen=2, lanc_dim_threshold=128)
+    assert len(st) == 39
+    for s, row in zip(st, ref):
+        assert s["left_len"] == int(row[0])
+        assert abs(s["e_per_site"][0] - row[1]) < 1e-10, (s["left_len"], s["e_per_site"][0] - row[1])
+        assert abs(s["e_per_site"][1] - row[2]) < 1e-10
+    for s, S in zip(st[:-1], refS[:-1]):
+        assert abs(s["entropy_left"] - S) < 1e-9
+    # kept-state counts per QN sector identical to the oracle's
+    p = O.Params(ldmrg=40, mdmrg=20, target_qn=(0.0,), hij=SPIN_HIJ, neigen=2, lanc_dim_threshold=128)
+    od = O.DMRG(O.spin_site(2), p)
+    od.infinite()
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert s["m_left"] == r.m_left and s["m_right"] == r.m_right
+        if s["renormalized"]:
+            assert d.kept_qn(i, "left") == _kept_oracle(r)
+            assert abs(s["trunc_left"] - r.trunc_left) < 1e-12
+
+
+def test_hubbard_golden_L40_M40(ctx, golden_dir):
+    ref = np.loadtxt(os.path.join(golden_dir, "hubbard_U0_L40_M40_energy.check"))
+    refS = np.loadtxt(os.path.join(golden_dir, "hubbard_U0_L40_M40_Sentropy.check"))
+    d, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=40, mdmrg=40,
+                target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=128, uloc=0.0, hfmode=True)
+    assert len(st) == 39
+    for s, e in zip(st, ref):
+        assert abs(s["e_per_site"][0] - e) < 1e-10, (s["left_len"], s["e_per_site"][0] - e)
+    for s, S in zip(st[:-1], refS[:-1]):
+        assert abs(s["entropy_left"] - S) < 1e-8
+
+
+def test_hubbard_U4_kept_counts_vs_oracle(ctx):
+    p = O.Params(ldmrg=12, mdmrg=32, target_qn=(0.5, 0.5), hij=HUB_HIJ, neigen=1, lanc_dim_threshold=128)
+    od = O.DMRG(O.electron_site(4.0, True), p)
+    od.infinite()
+    d, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=12, mdmrg=32,
+                target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=128, uloc=4.0, hfmode=True)
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert abs(s["energies"][0] - r.energies[0]) < 1e-10 * abs(r.energies[0])
+        assert abs(s["entropy_left"] - r.entropy_left) < 1e-9
+        assert s["m_sb"] == r.m_sb and s["nsect"] == r.nsect
+        if s["renormalized"]:
+            assert d.kept_qn(i, "left") == _kept_oracle(r)
+
+
+@pytest.mark.parametrize("model", ["spin12", "spin1", "hubbard"])
+def test_finite_sweeps_vs_oracle(ctx, model):
+    if model == "hubbard":
+        osite, site, hij, tq, U = O.electron_site(2.0, True), N.electron_site(), HUB_HIJ, (0.5, 0.5), 2.0
+        L, M, ms = 6, 24, [24, 32]
+    else:
+        sun = 2 if model == "spin12" else 3
+        osite, site, hij, tq, U = O.spin_site(sun), N.spin_site(sun), SPIN_HIJ, (0.0, 0.0), 0.0
+        L, M, ms = (8, 16, [16, 24]) if sun == 2 else (6, 20, [20, 28])
+    print(f"case finite {model}", flush=True)
+    p = O.Params(ldmrg=L, mdmrg=M, target_qn=tq[:2] if model == "hubbard" else (0.0,), hij=hij, neigen=1,
+                 lanc_dim_threshold=100, msweep=ms)
+    od = O.DMRG(osite, p)
+    od.finite()
+    d, st = run(ctx, "f", site=site, hij=np.array(hij), ldmrg=L, mdmrg=M, msweep=ms, target_qn=tq,
+                neigen=1, lanc_dim_threshold=100, uloc=U, hfmode=True)
+    assert len(st) == len(od.records)
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert s["type"] == r.kind and s["L"] == r.L
+        assert abs(s["energies"][0] - r.energies[0]) < 1e-10 * max(1.0, abs(r.energies[0])), i
+        if s["renormalized"]:
+            assert d.kept_qn(i, "left", W_MIN) == _kept_oracle(r, W_MIN), i
+            if min(s["cut_left"], r.cut_left) > W_MIN:
+                assert s["m_left"] == r.m_left, i
+
+
+def test_mixed_precision_refined_matches_fp64(ctx):
+    kw = dict(site=N.spin_site(3), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=48, msweep=[48],
+              target_qn=(0.0, 0.0), neigen=1, lanc_dim_threshold=128)
+    _, s64 = run(ctx, "f", **kw)
+    _, s32 = run(ctx, "f", precision="mixed", **kw)
+    assert len(s64) == len(s32)
+    for a, b in zip(s64, s32):
+        assert abs(a["e_per_site"][0] - b["e_per_site"][0]) <= 1e-9
+        if min(a["cut_left"], b["cut_left"]) > W_MIN:
+            assert a["m_left"] == b["m_left"]
+    assert sum(b["numop_fp32"] for b in s32) > 0
+
+
+def test_engine_superblock_spectrum_matches_oracle(ctx):
+    # the engine's last superblock (its own basis order) has the oracle's spectrum
+    p = O.Params(ldmrg=5, mdmrg=12, target_qn=(0.5, 0.5), hij=HUB_HIJ, neigen=1, lanc_dim_threshold=64)
+    od = O.DMRG(O.electron_site(3.0, True), p)
+    od.infinite()
+    d, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=5, mdmrg=12,
+                target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=64, uloc=3.0, hfmode=True)
+    ex = d.last_superblock()
+    sb = O.Superblock((0.0,), list(range(len(ex["Dl"]))), ex["Dl"].astype(np.int64), ex["Dr"].astype(np.int64),
+                      np.concatenate([[0], np.cumsum(ex["Dl"].astype(np.int64) * ex["Dr"])[:-1]]), [], [])
+    sb.HL, sb.HR = ex["HL"], ex["HR"]
+    sb.terms = [(int(a), int(b), A, B, 0) for a, b, A, B in zip(ex["dst"], ex["src"], ex["A"], ex["B"])]
+    e_eng = np.linalg.eigvalsh(O.sb_dense(sb))
+    e_ora = np.linalg.eigvalsh(O.sb_dense(od.last_sb))
+    assert np.allclose(e_eng[:20], e_ora[:20], atol=1e-10)
+    # and the engine's GPU plan applies exactly that matrix
+    v = O.splitmix64_uniform_fast(sb.n, 2)
+    assert np.linalg.norm(ctx.hv(v) - O.hv_direct(sb, v)) < 1e-12 * np.linalg.norm(O.hv_direct(sb, v))

@@ -232,3 +232,17 @@ def test_kept_count_rule():
     assert O.kept_count(tied, 2, 4, 0.1) == 3         # exact ties are never split
     # truncation error threshold mode (Estates): argmin |1-cumsum-E|
     assert O.kept_count(np.array([0.9, 0.09, 0.01]), 0, 3, 0.0, estates=0.01) == 2
+
+
+# ------------------------------------------------------ C restatement (baseline)
+@pytest.mark.parametrize("model", ["spin", "hubbard"])
+def test_c_restatement_matches_numpy_oracle(model):
+    from oracle import hv_ref
+    if model == "spin":
+        sb, _, _ = _sb_after(O.spin_site(2), SPIN_HIJ, 4, 16, (0.0,))
+    else:
+        sb, _, _ = _sb_after(O.electron_site(2.0, True), HUB_HIJ, 3, 24, (0.5, 0.5))
+    c = hv_ref.from_oracle(sb)
+    v = O.splitmix64_uniform_fast(sb.n, 4)
+    assert np.allclose(c.hv(v), O.hv_direct(sb, v), atol=1e-12)
+    assert np.allclose(c.hv(v), O.hv_direct_loops(sb, v), atol=1e-12)
