@@ -165,6 +165,7 @@ struct Engine {
   // last superblock (for export)
   Block lastL, lastR;
   std::vector<int> sbL, sbR;
+  std::vector<int> lastDl, lastDr;  // layout of the last superblock (engine-owned copy)
   std::vector<SbTerm> lastTerms;
   DevArr<double> psi;
   bool want32 = false;
@@ -388,6 +389,8 @@ struct Engine {
     NSS_REQUIRE(!sbL.empty(), NSS_E_STATE, "sb_get_states ERROR: total_states=0. There are no SB states.");
     const int nq = (int)sbL.size();
     sb_set_layout(ctx->sb, Dl, Dr, nullptr);
+    lastDl = Dl;
+    lastDr = Dr;
     std::map<QN, int> index;
     for (int k = 0; k < nq; ++k) index[L.sec[sbL[k]].q] = k;
     auto ref = [&](const BlockOp& op, const OpBlock& ob, int ld, int trans, double alpha) {
@@ -880,9 +883,11 @@ extern "C" int nss_dmrg_kept_qn(nss_dmrg* d, int i, int side, double min_weight,
 extern "C" int nss_dmrg_last_sb_dims(nss_dmrg* d, int* nq, int* nterms, int64_t* n) {
   NSS_API_BEGIN
   NSS_REQUIRE(d && d->e, NSS_E_ARG, "NULL handle");
-  if (nq) *nq = sb_nq(d->e->ctx->sb);
+  int64_t nn = 0;
+  for (size_t q = 0; q < d->e->lastDl.size(); ++q) nn += (int64_t)d->e->lastDl[q] * d->e->lastDr[q];
+  if (nq) *nq = (int)d->e->lastDl.size();
   if (nterms) *nterms = (int)d->e->lastTerms.size();
-  if (n) *n = sb_n(d->e->ctx->sb);
+  if (n) *n = nn;
   NSS_API_END
 }
 
@@ -893,10 +898,11 @@ extern "C" int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, in
   NSS_API_BEGIN
   NSS_REQUIRE(d && d->e, NSS_E_ARG, "NULL handle");
   Engine* E = d->e;
-  SbDevice* sb = E->ctx->sb;
-  const auto& dl = sb_Dl(sb);
-  const auto& dr = sb_Dr(sb);
+  const auto& dl = E->lastDl;
+  const auto& dr = E->lastDr;
   const int nq = (int)dl.size();
+  int64_t nn = 0;
+  for (int q = 0; q < nq; ++q) nn += (int64_t)dl[q] * dr[q];
   hipStream_t s = E->ctx->stream;
   auto fetch = [&](const OpRef& r, int rows, int cols, double* outp) {
     // r describes an operand (rows x cols) with ld/trans; alpha applied
@@ -940,7 +946,7 @@ extern "C" int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, in
     ob += (size_t)dr[T.dst] * dr[T.src];
   }
   if (psi) {
-    NSS_HIP(hipMemcpyAsync(psi, E->psi.p, sb_n(sb) * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipMemcpyAsync(psi, E->psi.p, nn * sizeof(double), hipMemcpyDeviceToHost, s));
     NSS_HIP(hipStreamSynchronize(s));
   }
   NSS_API_END

@@ -156,6 +156,10 @@ void launch_norm(const double* x, int64_t n, double* partial, double* out, hipSt
 // y[i] = x[i] * (1/den[0])
 void launch_div(const double* x, double* y, int64_t n, const double* den, hipStream_t s);
 
+// out[b] = max_{i<j} |a_ij - a_ji| of the d[b] x d[b] row-major block at a + off[b]
+void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, double* out,
+                     hipStream_t s);
+
 // Kronecker-scatter for block enlargement: dst sub-block (rows x cols, ld)
 // = sum of contributions scale * X (or X^T, or identity).
 struct KronContrib {

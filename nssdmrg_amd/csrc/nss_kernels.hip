@@ -316,6 +316,34 @@ void launch_div(const double* x, double* y, int64_t n, const double* den, hipStr
   NSS_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------ symmetry check
+__global__ __launch_bounds__(256) void k_asym_max(const double* __restrict__ a, const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ dim, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x;
+  const int d = dim[b];
+  const double* m = a + off[b];
+  double mx = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)d * d; e += 256) {
+    const int i = (int)(e / d), j = (int)(e % d);
+    if (j > i) mx = fmax(mx, fabs(m[(size_t)i * d + j] - m[(size_t)j * d + i]));
+  }
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = sh[0];
+}
+
+void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, double* out,
+                     hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_asym_max, dim3(nblk), dim3(256), 0, s, a, off, d, out);
+  NSS_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------ Kronecker scatter
 constexpr int KT = KRON_TILE;
 

@@ -10,9 +10,16 @@
 #include <cstdio>
 #include <mutex>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "nss_internal.h"
 
 namespace nss {
+
+// rho blocks up to this size are diagonalised on the host (Householder +
+// implicit QL); larger ones on the device with rocSOLVER dsyevd.
+constexpr int HOST_EIG_MAX = 48;
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -335,6 +342,10 @@ double nss_hv_timer_flush(nss_ctx* ctx) {
 void nss_release_rocblas(nss_ctx* ctx) {
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   ctx->ev.clear();
+  if (ctx->rocblas) {
+    (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
+    ctx->rocblas = nullptr;
+  }
 }
 
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
@@ -371,25 +382,92 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
   sb->R.upload(ctx->pool, ctx->stream);
   Bases b{psi_dev, nullptr, sb->rho.p};
   launch_gemm_f64(sb->R, b, ctx->stream, GEMM_GENERIC);
-  std::vector<double> h(tot);
+  // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the device
+  std::vector<int64_t> hoff;
+  std::vector<int32_t> hdim;
+  for (int q = 0; q < nq; ++q) {
+    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+    if (d > 1 && (int64_t)sb->Dl[q] * sb->Dr[q] > 0) {
+      hoff.push_back(roff[q]);
+      hdim.push_back(d);
+    }
+  }
+  DevArr<int64_t> doff;
+  DevArr<int32_t> ddim;
+  DevArr<double> dasym;
+  const int nchk = (int)hoff.size();
+  std::vector<double> asym(std::max(1, nchk), 0.0);
+  if (nchk) {
+    doff.resize(ctx->pool, nchk);
+    ddim.resize(ctx->pool, nchk);
+    dasym.resize(ctx->pool, nchk);
+    NSS_HIP(hipMemcpyAsync(doff.p, hoff.data(), nchk * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    NSS_HIP(hipMemcpyAsync(ddim.p, hdim.data(), nchk * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    launch_asym_max(sb->rho.p, doff.p, ddim.p, nchk, dasym.p, ctx->stream);
+    NSS_HIP(hipMemcpyAsync(asym.data(), dasym.p, nchk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  }
+  NSS_HIP(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < nchk; ++c)
+    if (!(asym[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
+  // large blocks: rocSOLVER dsyevd in place (eigenvectors overwrite rho)
+  std::vector<int64_t> eoff(nq, -1);
+  int64_t etot = 0;
+  for (int q = 0; q < nq; ++q) {
+    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+    if (d > HOST_EIG_MAX && (int64_t)sb->Dl[q] * sb->Dr[q] > 0) {
+      eoff[q] = etot;
+      etot += d;
+    }
+  }
+  DevArr<double> dD, dE;
+  DevArr<int32_t> dinfo;
+  if (etot) {
+    rocblas_handle hnd = static_cast<rocblas_handle>(ctx->rocblas);
+    if (!hnd) {
+      if (rocblas_create_handle(&hnd) != rocblas_status_success)
+        throw Error(NSS_E_HIP, "rocblas_create_handle failed");
+      ctx->rocblas = hnd;
+    }
+    if (rocblas_set_stream(hnd, ctx->stream) != rocblas_status_success)
+      throw Error(NSS_E_HIP, "rocblas_set_stream failed");
+    dD.resize(ctx->pool, etot);
+    dE.resize(ctx->pool, etot);
+    dinfo.resize(ctx->pool, nq);
+    for (int q = 0; q < nq; ++q) {
+      if (eoff[q] < 0) continue;
+      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+      const rocblas_status st = rocsolver_dsyevd(hnd, rocblas_evect_original, rocblas_fill_upper, d,
+                                                 sb->rho.p + roff[q], d, dD.p + eoff[q], dE.p + eoff[q],
+                                                 dinfo.p + q);
+      if (st != rocblas_status_success) throw Error(NSS_E_HIP, "rocsolver_dsyevd failed");
+    }
+  }
+  std::vector<double> h(tot), hD(std::max<int64_t>(1, etot));
+  std::vector<int32_t> hinfo(nq, 0);
   if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (etot) {
+    NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NSS_HIP(hipMemcpyAsync(hinfo.data(), dinfo.p, nq * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   evals.assign(nq, {});
   evecs.assign(nq, {});
   for (int q = 0; q < nq; ++q) {
     const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    if (d == 0 || (sb->Dl[q] * (int64_t)sb->Dr[q]) == 0) {
-      evals[q].assign(0, 0.0);
-      continue;
-    }
+    if (d == 0 || (sb->Dl[q] * (int64_t)sb->Dr[q]) == 0) continue;
     double* a = h.data() + roff[q];
-    for (int i = 0; i < d; ++i)
-      for (int j = i + 1; j < d; ++j)
-        if (std::fabs(a[(size_t)i * d + j] - a[(size_t)j * d + i]) > 1e-12)
-          throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
     evals[q].resize(d);
-    evecs[q].assign(a, a + (size_t)d * d);
-    sym_eig(d, evecs[q].data(), evals[q].data());
+    evecs[q].resize((size_t)d * d);
+    if (eoff[q] >= 0) {
+      if (hinfo[q] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
+      std::copy(hD.begin() + eoff[q], hD.begin() + eoff[q] + d, evals[q].begin());
+      // column-major eigenvectors -> row-major with eigenvectors as columns
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) evecs[q][(size_t)i * d + j] = a[(size_t)j * d + i];
+    } else {
+      std::copy(a, a + (size_t)d * d, evecs[q].begin());
+      sym_eig(d, evecs[q].data(), evals[q].data());
+    }
     for (double& e : evals[q])
       if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
   }

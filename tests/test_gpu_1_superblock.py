@@ -197,3 +197,22 @@ def test_hv_stats_count_calls(ctx):
     s = ctx.hv_stats(reset=True)
     assert s["calls"] == 3 and s["t_w"] > 0 and s["t_y"] > 0
     assert abs(s["flops"] - 3 * O.hv_flops(sb)) < 1e-6 * s["flops"]
+
+
+def test_rdm_eigh_large_blocks(ctx):
+    # sector dims above the host-eigensolver cut (rocSOLVER path) and below it
+    dims = [(200, 17), (97, 250), (40, 40), (130, 130), (1, 300)]
+    sb = _random_sb(4, dims, 6)
+    upload(ctx, sb)
+    psi = O.splitmix64_uniform_fast(sb.n, 21)
+    psi /= np.linalg.norm(psi)
+    for side in ("left", "right"):
+        evs, vecs = ctx.rdm_eigh(psi, side)
+        for k, (ev, vec) in enumerate(zip(evs, vecs)):
+            dl, dr, off = sb.Dl[k], sb.Dr[k], sb.offset[k]
+            p = psi[off:off + dl * dr].reshape(dl, dr)
+            rho = p @ p.T if side == "left" else p.T @ p
+            ref = np.linalg.eigvalsh(rho)
+            assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-15), (side, k)
+            assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14
+            assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12
