@@ -153,6 +153,8 @@ def main():
         "time_split_s": {k: float(sum(s[k] for s in steps)) for k in
                          ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
     }
+    if os.environ.get("NSS_PROFILE") == "1":
+        out["phase_timers_s"] = ctx.timers()
     if not a.no_mixed:
         print("[bench] FP32-MFMA path", flush=True)
         d32, r32 = run_path(N, ctx, w, a.warmup, a.steps, "mixed", a.verbose)

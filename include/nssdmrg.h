@@ -82,6 +82,10 @@ int nss_sb_hv_device(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n);
  * seconds in the Y launch (per-sector sums), algorithmic FLOPs of the W
  * launches, FLOPs of the Y launches, 0}; reset clears the counters. */
 int nss_ctx_hv_stats(nss_ctx* ctx, int reset, double* out);
+/* Phase timers collected when the environment sets NSS_PROFILE=1 (each
+ * phase ends with a stream synchronisation): a JSON object of seconds per
+ * phase name written to buf (truncated to len). */
+int nss_ctx_timers(nss_ctx* ctx, char* buf, int len);
 /* FP32-MFMA H*v (FP32 operands, FP32 accumulation) on host arrays. */
 int nss_sb_hv_fp32(nss_ctx* ctx, const double* v, double* hv, int64_t n);
 /* Algorithmic FLOPs of one H*v with dense sector blocks (reference order). */

@@ -112,6 +112,13 @@ class Context:
         r["flops32"] = r["flops_w32"] + r["flops_y32"]
         return r
 
+    def timers(self) -> dict:
+        """Phase timers (seconds) collected when NSS_PROFILE=1."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        F.check(F.lib().nss_ctx_timers(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode() or "{}")
+
     def lanczos(self, neigen=1, kmax=150, maxiter=512, tol=1e-12, seed=0, v0_mode=0, precision="fp64"):
         ev = np.zeros(neigen)
         vecs = np.zeros((neigen, self.n))

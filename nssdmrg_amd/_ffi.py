@@ -72,6 +72,7 @@ SIGNATURES = {
     "nss_sb_hv": (C.c_int, [_P, _D, _D, C.c_int64]),
     "nss_sb_hv_device": (C.c_int, [_P, _P, _P, C.c_int64]),
     "nss_ctx_hv_stats": (C.c_int, [_P, C.c_int, _D]),
+    "nss_ctx_timers": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "nss_sb_hv_fp32": (C.c_int, [_P, _D, _D, C.c_int64]),
     "nss_sb_flops": (C.c_int, [_P, _D]),
     "nss_lanczos": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,

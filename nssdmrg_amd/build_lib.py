@@ -52,5 +52,38 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+FORTRAN_SRC = os.path.join(HERE, "fortran", "nss_dmrg.f90")
+FORTRAN_OBJ = os.path.join(HERE, "fortran", "_build")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compiles the Fortran binding module (nss_dmrg) and the Fortran test
+    driver tests/fortran/spin_chain_golden.f90 with flang, linked against
+    libnssdmrg.so through an $ORIGIN-relative rpath.  Returns the driver
+    path, or None when flang is not installed."""
+    if not os.path.exists(FLANG):
+        return None
+    root = os.path.dirname(HERE)
+    tdir = os.path.join(root, "tests", "fortran")
+    tbuild = os.path.join(tdir, "_build")
+    os.makedirs(FORTRAN_OBJ, exist_ok=True)
+    os.makedirs(tbuild, exist_ok=True)
+    obj = os.path.join(FORTRAN_OBJ, "nss_dmrg.o")
+    r = subprocess.run([FLANG, "-O2", "-c", f"-J{FORTRAN_OBJ}", FORTRAN_SRC, "-o", obj], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed on {FORTRAN_SRC}:\n{r.stdout}{r.stderr}")
+    exe = os.path.join(tbuild, "spin_chain_golden")
+    rpath = "-Wl,-rpath,$ORIGIN/" + os.path.relpath(HERE, tbuild)
+    r = subprocess.run([FLANG, "-O2", f"-I{FORTRAN_OBJ}", os.path.join(tdir, "spin_chain_golden.f90"), obj,
+                        "-o", exe, f"-L{HERE}", "-lnssdmrg", rpath], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed on the Fortran driver:\n{r.stdout}{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran(verbose=True)

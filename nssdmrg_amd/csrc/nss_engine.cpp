@@ -635,8 +635,10 @@ struct Engine {
     I.type = type;
     I.label = label;
     I.sweep = sweep + (type == 'f' ? 1 : 0);
+    PhaseTimer pt(ctx, "step.enlarge");
     left = enlarge(left, true);
     right = enlarge(right, false);
+    pt.next("step.setup");
     const auto t1 = clk::now();
     I.t_enlarge = sec(t0, t1);
     I.left_len = left.length;
@@ -652,6 +654,7 @@ struct Engine {
     I.neigen = neig;
     psi.resize(ctx->pool, (size_t)neig * msb);
     LanczosOut lo;
+    pt.next("step.diag");
     const auto t2 = clk::now();
     if (neig == msb || msb <= par.lanc_dim_threshold) {
       lo = dense_diag(ctx, ctx->sb, neig, psi.p);
@@ -666,9 +669,11 @@ struct Engine {
     I.t_diag = sec(t2, t3);
     for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = lo.evals[i];
     // sb_get_rdm
+    pt.next("step.rdm");
     std::vector<std::vector<double>> evl, vl, evr, vr;
     sb_rdm_eigh(ctx, ctx->sb, psi.p, 0, evl, vl);
     sb_rdm_eigh(ctx, ctx->sb, psi.p, 1, evr, vr);
+    pt.next("step.spectrum");
     Spectrum sl = sort_spectrum(std::move(evl), std::move(vl));
     Spectrum sr = sort_spectrum(std::move(evr), std::move(vr));
     const auto t4 = clk::now();
@@ -683,6 +688,7 @@ struct Engine {
     I.m_right = right.dim();
     lastL = left;
     lastR = right;
+    pt.next("step.renorm");
     if (ren) {
       int ml = 0, mr = 0;
       double tl = 0.0, tr = 0.0;
@@ -695,6 +701,7 @@ struct Engine {
       I.trunc_left = tl;
       I.trunc_right = tr;
     }
+    pt.next("step.tail");
     const auto t5 = clk::now();
     I.t_renorm = sec(t4, t5);
     I.t_hv = nss_hv_timer_flush(ctx);

@@ -151,10 +151,39 @@ void launch_gemv_t(const double* V, int64_t ldv, int ncols, const double* w, int
 // w -= V h  (or y = V h when overwrite)
 void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, double* w, int64_t n,
                    int overwrite, hipStream_t s);
+// One classical Gram-Schmidt pass against V[0..m): h = V^T w (to hout,
+// device), w -= V h; with sq != nullptr, per-block sums of squares of the new
+// w are left in sq[0..orth_blocks(n)).  No host synchronisation.
+void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
+                      double* hout, double* sq, hipStream_t s);
+int orth_blocks(int64_t n);
+int64_t orth_chunk(int64_t n);
+// Lanczos three-term step on the device: w -= beta_prev v_prev (when
+// beta_prev != null), alpha = <v_j, w>, w -= alpha v_j; alpha to alpha_out,
+// beta_prev copied to beta_out.
+void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int64_t n, const double* beta_prev,
+                          double* beta_out, double* alpha_out, double* partial, hipStream_t s);
+// v = w / beta with beta = sqrt(sum sq) written to bout (device).
+void launch_normalize(const double* w, double* v, int64_t n, const double* sq, double* bout, hipStream_t s);
 // out[0] = sqrt(<x,x>)
 void launch_norm(const double* x, int64_t n, double* partial, double* out, hipStream_t s);
 // y[i] = x[i] * (1/den[0])
 void launch_div(const double* x, double* y, int64_t n, const double* den, hipStream_t s);
+
+// Batched Jacobi eigen-solver for symmetric blocks of dimension <= nmax
+// (nmax <= jacobi_max_dim): block b (dim[b] x dim[b] row-major at a+off[b])
+// -> eigenvectors as columns at vout+off[b], eigenvalues (unsorted) at
+// wout+eoff[b].
+int jacobi_max_dim(int device);
+void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
+                       double* wout, int nblk, int nmax, hipStream_t s);
+
+// Padding of blocks into ncls x ncls slabs for a strided-batched dense
+// eigensolve, and the inverse copy of the eigenpairs (see nss_kernels.hip).
+void launch_pad_blocks(const double* src, const int64_t* off, const int32_t* dim, int nb, int ncls, double* dst,
+                       hipStream_t s);
+void launch_unpad_eig(const double* vecs, const double* vals, int ncls, const int64_t* off, const int32_t* dim,
+                      const int64_t* eoff, int nb, double* dst, double* w, hipStream_t s);
 
 // out[b] = max_{i<j} |a_ij - a_ji| of the d[b] x d[b] row-major block at a + off[b]
 void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, double* out,
@@ -207,7 +236,24 @@ struct nss_ctx {
   // engines created on this context: destroyed (their device memory
   // returned) before the context's pool goes away
   std::vector<struct nss_dmrg*> engines;
+  // phase timers (NSS_PROFILE=1): host wall time with a stream sync at the
+  // end of each timed phase; off by default (no extra syncs)
+  bool prof = false;
+  std::map<std::string, double> timers;
 };
+
+namespace nss {
+// Accumulates wall time into ctx->timers[phase] when profiling: a phase runs
+// from construction or next() to the following next() or destruction.
+struct PhaseTimer {
+  ::nss_ctx* c;
+  const char* name;
+  double t0;
+  PhaseTimer(::nss_ctx* ctx, const char* n);
+  void next(const char* n);
+  ~PhaseTimer();
+};
+}  // namespace nss
 
 namespace nss {
 void nss_release_rocblas(::nss_ctx* ctx);

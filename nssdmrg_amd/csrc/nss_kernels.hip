@@ -44,9 +44,44 @@ __device__ __forceinline__ const T* resolve_c(int base, const void* abs, int64_t
 }
 
 template <typename T>
+struct SliceRegs {  // one K-slice of A and B held in registers while the previous slice computes
+  T a[8], b[8];
+};
+
+// Loads the K-slice [k0, k0+32) of segment g into registers (zero-padded).
+template <typename T>
+__device__ __forceinline__ void load_slice(const GemmSeg& g, const GemmTile& tl, const Bases& bs, int k0, int tid,
+                                           SliceRegs<T>& r) {
+  constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = 32;
+  const T* A = resolve_c<T>(g.abase, g.a, g.aoff, bs);
+  const T* B = resolve_c<T>(g.bbase, g.b, g.boff, bs);
+  const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
+#pragma unroll
+  for (int e = 0; e < (BM * BK) / 256; ++e) {
+    const int idx = tid + e * 256;
+    int i, k;
+    if (!ta) { i = idx / BK; k = idx % BK; } else { k = idx / BM; i = idx % BM; }
+    const int gi = tl.row0 + i, gk = k0 + k;
+    T v = T(0);
+    if (gi < tl.M && gk < g.K) v = ta ? A[(size_t)gk * g.lda + gi] : A[(size_t)gi * g.lda + gk];
+    r.a[e] = v;
+  }
+#pragma unroll
+  for (int e = 0; e < (BN * BK) / 256; ++e) {
+    const int idx = tid + e * 256;
+    int j, k;
+    if (!tb) { k = idx / BN; j = idx % BN; } else { j = idx / BK; k = idx % BK; }
+    const int gj = tl.col0 + j, gk = k0 + k;
+    T v = T(0);
+    if (gj < tl.N && gk < g.K) v = tb ? B[(size_t)gj * g.ldb + gk] : B[(size_t)gk * g.ldb + gj];
+    r.b[e] = v;
+  }
+}
+
+template <typename T>
 __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
                                           const Bases& bs) {
-  constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = 16;
+  constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = 32;
   constexpr int PAD = 1;
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
@@ -67,49 +102,54 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
-  for (int s = tl.seg0; s < tl.seg0 + tl.nseg; ++s) {
-    const GemmSeg g = segs[s];
-    const T* A = resolve_c<T>(g.abase, g.a, g.aoff, bs);
-    const T* B = resolve_c<T>(g.bbase, g.b, g.boff, bs);
+  const int send = tl.seg0 + tl.nseg;
+  int s = tl.seg0, k0 = 0;
+  GemmSeg g;
+  SliceRegs<T> r;
+  if (s < send) {
+    g = segs[s];
+    load_slice<T>(g, tl, bs, 0, tid, r);
+  }
+  while (s < send) {
+    // registers -> LDS (alpha applied to A), layout As[k][i], Bs[k][j]
     const T alpha = static_cast<T>(g.alpha);
     const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
 #pragma unroll
-      for (int e = 0; e < (BM * BK) / 256; ++e) {
-        const int idx = tid + e * 256;
-        int i, k;
-        if (!ta) { i = idx / BK; k = idx % BK; } else { k = idx / BM; i = idx % BM; }
-        const int gi = tl.row0 + i, gk = k0 + k;
-        T v = T(0);
-        if (gi < tl.M && gk < g.K)
-          v = ta ? A[(size_t)gk * g.lda + gi] : A[(size_t)gi * g.lda + gk];
-        As[k][i] = alpha * v;
-      }
-#pragma unroll
-      for (int e = 0; e < (BN * BK) / 256; ++e) {
-        const int idx = tid + e * 256;
-        int j, k;
-        if (!tb) { k = idx / BN; j = idx % BN; } else { j = idx / BK; k = idx % BK; }
-        const int gj = tl.col0 + j, gk = k0 + k;
-        T v = T(0);
-        if (gj < tl.N && gk < g.K)
-          v = tb ? B[(size_t)gj * g.ldb + gk] : B[(size_t)gk * g.ldb + gj];
-        Bs[k][j] = v;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 4) {
-        const T a0 = As[kk + fk][wm + fr];
-        const T a1 = As[kk + fk][wm + 16 + fr];
-        const T b0 = Bs[kk + fk][wn + fr];
-        const T b1 = Bs[kk + fk][wn + 16 + fr];
-        acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
-        acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
-        acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
-        acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
-      }
-      __syncthreads();
+    for (int e = 0; e < (BM * BK) / 256; ++e) {
+      const int idx = tid + e * 256;
+      int i, k;
+      if (!ta) { i = idx / BK; k = idx % BK; } else { k = idx / BM; i = idx % BM; }
+      As[k][i] = alpha * r.a[e];
     }
+#pragma unroll
+    for (int e = 0; e < (BN * BK) / 256; ++e) {
+      const int idx = tid + e * 256;
+      int j, k;
+      if (!tb) { k = idx / BN; j = idx % BN; } else { j = idx / BK; k = idx % BK; }
+      Bs[k][j] = r.b[e];
+    }
+    __syncthreads();
+    // next slice (possibly of the next segment): loads in flight during the MFMAs
+    int ns = s, nk0 = k0 + BK;
+    if (nk0 >= g.K) { ns = s + 1; nk0 = 0; }
+    if (ns < send) {
+      if (ns != s) g = segs[ns];
+      load_slice<T>(g, tl, bs, nk0, tid, r);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const T a0 = As[kk + fk][wm + fr];
+      const T a1 = As[kk + fk][wm + 16 + fr];
+      const T b0 = Bs[kk + fk][wn + fr];
+      const T b1 = Bs[kk + fk][wn + 16 + fr];
+      acc[0][0] = Mfma<T>::mma(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma<T>::mma(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma<T>::mma(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma<T>::mma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+    s = ns;
+    k0 = nk0;
   }
   T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
 #pragma unroll
@@ -117,14 +157,14 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
 #pragma unroll
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // accumulator item r of lane l holds D[row][l%16] with
-        // row = l/16 + 4r (f64 16x16x4) or 4*(l/16) + r (f32 16x16x4)
-        const int row = tl.row0 + wm + mi * 16 + (sizeof(T) == 8 ? fk + 4 * r : fk * 4 + r);
+      for (int q = 0; q < 4; ++q) {
+        // accumulator item q of lane l holds D[row][l%16] with
+        // row = l/16 + 4q (f64 16x16x4) or 4*(l/16) + q (f32 16x16x4)
+        const int row = tl.row0 + wm + mi * 16 + (sizeof(T) == 8 ? fk + 4 * q : fk * 4 + q);
         const int col = tl.col0 + wn + ni * 16 + fr;
         if (row < tl.M && col < tl.N) {
           T* p = C + (size_t)row * tl.ldc + col;
-          *p = tl.beta ? *p + acc[mi][ni][r] : acc[mi][ni][r];
+          *p = tl.beta ? *p + acc[mi][ni][q] : acc[mi][ni][q];
         }
       }
 }
@@ -275,6 +315,157 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
   NSS_HIP(hipGetLastError());
 }
 
+// ---- Lanczos iteration kernels (no host round trip per iteration)
+// All reductions are fixed-order: per-chunk partials, then a block-cooperative
+// sum (strided over 256 threads, then wave shuffles and LDS) whose order does
+// not depend on timing, so every run is bit-reproducible.
+constexpr int64_t LZ_CHUNK = 2048;  // elements per partial-sum chunk
+
+__device__ __forceinline__ double coop_sum(const double* __restrict__ p, int64_t cnt, double* sh) {
+  double a = 0.0;
+  for (int64_t c = threadIdx.x; c < cnt; c += 256) a += p[c];
+  return block_sum256(a, sh);  // valid on thread 0
+}
+
+int64_t lz_nch(int64_t n) { return (n + LZ_CHUNK - 1) / LZ_CHUNK; }
+int orth_blocks(int64_t n) { return (int)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256)); }
+int64_t orth_chunk(int64_t) { return LZ_CHUNK; }
+
+// w -= beta_{j-1} v_{j-1} (when beta_prev != null); partial[c] = <v_j, w> over chunk c;
+// block 0 copies beta_{j-1} to beta_out.
+__global__ __launch_bounds__(256) void k_lz_alpha(double* __restrict__ w, const double* __restrict__ vprev,
+                                                  const double* __restrict__ vj, int64_t n,
+                                                  const double* __restrict__ beta_prev, double* __restrict__ partial,
+                                                  double* __restrict__ beta_out) {
+  __shared__ double sh[4];
+  const int64_t c = blockIdx.x;
+  const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
+  const double b = beta_prev ? beta_prev[0] : 0.0;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    double x = w[i];
+    if (beta_prev) {
+      x -= b * vprev[i];
+      w[i] = x;
+    }
+    acc += vj[i] * x;
+  }
+  const double r = block_sum256(acc, sh);
+  if (threadIdx.x == 0) {
+    partial[c] = r;
+    if (c == 0 && beta_out) beta_out[0] = b;
+  }
+}
+
+// alpha = sum_c partial[c]; w -= alpha v_j; block 0 stores alpha.
+__global__ __launch_bounds__(256) void k_lz_sub(double* __restrict__ w, const double* __restrict__ vj, int64_t n,
+                                                const double* __restrict__ partial, int64_t nch,
+                                                double* __restrict__ alpha_out) {
+  __shared__ double sh[4];
+  __shared__ double ash;
+  const double a0 = coop_sum(partial, nch, sh);
+  if (threadIdx.x == 0) {
+    ash = a0;
+    if (blockIdx.x == 0) alpha_out[0] = a0;
+  }
+  __syncthreads();
+  const double a = ash;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) w[i] -= a * vj[i];
+}
+
+// partial[j*nch + c] = <V_j, w> over chunk c
+__global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv,
+                                                  const double* __restrict__ w, int64_t n,
+                                                  double* __restrict__ partial, int64_t nch) {
+  __shared__ double sh[4];
+  const int64_t c = blockIdx.x;
+  const int j = blockIdx.y;
+  const double* v = V + (size_t)j * ldv;
+  const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * w[i];
+  const double r = block_sum256(acc, sh);
+  if (threadIdx.x == 0) partial[(size_t)j * nch + c] = r;
+}
+
+// h[j] = sum_c partial[j*nch + c]   (one block per j)
+__global__ __launch_bounds__(256) void k_lz_hreduce(const double* __restrict__ partial, int64_t nch,
+                                                    double* __restrict__ h) {
+  __shared__ double sh[4];
+  const int j = blockIdx.x;
+  const double r = coop_sum(partial + (size_t)j * nch, nch, sh);
+  if (threadIdx.x == 0) h[j] = r;
+}
+
+// w -= V[0..m) h; sq[block] = sum of squares of the block's new rows
+__global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ V, int64_t ldv, int m,
+                                                     const double* __restrict__ h, double* __restrict__ w,
+                                                     int64_t n, double* __restrict__ sq) {
+  extern __shared__ double hs[];
+  __shared__ double red[4];
+  for (int j = threadIdx.x; j < m; j += 256) hs[j] = h[j];
+  __syncthreads();
+  double ss = 0.0;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int j = 0;
+    for (; j + 4 <= m; j += 4) {
+      a0 += V[(size_t)j * ldv + i] * hs[j];
+      a1 += V[(size_t)(j + 1) * ldv + i] * hs[j + 1];
+      a2 += V[(size_t)(j + 2) * ldv + i] * hs[j + 2];
+      a3 += V[(size_t)(j + 3) * ldv + i] * hs[j + 3];
+    }
+    for (; j < m; ++j) a0 += V[(size_t)j * ldv + i] * hs[j];
+    const double x = w[i] - ((a0 + a1) + (a2 + a3));
+    w[i] = x;
+    ss += x * x;
+  }
+  const double r = block_sum256(ss, red);
+  if (threadIdx.x == 0) sq[blockIdx.x] = r;
+}
+
+// beta = sqrt(sum sq[0..nb)); v = w / beta; block 0 stores beta.
+__global__ __launch_bounds__(256) void k_normalize(const double* __restrict__ w, double* __restrict__ v,
+                                                   int64_t n, const double* __restrict__ sq, int nb,
+                                                   double* __restrict__ bout) {
+  __shared__ double sh[4];
+  __shared__ double bsh;
+  const double ssum = coop_sum(sq, nb, sh);
+  if (threadIdx.x == 0) {
+    bsh = sqrt(ssum);
+    if (blockIdx.x == 0) bout[0] = bsh;
+  }
+  __syncthreads();
+  const double inv = 1.0 / bsh;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v[i] = w[i] * inv;
+}
+
+static unsigned wide_grid(int64_t n) { return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256)); }
+
+void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int64_t n, const double* beta_prev,
+                          double* beta_out, double* alpha_out, double* partial, hipStream_t s) {
+  const int64_t nch = lz_nch(n);
+  hipLaunchKernelGGL(k_lz_alpha, dim3((unsigned)nch), dim3(256), 0, s, w, vprev, vj, n, beta_prev, partial,
+                     beta_out);
+  hipLaunchKernelGGL(k_lz_sub, dim3(wide_grid(n)), dim3(256), 0, s, w, vj, n, partial, nch, alpha_out);
+  NSS_HIP(hipGetLastError());
+}
+
+void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
+                      double* hout, double* sq, hipStream_t s) {
+  const int64_t nch = lz_nch(n);
+  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch);
+  hipLaunchKernelGGL(k_lz_hreduce, dim3((unsigned)m), dim3(256), 0, s, partial, nch, hout);
+  hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
+                     ldv, m, hout, w, n, sq);
+  NSS_HIP(hipGetLastError());
+}
+
+void launch_normalize(const double* w, double* v, int64_t n, const double* sq, double* bout, hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize, dim3(wide_grid(n)), dim3(256), 0, s, w, v, n, sq, orth_blocks(n), bout);
+  NSS_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ x, int64_t n,
                                                        double* __restrict__ partial) {
   __shared__ double sh[4];
@@ -313,6 +504,202 @@ void launch_f2d(const float* x, double* y, int64_t n, hipStream_t s) {
 }
 void launch_div(const double* x, double* y, int64_t n, const double* den, hipStream_t s) {
   hipLaunchKernelGGL(k_div, dim3(grid_for(n)), dim3(256), 0, s, x, y, n, den);
+  NSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ batched Jacobi
+// Cyclic two-sided Jacobi on each symmetric block (one 512-thread workgroup
+// per block, the block A and V^T resident in LDS).  A round of the
+// round-robin tournament pairs all indices into n/2 disjoint rotations; each
+// 2x2 sub-block A[{p,q},{r,s}] is then updated once as J_pq^T B J_rs (rows
+// and columns in one pass), and rows p,q of V^T rotate with J_pq.  Rotation
+// angles: Golub & Van Loan, Algorithm 8.4.1 (symSchur2).  Output:
+// eigenvalues (unsorted) and eigenvectors as columns, row-major.
+constexpr int JAC_THREADS = 512;
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = JAC_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __restrict__ a,
+                                                            const int64_t* __restrict__ off,
+                                                            const int32_t* __restrict__ dim,
+                                                            const int64_t* __restrict__ eoff,
+                                                            double* __restrict__ vout, double* __restrict__ wout,
+                                                            int max_sweeps) {
+  extern __shared__ double sm[];
+  __shared__ double cs_c[64], cs_s[64];
+  __shared__ int pr_p[64], pr_q[64];
+  __shared__ double red[JAC_THREADS];
+  const int b = blockIdx.x;
+  const int n = dim[b];
+  const int np = n + (n & 1);  // even number of slots (one dummy when n is odd)
+  const int ld = np + 1;
+  const int half = np / 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = JAC_THREADS / 64;
+  double* A = sm;
+  double* Vt = sm + (size_t)np * ld;
+  const double* src = a + off[b];
+  for (int i = wave; i < np; i += nwave)
+    for (int j = lane; j < np; j += 64) {
+      A[i * ld + j] = (i < n && j < n) ? src[(size_t)i * n + j] : 0.0;
+      Vt[i * ld + j] = (i == j) ? 1.0 : 0.0;
+    }
+  __syncthreads();
+  double f = 0.0;
+  for (int i = wave; i < np; i += nwave)
+    for (int j = lane; j < np; j += 64) f += A[i * ld + j] * A[i * ld + j];
+  red[threadIdx.x] = f;
+  __syncthreads();
+  for (int st = JAC_THREADS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  // rotations are skipped, and the sweeps stop, once every off-diagonal
+  // element is below 1e-15 ||A||_F: a few ulps of ||A||, above the rounding
+  // noise that rotations leave behind, so the absolute accuracy of the
+  // eigenvalues is ~eps*||A|| as LAPACK gives for a PSD matrix
+  const double thr = 1e-15 * sqrt(red[0]);
+  __syncthreads();
+  for (int sw = 0; sw < max_sweeps; ++sw) {
+    double o = 0.0;
+    for (int i = wave; i < np; i += nwave)
+      for (int j = lane; j < np; j += 64)
+        if (i != j) o = fmax(o, fabs(A[i * ld + j]));
+    if (!(block_max(o, red) > thr)) break;
+    for (int r = 0; r < np - 1; ++r) {
+      if ((int)threadIdx.x < half) {
+        const int i = threadIdx.x;
+        const int k1 = np - 1 - i;
+        int p = (i == 0) ? 0 : ((i - 1 + r) % (np - 1)) + 1;
+        int q = ((k1 - 1 + r) % (np - 1)) + 1;
+        if (p > q) { const int t = p; p = q; q = t; }
+        double c = 1.0, sn = 0.0;
+        const double apq = A[p * ld + q];
+        if (q < n && fabs(apq) > thr) {
+          const double tau = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
+          const double t = (tau >= 0.0) ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          sn = t * c;
+        }
+        cs_c[i] = c;
+        cs_s[i] = sn;
+        pr_p[i] = p;
+        pr_q[i] = q;
+      }
+      __syncthreads();
+      // A <- J^T A J, one 2x2 sub-block per item
+      for (int i = wave; i < half; i += nwave) {
+        const int p = pr_p[i], q = pr_q[i];
+        const double ci = cs_c[i], si = cs_s[i];
+        for (int j = lane; j < half; j += 64) {
+          const int rr = pr_p[j], ss = pr_q[j];
+          const double cj = cs_c[j], sj = cs_s[j];
+          const double b00 = A[p * ld + rr], b01 = A[p * ld + ss], b10 = A[q * ld + rr], b11 = A[q * ld + ss];
+          const double t00 = ci * b00 - si * b10, t01 = ci * b01 - si * b11;
+          const double t10 = si * b00 + ci * b10, t11 = si * b01 + ci * b11;
+          double n01 = sj * t00 + cj * t01, n10 = cj * t10 - sj * t11;
+          if (i == j && si != 0.0) n01 = n10 = 0.0;  // the rotated pair is annihilated
+          A[p * ld + rr] = cj * t00 - sj * t01;
+          A[p * ld + ss] = n01;
+          A[q * ld + rr] = n10;
+          A[q * ld + ss] = sj * t10 + cj * t11;
+        }
+        // V <- V J  (rows p, q of V^T)
+        if (si != 0.0)
+          for (int k = lane; k < np; k += 64) {
+            const double vp = Vt[p * ld + k], vq = Vt[q * ld + k];
+            Vt[p * ld + k] = ci * vp - si * vq;
+            Vt[q * ld + k] = si * vp + ci * vq;
+          }
+      }
+      __syncthreads();
+    }
+  }
+  double* vo = vout + off[b];
+  for (int i = wave; i < n; i += nwave)
+    for (int j = lane; j < n; j += 64) vo[(size_t)i * n + j] = Vt[j * ld + i];
+  for (int i = threadIdx.x; i < n; i += JAC_THREADS) wout[eoff[b] + i] = A[i * ld + i];
+}
+
+int jacobi_max_dim(int device) {
+  static int cached = -1;
+  if (cached >= 0) return cached;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
+  const int budget = lds - 8192;  // static LDS of the kernel
+  int nmax = 0;
+  for (int n = 2; n <= 128; n += 2)
+    if (2 * n * (n + 1) * (int)sizeof(double) <= budget) nmax = n;
+  if (nmax > 0)
+    (void)hipFuncSetAttribute((const void*)k_jacobi_eig, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              2 * nmax * (nmax + 1) * (int)sizeof(double));
+  cached = nmax;
+  return cached;
+}
+
+void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
+                       double* wout, int nblk, int nmax, hipStream_t s) {
+  if (nblk <= 0) return;
+  const int np = nmax + (nmax & 1);
+  const size_t lds = (size_t)2 * np * (np + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_jacobi_eig, dim3(nblk), dim3(JAC_THREADS), lds, s, a, off, dim, eoff, vout, wout, 40);
+  NSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ batched dense eigensolve staging
+// Block b (dim[b] x dim[b] at src+off[b]) is copied into the b-th ncls x ncls
+// slab of dst, the rest of the slab holding -1 on the diagonal (below any
+// rho eigenvalue, decoupled from the block).
+__global__ __launch_bounds__(256) void k_pad_blocks(const double* __restrict__ src, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ dim, int ncls,
+                                                    double* __restrict__ dst) {
+  const int b = blockIdx.y;
+  const int n = dim[b];
+  double* o = dst + (size_t)b * ncls * ncls;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < (int64_t)ncls * ncls; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e / ncls), j = (int)(e % ncls);
+    o[e] = (i < n && j < n) ? src[off[b] + (size_t)i * n + j] : (i == j ? -1.0 : 0.0);
+  }
+}
+
+// After an ascending column-major eigen-decomposition of each slab, the
+// block's eigenpairs are the last dim[b]: vectors (column-major, rows < n)
+// back to dst+off[b], values to w+eoff[b].
+__global__ __launch_bounds__(256) void k_unpad_eig(const double* __restrict__ vecs, const double* __restrict__ vals,
+                                                   int ncls, const int64_t* __restrict__ off,
+                                                   const int32_t* __restrict__ dim, const int64_t* __restrict__ eoff,
+                                                   double* __restrict__ dst, double* __restrict__ w) {
+  const int b = blockIdx.y;
+  const int n = dim[b];
+  const double* v = vecs + (size_t)b * ncls * ncls;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * 256) {
+    const int k = (int)(e / n), i = (int)(e % n);
+    dst[off[b] + e] = v[(size_t)(ncls - n + k) * ncls + i];
+  }
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < n; k += 256) w[eoff[b] + k] = vals[(size_t)b * ncls + ncls - n + k];
+}
+
+void launch_pad_blocks(const double* src, const int64_t* off, const int32_t* dim, int nb, int ncls, double* dst,
+                       hipStream_t s) {
+  const unsigned gx = (unsigned)std::min<int64_t>(64, ((int64_t)ncls * ncls + 255) / 256);
+  hipLaunchKernelGGL(k_pad_blocks, dim3(gx, nb), dim3(256), 0, s, src, off, dim, ncls, dst);
+  NSS_HIP(hipGetLastError());
+}
+
+void launch_unpad_eig(const double* vecs, const double* vals, int ncls, const int64_t* off, const int32_t* dim,
+                      const int64_t* eoff, int nb, double* dst, double* w, hipStream_t s) {
+  const unsigned gx = (unsigned)std::min<int64_t>(64, ((int64_t)ncls * ncls + 255) / 256);
+  hipLaunchKernelGGL(k_unpad_eig, dim3(gx, nb), dim3(256), 0, s, vecs, vals, ncls, off, dim, eoff, dst, w);
   NSS_HIP(hipGetLastError());
 }
 

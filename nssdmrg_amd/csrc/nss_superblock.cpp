@@ -6,6 +6,8 @@
 // which is the reference's evaluation order (B first, then A).
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <mutex>
@@ -18,8 +20,9 @@
 namespace nss {
 
 // rho blocks up to this size are diagonalised on the host (Householder +
-// implicit QL); larger ones on the device with rocSOLVER dsyevd.
-constexpr int HOST_EIG_MAX = 48;
+// implicit QL); larger ones on the device (batched Jacobi, or rocSOLVER
+// dsyevd beyond the Jacobi kernel's LDS limit).
+constexpr int HOST_EIG_MAX = 12;
 
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
@@ -339,6 +342,28 @@ double nss_hv_timer_flush(nss_ctx* ctx) {
   return tot;
 }
 
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+PhaseTimer::PhaseTimer(nss_ctx* ctx, const char* n) : c(ctx), name(n), t0(ctx->prof ? now_s() : 0.0) {}
+
+void PhaseTimer::next(const char* n) {
+  if (c->prof) {
+    (void)hipStreamSynchronize(c->stream);
+    const double t = now_s();
+    c->timers[name] += t - t0;
+    t0 = t;
+  }
+  name = n;
+}
+
+PhaseTimer::~PhaseTimer() {
+  if (!c->prof) return;
+  (void)hipStreamSynchronize(c->stream);
+  c->timers[name] += now_s() - t0;
+}
+
 void nss_release_rocblas(nss_ctx* ctx) {
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   ctx->ev.clear();
@@ -379,9 +404,11 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
       sb->R.add_output(BASE_Y, nullptr, roff[q], dr, dr, dr, {g});
     }
   }
+  PhaseTimer pt(ctx, "rdm.gemm");
   sb->R.upload(ctx->pool, ctx->stream);
   Bases b{psi_dev, nullptr, sb->rho.p};
   launch_gemm_f64(sb->R, b, ctx->stream, GEMM_GENERIC);
+  pt.next("rdm.symcheck");
   // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the device
   std::vector<int64_t> hoff;
   std::vector<int32_t> hdim;
@@ -409,19 +436,66 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   for (int c = 0; c < nchk; ++c)
     if (!(asym[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
-  // large blocks: rocSOLVER dsyevd in place (eigenvectors overwrite rho)
+  pt.next("rdm.jacobi");
+  // eigen-decompositions: tiny blocks on the host, blocks up to the LDS
+  // limit with the batched GPU Jacobi kernel, larger ones with rocSOLVER
+  // dsyevd; device results overwrite rho in place (eigenvectors as columns)
+  const int jmax = jacobi_max_dim(ctx->device);
   std::vector<int64_t> eoff(nq, -1);
+  std::vector<int> kind(nq, 0);  // 0 host, 1 jacobi, 2 rocsolver
   int64_t etot = 0;
+  std::vector<int64_t> joff, jeoff;
+  std::vector<int32_t> jdim;
+  int jn = 0;
   for (int q = 0; q < nq; ++q) {
     const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    if (d > HOST_EIG_MAX && (int64_t)sb->Dl[q] * sb->Dr[q] > 0) {
-      eoff[q] = etot;
-      etot += d;
+    if (d <= HOST_EIG_MAX || (int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
+    eoff[q] = etot;
+    etot += d;
+    if (ctx->prof) {
+      ctx->timers[d <= jmax ? "rdm.count_jacobi" : "rdm.count_rocsolver"] += 1;
+      if (d > jmax) ctx->timers["rdm.sum_dim_rocsolver"] += d;
+    }
+    if (d <= jmax) {
+      kind[q] = 1;
+      joff.push_back(roff[q]);
+      jeoff.push_back(eoff[q]);
+      jdim.push_back(d);
+      jn = std::max(jn, d);
+    } else {
+      kind[q] = 2;
     }
   }
   DevArr<double> dD, dE;
   DevArr<int32_t> dinfo;
+  DevArr<int64_t> djoff, djeoff;
+  DevArr<int32_t> djdim;
   if (etot) {
+    dD.resize(ctx->pool, etot);
+    dE.resize(ctx->pool, etot);
+    dinfo.resize(ctx->pool, nq);
+    NSS_HIP(hipMemsetAsync(dinfo.p, 0, nq * sizeof(int32_t), ctx->stream));
+  }
+  if (!jdim.empty()) {
+    const int nb = (int)jdim.size();
+    djoff.resize(ctx->pool, nb);
+    djeoff.resize(ctx->pool, nb);
+    djdim.resize(ctx->pool, nb);
+    NSS_HIP(hipMemcpyAsync(djoff.p, joff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    NSS_HIP(hipMemcpyAsync(djeoff.p, jeoff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    NSS_HIP(hipMemcpyAsync(djdim.p, jdim.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    launch_jacobi_eig(sb->rho.p, djoff.p, djdim.p, djeoff.p, sb->rho.p, dD.p, nb, jn, ctx->stream);
+  }
+  pt.next("rdm.rocsolver");
+  // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
+  // size class (dims rounded up to a multiple of 64, padded slabs)
+  std::map<int, std::vector<int>> classes;
+  for (int q = 0; q < nq; ++q)
+    if (kind[q] == 2) {
+      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+      classes[(d + 63) / 64 * 64].push_back(q);
+    }
+  if (!classes.empty()) {
     rocblas_handle hnd = static_cast<rocblas_handle>(ctx->rocblas);
     if (!hnd) {
       if (rocblas_create_handle(&hnd) != rocblas_status_success)
@@ -430,18 +504,43 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
     }
     if (rocblas_set_stream(hnd, ctx->stream) != rocblas_status_success)
       throw Error(NSS_E_HIP, "rocblas_set_stream failed");
-    dD.resize(ctx->pool, etot);
-    dE.resize(ctx->pool, etot);
-    dinfo.resize(ctx->pool, nq);
-    for (int q = 0; q < nq; ++q) {
-      if (eoff[q] < 0) continue;
-      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-      const rocblas_status st = rocsolver_dsyevd(hnd, rocblas_evect_original, rocblas_fill_upper, d,
-                                                 sb->rho.p + roff[q], d, dD.p + eoff[q], dE.p + eoff[q],
-                                                 dinfo.p + q);
-      if (st != rocblas_status_success) throw Error(NSS_E_HIP, "rocsolver_dsyevd failed");
+    for (auto& kv : classes) {
+      const int ncls = kv.first;
+      const int nb = (int)kv.second.size();
+      std::vector<int64_t> co, ceo;
+      std::vector<int32_t> cd;
+      for (int q : kv.second) {
+        co.push_back(roff[q]);
+        ceo.push_back(eoff[q]);
+        cd.push_back(side == 0 ? sb->Dl[q] : sb->Dr[q]);
+      }
+      DevArr<int64_t> dco, dceo;
+      DevArr<int32_t> dcd, dci;
+      DevArr<double> slab, sD, sE;
+      dco.resize(ctx->pool, nb);
+      dceo.resize(ctx->pool, nb);
+      dcd.resize(ctx->pool, nb);
+      dci.resize(ctx->pool, nb);
+      slab.resize(ctx->pool, (size_t)nb * ncls * ncls);
+      sD.resize(ctx->pool, (size_t)nb * ncls);
+      sE.resize(ctx->pool, (size_t)nb * ncls);
+      NSS_HIP(hipMemcpyAsync(dco.p, co.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+      NSS_HIP(hipMemcpyAsync(dceo.p, ceo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+      NSS_HIP(hipMemcpyAsync(dcd.p, cd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+      launch_pad_blocks(sb->rho.p, dco.p, dcd.p, nb, ncls, slab.p, ctx->stream);
+      const rocblas_status st = rocsolver_dsyevd_strided_batched(
+          hnd, rocblas_evect_original, rocblas_fill_upper, ncls, slab.p, ncls, (rocblas_stride)ncls * ncls, sD.p,
+          ncls, sE.p, ncls, dci.p, nb);
+      if (st != rocblas_status_success) throw Error(NSS_E_HIP, "rocsolver_dsyevd_strided_batched failed");
+      launch_unpad_eig(slab.p, sD.p, ncls, dco.p, dcd.p, dceo.p, nb, sb->rho.p, dD.p, ctx->stream);
+      std::vector<int32_t> ci(nb);
+      NSS_HIP(hipMemcpyAsync(ci.data(), dci.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+      NSS_HIP(hipStreamSynchronize(ctx->stream));
+      for (int c = 0; c < nb; ++c)
+        if (ci[c] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
     }
   }
+  pt.next("rdm.download");
   std::vector<double> h(tot), hD(std::max<int64_t>(1, etot));
   std::vector<int32_t> hinfo(nq, 0);
   if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -450,6 +549,7 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
     NSS_HIP(hipMemcpyAsync(hinfo.data(), dinfo.p, nq * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
+  pt.next("rdm.host");
   evals.assign(nq, {});
   evecs.assign(nq, {});
   for (int q = 0; q < nq; ++q) {
@@ -458,12 +558,21 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
     double* a = h.data() + roff[q];
     evals[q].resize(d);
     evecs[q].resize((size_t)d * d);
-    if (eoff[q] >= 0) {
-      if (hinfo[q] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
+    if (kind[q] == 2) {
       std::copy(hD.begin() + eoff[q], hD.begin() + eoff[q] + d, evals[q].begin());
       // column-major eigenvectors -> row-major with eigenvectors as columns
       for (int i = 0; i < d; ++i)
         for (int j = 0; j < d; ++j) evecs[q][(size_t)i * d + j] = a[(size_t)j * d + i];
+    } else if (kind[q] == 1) {
+      // Jacobi: unsorted eigenpairs, row-major columns -> ascending order
+      std::vector<int> idx(d);
+      for (int i = 0; i < d; ++i) idx[i] = i;
+      const double* w = hD.data() + eoff[q];
+      std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
+      for (int j = 0; j < d; ++j) {
+        evals[q][j] = w[idx[j]];
+        for (int i = 0; i < d; ++i) evecs[q][(size_t)i * d + j] = a[(size_t)i * d + idx[j]];
+      }
     } else {
       std::copy(a, a + (size_t)d * d, evecs[q].begin());
       sym_eig(d, evecs[q].data(), evals[q].data());
@@ -518,6 +627,8 @@ extern "C" int nss_ctx_create(int device, nss_ctx** out) {
   NSS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->pool = new DevicePool(device);
   c->sb = sb_create(c);
+  const char* pe = std::getenv("NSS_PROFILE");
+  c->prof = pe && pe[0] == '1';
   *out = c;
   NSS_API_END
 }
@@ -685,6 +796,23 @@ extern "C" int nss_ctx_hv_stats(nss_ctx* ctx, int reset, double* out) {
       ctx->st_calls[k] = 0;
       ctx->st_tw[k] = ctx->st_ty[k] = ctx->st_fw[k] = ctx->st_fy[k] = 0.0;
     }
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_ctx_timers(nss_ctx* ctx, char* buf, int len) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx, NSS_E_ARG, "nss_ctx_timers: NULL ctx");
+  std::string out = "{";
+  bool first = true;
+  for (auto& kv : ctx->timers) {
+    out += (first ? "\"" : ", \"") + kv.first + "\": " + std::to_string(kv.second);
+    first = false;
+  }
+  out += "}";
+  if (buf && len > 0) {
+    std::strncpy(buf, out.c_str(), (size_t)len - 1);
+    buf[len - 1] = 0;
   }
   NSS_API_END
 }

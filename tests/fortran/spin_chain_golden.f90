@@ -1,0 +1,163 @@
+!> Fortran driver of the GPU path, modelled on the reference's own test
+!> program test/src/Spin1d_SU2/spinSU2_1d.f90 (spin-1/2 Heisenberg, iDMRG
+!> LDMRG=40, MDMRG=20, LANC_NEIGEN=2, LANC_DIM_THRESHOLD=128) and its golden
+!> energies (tests/golden/spin12_L40_M20_energy.check, copied data).
+!> Part 2 binds spMatVec_gpu_main to a procedure pointer of the reference's
+!> abstract interface sparse_HxV and checks it against the direct-product
+!> loops of spMatVec_direct_main written out here on the host.
+program spin_chain_golden
+  use iso_c_binding
+  use nss_dmrg
+  implicit none
+
+  abstract interface
+     subroutine sparse_HxV(Nloc, v, Hv)   ! src/DMRG_GLOBAL.f90:41-53
+       integer :: Nloc
+       real(8), dimension(Nloc) :: v
+       real(8), dimension(Nloc) :: Hv
+     end subroutine sparse_HxV
+  end interface
+  procedure(sparse_HxV), pointer :: spHtimesV_p => null()
+
+  type(nss_model) :: model
+  type(nss_params) :: par
+  type(nss_step_info) :: info
+  real(8) :: Hij(2, 2), ref(3), err, maxerr
+  integer :: i, u, ios, nbad
+  character(len=512) :: golden
+
+  ! ---------------------------------------------------------------- part 1
+  call get_command_argument(1, golden)
+  Hij = 0d0
+  Hij(1, 1) = 1d0      ! Jp
+  Hij(2, 2) = 0.5d0    ! Jx/2
+  model%model = NSS_MODEL_SPIN12
+  par%ldmrg = 40
+  par%mdmrg = 20
+  par%neigen = 2
+  par%lanc_dim_threshold = 128
+  par%lanc_tol = 1d-12
+  call init_dmrg_gpu(Hij, model, par)
+  call run_dmrg_gpu('i')
+  open(newunit=u, file=trim(golden), status="old", action="read")
+  maxerr = 0d0
+  nbad = 0
+  do i = 1, dmrg_gpu_nsteps()
+     read(u, *, iostat=ios) ref
+     if (ios /= 0) exit
+     info = dmrg_gpu_step(i)
+     if (info%left_len /= nint(ref(1))) nbad = nbad + 1
+     err = max(abs(info%energies(1) / info%L - ref(2)), abs(info%energies(2) / info%L - ref(3)))
+     maxerr = max(maxerr, err)
+  enddo
+  close(u)
+  call finalize_dmrg_gpu()
+  write(*, "(A,I3,A,ES10.3)") "golden iDMRG steps:", dmrg_steps_checked(), "  max |dE/L| =", maxerr
+  if (maxerr > 1d-10 .or. nbad > 0) then
+     write(*, "(A)") "FAIL golden energies"
+     stop 1
+  endif
+
+  ! ---------------------------------------------------------------- part 2
+  call check_drop_in()
+  call nss_gpu_finalize()
+  write(*, "(A)") "PASS"
+
+contains
+
+  integer function dmrg_steps_checked()
+    dmrg_steps_checked = 39
+  end function dmrg_steps_checked
+
+  subroutine check_drop_in()
+    integer, parameter :: nq = 3, nt = 4
+    integer :: Dls(nq), Drs(nq), dst(nt), src(nt), offs(nq)
+    type(dense_block) :: HL(nq), HR(nq), A(nt), B(nt)
+    real(8), allocatable :: v(:), hv(:), ref(:), C(:,:), evecs(:,:)
+    real(8) :: evals(1), res
+    integer :: q, t, n, il, ir, jl, jr, ai, aj, bi, bj, nop
+    integer, allocatable :: seed(:)
+    call random_seed(size=n)
+    allocate(seed(n))
+    seed = 20260
+    call random_seed(put=seed)
+    Dls = [3, 70, 5]
+    Drs = [65, 4, 33]
+    dst = [1, 2, 2, 3]
+    src = [2, 1, 3, 3]
+    n = 0
+    do q = 1, nq
+       offs(q) = n
+       n = n + Dls(q) * Drs(q)
+       allocate(HL(q)%m(Dls(q), Dls(q)), HR(q)%m(Drs(q), Drs(q)))
+       call random_number(HL(q)%m)
+       call random_number(HR(q)%m)
+       HL(q)%m = HL(q)%m + transpose(HL(q)%m)
+       HR(q)%m = HR(q)%m + transpose(HR(q)%m)
+    enddo
+    do t = 1, nt
+       allocate(A(t)%m(Dls(dst(t)), Dls(src(t))), B(t)%m(Drs(dst(t)), Drs(src(t))))
+       call random_number(A(t)%m)
+       call random_number(B(t)%m)
+    enddo
+    call nss_gpu_set_superblock(Dls, Drs, HL, HR, dst, src, A, B)
+    allocate(v(n), hv(n), ref(n))
+    call random_number(v)
+    spHtimesV_p => spMatVec_gpu_main
+    call spHtimesV_p(n, v, hv)
+    ! reference loops (spMatVec_direct_main, src/DMRG_SUPERBLOCK_SETUP.f90:760-857)
+    ref = 0d0
+    do q = 1, nq
+       do il = 1, Dls(q)
+          do ir = 1, Drs(q)
+             do jr = 1, Drs(q)
+                ref(ir + (il-1)*Drs(q) + offs(q)) = ref(ir + (il-1)*Drs(q) + offs(q)) + &
+                     HR(q)%m(ir, jr) * v(jr + (il-1)*Drs(q) + offs(q))
+             enddo
+             do jl = 1, Dls(q)
+                ref(ir + (il-1)*Drs(q) + offs(q)) = ref(ir + (il-1)*Drs(q) + offs(q)) + &
+                     HL(q)%m(il, jl) * v(ir + (jl-1)*Drs(q) + offs(q))
+             enddo
+          enddo
+       enddo
+    enddo
+    do t = 1, nt
+       allocate(C(size(B(t)%m, 1), size(A(t)%m, 2)))
+       C = 0d0
+       do aj = 1, size(A(t)%m, 2)
+          do bi = 1, size(B(t)%m, 1)
+             do bj = 1, size(B(t)%m, 2)
+                C(bi, aj) = C(bi, aj) + B(t)%m(bi, bj) * v(bj + (aj-1)*size(B(t)%m, 2) + offs(src(t)))
+             enddo
+          enddo
+       enddo
+       do bi = 1, size(B(t)%m, 1)
+          do ai = 1, size(A(t)%m, 1)
+             do aj = 1, size(A(t)%m, 2)
+                ref(bi + (ai-1)*size(B(t)%m, 1) + offs(dst(t))) = ref(bi + (ai-1)*size(B(t)%m, 1) + offs(dst(t))) + &
+                     A(t)%m(ai, aj) * C(bi, aj)
+             enddo
+          enddo
+       enddo
+       deallocate(C)
+    enddo
+    res = maxval(abs(hv - ref)) / maxval(abs(ref))
+    write(*, "(A,ES10.3)") "drop-in spMatVec_gpu_main vs direct loops, max rel err =", res
+    if (res > 1d-13) then
+       write(*, "(A)") "FAIL drop-in H*v"
+       stop 1
+    endif
+    ! Lanczos on a symmetric superblock: keep only the sector-diagonal terms
+    call nss_gpu_set_superblock(Dls, Drs, HL, HR, [integer ::], [integer ::], A(1:0), B(1:0))
+    allocate(evecs(n, 1))
+    call nss_gpu_lanczos(1, evals, evecs, tol=1d-12, seed=1, numop=nop, kmax=80)
+    call spHtimesV_p(n, evecs(:, 1), hv)
+    res = sqrt(sum((hv - evals(1) * evecs(:, 1))**2))
+    write(*, "(A,F14.8,A,ES10.3,A,I4)") "drop-in Lanczos E0 =", evals(1), "  residual =", res, "  H*v calls =", nop
+    if (res > 1d-8 * max(1d0, abs(evals(1)))) then
+       write(*, "(A)") "FAIL drop-in Lanczos residual"
+       stop 1
+    endif
+  end subroutine check_drop_in
+
+end program spin_chain_golden

@@ -213,6 +213,6 @@ def test_rdm_eigh_large_blocks(ctx):
             p = psi[off:off + dl * dr].reshape(dl, dr)
             rho = p @ p.T if side == "left" else p.T @ p
             ref = np.linalg.eigvalsh(rho)
-            assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-15), (side, k)
+            assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-14), (side, k)
             assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14
             assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12
