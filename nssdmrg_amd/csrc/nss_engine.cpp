@@ -670,12 +670,11 @@ struct Engine {
     for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = lo.evals[i];
     // sb_get_rdm
     pt.next("step.rdm");
-    std::vector<std::vector<double>> evl, vl, evr, vr;
-    sb_rdm_eigh(ctx, ctx->sb, psi.p, 0, evl, vl);
-    sb_rdm_eigh(ctx, ctx->sb, psi.p, 1, evr, vr);
+    std::vector<std::vector<double>> evs[2], vecs[2];
+    sb_rdm_eigh2(ctx, ctx->sb, psi.p, 3, evs, vecs);
     pt.next("step.spectrum");
-    Spectrum sl = sort_spectrum(std::move(evl), std::move(vl));
-    Spectrum sr = sort_spectrum(std::move(evr), std::move(vr));
+    Spectrum sl = sort_spectrum(std::move(evs[0]), std::move(vecs[0]));
+    Spectrum sr = sort_spectrum(std::move(evs[1]), std::move(vecs[1]));
     const auto t4 = clk::now();
     I.t_rdm = sec(t3, t4);
     I.entropy_left = entropy(sl.all_desc);

@@ -318,6 +318,8 @@ const std::vector<int64_t>& sb_off(const SbDevice* sb);
 // rho blocks of a device vector: side 0 left, 1 right -> host eigen-decomposition.
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
                  std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs);
-void sb_take_fp32_ops(nss_ctx* ctx, SbDevice* sb);
+// both sides at once (sides bit 0: left, bit 1: right)
+void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
+                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2]);
 
 }  // namespace nss

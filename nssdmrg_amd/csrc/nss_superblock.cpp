@@ -373,57 +373,68 @@ void nss_release_rocblas(nss_ctx* ctx) {
   }
 }
 
-void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
-                 std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs) {
+// rho blocks of both sides in one batch: items (side, sector) share one
+// rho buffer, one GEMM launch, one symmetry check, one Jacobi launch and one
+// strided-batched rocSOLVER call per size class.
+void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
+                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2]) {
   const int nq = (int)sb->Dl.size();
-  std::vector<int64_t> roff(nq, 0);
+  struct Item {
+    int side, q, d;
+    int64_t roff, eoff;
+    int kind;  // 0 host, 1 jacobi, 2 rocsolver
+  };
+  std::vector<Item> items;
   int64_t tot = 0;
-  for (int q = 0; q < nq; ++q) {
-    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    roff[q] = tot;
-    tot += (int64_t)d * d;
+  for (int side = 0; side < 2; ++side) {
+    evals[side].assign(nq, {});
+    evecs[side].assign(nq, {});
+    if (!(sides & (1 << side))) continue;
+    for (int q = 0; q < nq; ++q) {
+      if ((int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
+      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+      items.push_back({side, q, d, tot, -1, 0});
+      tot += (int64_t)d * d;
+    }
   }
+  const int ni = (int)items.size();
   sb->rho.resize(ctx->pool, std::max<int64_t>(1, tot));
+  PhaseTimer pt(ctx, "rdm.gemm");
   sb->R.clear();
-  for (int q = 0; q < nq; ++q) {
-    const int dl = sb->Dl[q], dr = sb->Dr[q];
-    if (dl == 0 || dr == 0) continue;
+  for (const Item& it : items) {
+    const int dl = sb->Dl[it.q], dr = sb->Dr[it.q];
     GemmSeg g{};
     g.alpha = 1.0;
     g.abase = BASE_X;
     g.bbase = BASE_X;
-    g.aoff = g.boff = sb->off[q];
+    g.aoff = g.boff = sb->off[it.q];
     g.lda = g.ldb = dr;
-    if (side == 0) {  // rho_L = Psi Psi^T : K = Dr
+    if (it.side == 0) {  // rho_L = Psi Psi^T : K = Dr
       g.K = dr;
       g.flags = SEG_TB;
-      sb->R.add_output(BASE_Y, nullptr, roff[q], dl, dl, dl, {g});
     } else {  // rho_R = Psi^T Psi : K = Dl
       g.K = dl;
       g.flags = SEG_TA;
-      sb->R.add_output(BASE_Y, nullptr, roff[q], dr, dr, dr, {g});
     }
+    sb->R.add_output(BASE_Y, nullptr, it.roff, it.d, it.d, it.d, {g});
   }
-  PhaseTimer pt(ctx, "rdm.gemm");
   sb->R.upload(ctx->pool, ctx->stream);
   Bases b{psi_dev, nullptr, sb->rho.p};
   launch_gemm_f64(sb->R, b, ctx->stream, GEMM_GENERIC);
-  pt.next("rdm.symcheck");
   // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the device
+  pt.next("rdm.symcheck");
   std::vector<int64_t> hoff;
   std::vector<int32_t> hdim;
-  for (int q = 0; q < nq; ++q) {
-    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    if (d > 1 && (int64_t)sb->Dl[q] * sb->Dr[q] > 0) {
-      hoff.push_back(roff[q]);
-      hdim.push_back(d);
+  for (const Item& it : items)
+    if (it.d > 1) {
+      hoff.push_back(it.roff);
+      hdim.push_back(it.d);
     }
-  }
+  const int nchk = (int)hoff.size();
+  std::vector<double> asym(std::max(1, nchk), 0.0);
   DevArr<int64_t> doff;
   DevArr<int32_t> ddim;
   DevArr<double> dasym;
-  const int nchk = (int)hoff.size();
-  std::vector<double> asym(std::max(1, nchk), 0.0);
   if (nchk) {
     doff.resize(ctx->pool, nchk);
     ddim.resize(ctx->pool, nchk);
@@ -436,46 +447,37 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   for (int c = 0; c < nchk; ++c)
     if (!(asym[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
-  pt.next("rdm.jacobi");
   // eigen-decompositions: tiny blocks on the host, blocks up to the LDS
   // limit with the batched GPU Jacobi kernel, larger ones with rocSOLVER
-  // dsyevd; device results overwrite rho in place (eigenvectors as columns)
+  // dsyevd; device results overwrite rho in place
+  pt.next("rdm.jacobi");
   const int jmax = jacobi_max_dim(ctx->device);
-  std::vector<int64_t> eoff(nq, -1);
-  std::vector<int> kind(nq, 0);  // 0 host, 1 jacobi, 2 rocsolver
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff;
   std::vector<int32_t> jdim;
   int jn = 0;
-  for (int q = 0; q < nq; ++q) {
-    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    if (d <= HOST_EIG_MAX || (int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
-    eoff[q] = etot;
-    etot += d;
+  for (Item& it : items) {
+    if (it.d <= HOST_EIG_MAX) continue;
+    it.eoff = etot;
+    etot += it.d;
     if (ctx->prof) {
-      ctx->timers[d <= jmax ? "rdm.count_jacobi" : "rdm.count_rocsolver"] += 1;
-      if (d > jmax) ctx->timers["rdm.sum_dim_rocsolver"] += d;
+      ctx->timers[it.d <= jmax ? "rdm.count_jacobi" : "rdm.count_rocsolver"] += 1;
+      if (it.d > jmax) ctx->timers["rdm.sum_dim_rocsolver"] += it.d;
     }
-    if (d <= jmax) {
-      kind[q] = 1;
-      joff.push_back(roff[q]);
-      jeoff.push_back(eoff[q]);
-      jdim.push_back(d);
-      jn = std::max(jn, d);
+    if (it.d <= jmax) {
+      it.kind = 1;
+      joff.push_back(it.roff);
+      jeoff.push_back(it.eoff);
+      jdim.push_back(it.d);
+      jn = std::max(jn, it.d);
     } else {
-      kind[q] = 2;
+      it.kind = 2;
     }
   }
-  DevArr<double> dD, dE;
-  DevArr<int32_t> dinfo;
+  DevArr<double> dD;
+  if (etot) dD.resize(ctx->pool, etot);
   DevArr<int64_t> djoff, djeoff;
   DevArr<int32_t> djdim;
-  if (etot) {
-    dD.resize(ctx->pool, etot);
-    dE.resize(ctx->pool, etot);
-    dinfo.resize(ctx->pool, nq);
-    NSS_HIP(hipMemsetAsync(dinfo.p, 0, nq * sizeof(int32_t), ctx->stream));
-  }
   if (!jdim.empty()) {
     const int nb = (int)jdim.size();
     djoff.resize(ctx->pool, nb);
@@ -486,15 +488,12 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
     NSS_HIP(hipMemcpyAsync(djdim.p, jdim.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     launch_jacobi_eig(sb->rho.p, djoff.p, djdim.p, djeoff.p, sb->rho.p, dD.p, nb, jn, ctx->stream);
   }
-  pt.next("rdm.rocsolver");
   // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
   // size class (dims rounded up to a multiple of 64, padded slabs)
+  pt.next("rdm.rocsolver");
   std::map<int, std::vector<int>> classes;
-  for (int q = 0; q < nq; ++q)
-    if (kind[q] == 2) {
-      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-      classes[(d + 63) / 64 * 64].push_back(q);
-    }
+  for (int i = 0; i < ni; ++i)
+    if (items[i].kind == 2) classes[(items[i].d + 63) / 64 * 64].push_back(i);
   if (!classes.empty()) {
     rocblas_handle hnd = static_cast<rocblas_handle>(ctx->rocblas);
     if (!hnd) {
@@ -509,10 +508,10 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
       const int nb = (int)kv.second.size();
       std::vector<int64_t> co, ceo;
       std::vector<int32_t> cd;
-      for (int q : kv.second) {
-        co.push_back(roff[q]);
-        ceo.push_back(eoff[q]);
-        cd.push_back(side == 0 ? sb->Dl[q] : sb->Dr[q]);
+      for (int i : kv.second) {
+        co.push_back(items[i].roff);
+        ceo.push_back(items[i].eoff);
+        cd.push_back(items[i].d);
       }
       DevArr<int64_t> dco, dceo;
       DevArr<int32_t> dcd, dci;
@@ -541,45 +540,48 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
     }
   }
   pt.next("rdm.download");
-  std::vector<double> h(tot), hD(std::max<int64_t>(1, etot));
-  std::vector<int32_t> hinfo(nq, 0);
+  std::vector<double> h(std::max<int64_t>(1, tot)), hD(std::max<int64_t>(1, etot));
   if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  if (etot) {
-    NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    NSS_HIP(hipMemcpyAsync(hinfo.data(), dinfo.p, nq * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  }
+  if (etot) NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   pt.next("rdm.host");
-  evals.assign(nq, {});
-  evecs.assign(nq, {});
-  for (int q = 0; q < nq; ++q) {
-    const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-    if (d == 0 || (sb->Dl[q] * (int64_t)sb->Dr[q]) == 0) continue;
-    double* a = h.data() + roff[q];
-    evals[q].resize(d);
-    evecs[q].resize((size_t)d * d);
-    if (kind[q] == 2) {
-      std::copy(hD.begin() + eoff[q], hD.begin() + eoff[q] + d, evals[q].begin());
+  for (const Item& it : items) {
+    const int d = it.d;
+    std::vector<double>& ev = evals[it.side][it.q];
+    std::vector<double>& vec = evecs[it.side][it.q];
+    const double* a = h.data() + it.roff;
+    ev.resize(d);
+    vec.resize((size_t)d * d);
+    if (it.kind == 2) {
+      std::copy(hD.begin() + it.eoff, hD.begin() + it.eoff + d, ev.begin());
       // column-major eigenvectors -> row-major with eigenvectors as columns
       for (int i = 0; i < d; ++i)
-        for (int j = 0; j < d; ++j) evecs[q][(size_t)i * d + j] = a[(size_t)j * d + i];
-    } else if (kind[q] == 1) {
+        for (int j = 0; j < d; ++j) vec[(size_t)i * d + j] = a[(size_t)j * d + i];
+    } else if (it.kind == 1) {
       // Jacobi: unsorted eigenpairs, row-major columns -> ascending order
       std::vector<int> idx(d);
       for (int i = 0; i < d; ++i) idx[i] = i;
-      const double* w = hD.data() + eoff[q];
+      const double* w = hD.data() + it.eoff;
       std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
       for (int j = 0; j < d; ++j) {
-        evals[q][j] = w[idx[j]];
-        for (int i = 0; i < d; ++i) evecs[q][(size_t)i * d + j] = a[(size_t)i * d + idx[j]];
+        ev[j] = w[idx[j]];
+        for (int i = 0; i < d; ++i) vec[(size_t)i * d + j] = a[(size_t)i * d + idx[j]];
       }
     } else {
-      std::copy(a, a + (size_t)d * d, evecs[q].begin());
-      sym_eig(d, evecs[q].data(), evals[q].data());
+      std::copy(a, a + (size_t)d * d, vec.begin());
+      sym_eig(d, vec.data(), ev.data());
     }
-    for (double& e : evals[q])
+    for (double& e : ev)
       if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
   }
+}
+
+void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
+                 std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs) {
+  std::vector<std::vector<double>> ev[2], vec[2];
+  sb_rdm_eigh2(ctx, sb, psi_dev, side == 0 ? 1 : 2, ev, vec);
+  evals = std::move(ev[side]);
+  evecs = std::move(vec[side]);
 }
 
 }  // namespace nss
