@@ -23,6 +23,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -103,6 +104,35 @@ def cpu_baseline(d, budget_s=15.0):
             "per_hv_ms": 1e3 * el / n}
 
 
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE over the timed sweep's
+    dispatches, scaled by a plain-copy calibration), or None."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def copy_probe(mib, reps):
+    """Plain device copy of `mib` MiB (reads mib, writes mib) timed with HIP
+    events (nss_probe_copy): the known-size dispatch the FETCH_SIZE /
+    WRITE_SIZE counters are scaled by, and the measured HBM copy bandwidth."""
+    import ctypes as C
+    from nssdmrg_amd import _ffi as F
+    n = mib * (1 << 20) // 8
+    sec = C.c_double(0.0)
+    F.check(F.lib().nss_probe_copy(n, reps, C.byref(sec)))
+    return 2.0 * n * 8 / sec.value / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +142,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-MFMA comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="end with a plain device copy of MIB MiB (counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); the QN-sector "
@@ -133,7 +165,8 @@ def main():
     y_flops = st["flops_y"] / calls
     roofline = {"bound": "mfma", "kernel": "k_hv_y_f64", "achieved": y_flops / y_avg_s / 1e12,
                 "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
-                "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA, "traffic": None,
+                "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA,
+                "traffic": profiled_traffic(a.workload, "k_hv_y_f64"),
                 "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
                 "w_kernel": {"name": "k_hv_w_f64", "avg_launch_us": 1e6 * st["t_w"] / calls,
                              "flops_per_launch": st["flops_w"] / calls}}
@@ -153,6 +186,7 @@ def main():
         "time_split_s": {k: float(sum(s[k] for s in steps)) for k in
                          ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
     }
+    out["hbm_copy_probe_gbs"] = copy_probe(1024, 10)
     if os.environ.get("NSS_PROFILE") == "1":
         out["phase_timers_s"] = ctx.timers()
     if not a.no_mixed:
@@ -179,6 +213,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(d64)
     d64.close()
     ctx.close()
+    if a.calib_copy:
+        copy_probe(a.calib_copy, 1)
+        print(f"[bench] calibration copy: {a.calib_copy} MiB read + {a.calib_copy} MiB written", flush=True)
     print(json.dumps(out), flush=True)
 
 

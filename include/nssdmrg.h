@@ -138,6 +138,12 @@ int nss_host_tridiag_eig(int n, const double* d, const double* e, double* w, dou
  * d[4*l + r] receives accumulator item r of lane l.  a, b: 64, d: 256. */
 int nss_probe_mfma(int prec, const double* a, const double* b, double* d);
 
+/* Plain device copy of n doubles (n even), timed over reps launches on the
+ * null stream: *seconds = average launch time.  A probe with no counterpart
+ * in the reference: it calibrates HBM byte counters and measures the copy
+ * bandwidth that bounds the streaming Lanczos kernels. */
+int nss_probe_copy(int64_t n, int reps, double* seconds);
+
 /* ------------------------------------------------------------------------
  * Device-resident DMRG engine: init_dmrg / run_DMRG / finalize_dmrg
  * (src/DMRG_MAIN.f90:22-241) with every block operator kept in HBM as

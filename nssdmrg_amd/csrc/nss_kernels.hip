@@ -777,6 +777,52 @@ __global__ void k_probe_mfma(const double* a, const double* b, double* d, int pr
 }
 }  // namespace nss
 
+namespace nss {
+// Plain streaming copy y = x (16-byte accesses): the known-size dispatch that
+// HBM counters (FETCH_SIZE, WRITE_SIZE) are calibrated against, and a
+// measured bandwidth ceiling for the streaming kernels.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2* __restrict__ x, double2* __restrict__ y,
+                                                    int64_t n2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+}  // namespace nss
+
+extern "C" int nss_probe_copy(int64_t n, int reps, double* seconds) {
+  using namespace nss;
+  try {
+    NSS_REQUIRE(n > 0 && n % 2 == 0 && reps > 0 && seconds, NSS_E_ARG, "probe_copy: n must be even and > 0");
+    double *dx = nullptr, *dy = nullptr;
+    hipEvent_t e0, e1;
+    NSS_HIP(hipMalloc(&dx, n * sizeof(double)));
+    NSS_HIP(hipMalloc(&dy, n * sizeof(double)));
+    NSS_HIP(hipMemset(dx, 0, n * sizeof(double)));
+    NSS_HIP(hipEventCreate(&e0));
+    NSS_HIP(hipEventCreate(&e1));
+    const int64_t n2 = n / 2;
+    const unsigned grid = (unsigned)std::min<int64_t>((n2 + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const double2*)dx, (double2*)dy, n2);
+    NSS_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, 0, (const double2*)dx, (double2*)dy, n2);
+    NSS_HIP(hipGetLastError());
+    NSS_HIP(hipEventRecord(e1, 0));
+    NSS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    NSS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *seconds = 1e-3 * ms / reps;
+    NSS_HIP(hipEventDestroy(e0));
+    NSS_HIP(hipEventDestroy(e1));
+    NSS_HIP(hipFree(dx));
+    NSS_HIP(hipFree(dy));
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  }
+  return NSS_OK;
+}
+
 extern "C" int nss_probe_mfma(int prec, const double* a, const double* b, double* d) {
   using namespace nss;
   try {
