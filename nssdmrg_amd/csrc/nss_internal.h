@@ -116,6 +116,9 @@ struct Bases {
   void* y;
 };
 
+// Tile edge for new plans: 32 unless the environment sets NSS_GEMM_TILE=64.
+int gemm_default_tile();
+
 // One grouped launch: tiles over outputs, segments over K-contributions.
 struct GemmPlan {
   std::vector<GemmTile> htiles;
@@ -123,6 +126,10 @@ struct GemmPlan {
   DevArr<GemmTile> tiles;
   DevArr<GemmSeg> segs;
   double flops = 0.0;
+  // Output tile edge: 32 -> k_*  (32x32 tile, K chain split over the four
+  // waves of a workgroup, reduced through LDS); 64 -> k_*_t64 (64x64 tile,
+  // four waves share LDS-staged operands).  Set before add_output.
+  int tile = gemm_default_tile();
   void clear() { htiles.clear(); hsegs.clear(); flops = 0.0; }
   // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
   // the same list of segments.

@@ -87,7 +87,7 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
   __shared__ T Bs[BK][BN + PAD];
   using acc_t = typename Mfma<T>::acc_t;
 
-  const GemmTile tl = tiles[blockIdx.x];
+  const GemmTile tl = tiles[0];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -169,13 +169,219 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
       }
 }
 
+// ---------------------------------------------------------------------------
+// 32x32 output tile per workgroup; the tile's K chain (the concatenation of its
+// segments) is cut into 16-deep chunks dealt round-robin to the four waves, so
+// a tile runs on four SIMDs at once and small sectors waste little MFMA work.
+// Each wave loads its operand fragments straight from global memory (no
+// operand is shared between waves) one chunk ahead of the MFMAs, and the four
+// partial tiles are summed through LDS in a fixed order (deterministic).
+// Fragment layout, MFMA 16x16x4 step r of a chunk at k0: lane l supplies
+// A[i = l%16][k0 + 4r + l/16] and B[k0 + 4r + l/16][j = l%16].
+template <typename T>
+struct Frag32 {
+  T a[2][4], b[2][4];
+};
+
+// Operand pointers in the global address space: loads through them compile
+// to global_load (counted by vmcnt only) rather than flat_load, whose
+// lgkmcnt share would make every scalar descriptor wait drain them.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+// Per-lane view of the current segment: base of the lane's two A rows and
+// two B columns, and the element stride along k of each operand.
+template <typename T>
+struct SegView32 {
+  gptr<T> pa[2], pb[2];
+  int ska, skb;
+  int K;
+  T alpha;
+};
+
+template <typename T>
+__device__ __forceinline__ void seg_view32(const GemmSeg& g, const Bases& bs, const GemmTile& tl, int fr,
+                                           SegView32<T>& v) {
+  const T* A = resolve_c<T>(g.abase, g.a, g.aoff, bs);
+  const T* B = resolve_c<T>(g.bbase, g.b, g.boff, bs);
+  const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
+  const int64_t sai = ta ? 1 : (int64_t)g.lda, sbj = tb ? (int64_t)g.ldb : 1;
+  v.ska = ta ? g.lda : 1;
+  v.skb = tb ? 1 : g.ldb;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    // clamped rows/columns: always in bounds; rows >= M and columns >= N
+    // only reach output elements that are never stored
+    const int64_t i = min(tl.row0 + 16 * m + fr, tl.M - 1);
+    const int64_t j = min(tl.col0 + 16 * m + fr, tl.N - 1);
+    v.pa[m] = (gptr<T>)(A + i * sai);
+    v.pb[m] = (gptr<T>)(B + j * sbj);
+  }
+  v.K = g.K;
+  v.alpha = static_cast<T>(g.alpha);
+}
+
+// Raw fragment loads from clamped addresses: no branch and no use of the
+// loaded values, so a whole chunk of loads stays in flight.  The K bound and
+// alpha are applied to A at use (mma_chunk32).
+template <typename T>
+__device__ __forceinline__ void load_frag32(const SegView32<T>& v, int k0, int fk, Frag32<T>& f) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int k = min(k0 + 4 * r + fk, v.K - 1);
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      f.a[m][r] = v.pa[m][k * v.ska];
+      f.b[m][r] = v.pb[m][k * v.skb];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void mma_chunk32(const Frag32<T>& f, int k0, int K, T alpha, int fk,
+                                            typename Mfma<T>::acc_t (&acc)[2][2]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const T sc = (k0 + 4 * r + fk < K) ? alpha : T(0);
+    const T a0 = sc * f.a[0][r], a1 = sc * f.a[1][r];
+    acc[0][0] = Mfma<T>::mma(a0, f.b[0][r], acc[0][0]);
+    acc[0][1] = Mfma<T>::mma(a0, f.b[1][r], acc[0][1]);
+    acc[1][0] = Mfma<T>::mma(a1, f.b[0][r], acc[1][0]);
+    acc[1][1] = Mfma<T>::mma(a1, f.b[1][r], acc[1][1]);
+  }
+}
+
+// Maps the launch index to a tile so that each XCD (workgroups are dealt to
+// the 8 XCDs round-robin) receives a contiguous run of the tile list: the
+// tiles of one sector then share that XCD's L2.  Returns -1 for padding.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+  const int per = (ntiles + 7) >> 3;
+  const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  return t < ntiles ? t : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
+                                            int ntiles, const Bases& bs) {
+  using acc_t = typename Mfma<T>::acc_t;
+  __shared__ T red[4][32][33];
+  const int t = xcd_tile(ntiles);
+  if (t < 0) return;
+  const GemmTile tl = tiles[t];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int fr = lane & 15, fk = lane >> 4;
+
+  acc_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  // cursor (segment s, chunk ch): this wave's chunks are the global chunk
+  // indices congruent to `wave` mod 4, counted across the segment chain.
+  const int send = tl.seg0 + tl.nseg;
+  int s = tl.seg0, ch = wave - 4, nch = 0;
+  SegView32<T> v;
+  // Advances the cursor to this wave's next chunk: (k0, K) of the chunk go
+  // to m, its alpha to alpha.  When the chain is exhausted it returns false,
+  // sets alpha = 0 and leaves m and the segment view as they were, so the
+  // following load still reads valid (clamped) addresses and the MFMA pass
+  // over it adds exactly zero.
+  auto advance = [&](int (&m)[2], T& alpha) -> bool {
+    if (s >= send) {  // exhausted for good (ch and nch are stale from here on)
+      alpha = T(0);
+      return false;
+    }
+    ch += 4;
+    if (ch >= nch) {
+      GemmSeg g;
+      bool moved = false;
+      while (s < send) {
+        if (moved || nch > 0) {
+          ch -= nch;
+          ++s;
+          if (s >= send) break;
+        }
+        moved = true;
+        g = segs[s];
+        nch = (g.K + 15) >> 4;
+        if (ch < nch) break;
+      }
+      if (s >= send) {
+        alpha = T(0);
+        return false;
+      }
+      seg_view32<T>(g, bs, tl, fr, v);
+    }
+    m[0] = ch << 4;
+    m[1] = v.K;
+    alpha = v.alpha;
+    return true;
+  };
+  // Two fragment buffers in ping-pong, each refilled right after its chunk
+  // has gone to the MFMA pipe, so loads run one chunk ahead.  Every refill
+  // issues its loads (an exhausted buffer re-reads valid addresses with
+  // alpha = 0), which keeps the loop straight-line with a single exit: the
+  // accumulators stay in place and each buffer waits only on its own loads.
+  int m0[2] = {0, 1}, m1[2] = {0, 1};
+  T al0 = T(0), al1 = T(0);
+  if (advance(m0, al0)) {
+    Frag32<T> f0, f1;
+    load_frag32<T>(v, m0[0], fk, f0);
+    advance(m1, al1);
+    load_frag32<T>(v, m1[0], fk, f1);
+    bool h0;
+    do {
+      mma_chunk32<T>(f0, m0[0], m0[1], al0, fk, acc);
+      h0 = advance(m0, al0);
+      load_frag32<T>(v, m0[0], fk, f0);
+      mma_chunk32<T>(f1, m1[0], m1[1], al1, fk, acc);
+      advance(m1, al1);
+      load_frag32<T>(v, m1[0], fk, f1);
+    } while (h0);
+  }
+
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // accumulator item q of lane l holds D[row][l%16] with
+        // row = l/16 + 4q (f64 16x16x4) or 4*(l/16) + q (f32 16x16x4)
+        const int row = mi * 16 + (sizeof(T) == 8 ? fk + 4 * q : fk * 4 + q);
+        red[wave][row][ni * 16 + fr] = acc[mi][ni][q];
+      }
+  __syncthreads();
+  T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = threadIdx.x + 256 * u;
+    const int row = e >> 5, col = e & 31;
+    const int gi = tl.row0 + row, gj = tl.col0 + col;
+    if (gi < tl.M && gj < tl.N) {
+      const T v = (red[0][row][col] + red[1][row][col]) + (red[2][row][col] + red[3][row][col]);
+      T* p = C + (size_t)gi * tl.ldc + gj;
+      *p = tl.beta ? *p + v : v;
+    }
+  }
+}
+
 // One kernel symbol per use so that traces separate them: the two H*v
 // phases (W: Psi.B^T per term, Y: per-sector accumulation) and the generic
-// grouped GEMM (rho, rotations).
-#define NSS_GEMM_KERNEL(name, T)                                                             \
-  __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,            \
-                                              const GemmSeg* __restrict__ segs, Bases bs) { \
-    gemm_tile<T>(tiles, segs, bs);                                                           \
+// grouped GEMM (rho, rotations); the _t64 symbols are the 64x64 LDS-staged
+// variant of each.
+#define NSS_GEMM_KERNEL(name, T)                                                                        \
+  __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,                       \
+                                              const GemmSeg* __restrict__ segs, int ntiles, Bases bs) { \
+    gemm_tile32<T>(tiles, segs, ntiles, bs);                                                            \
+  }                                                                                                     \
+  __global__ __launch_bounds__(256) void name##_t64(const GemmTile* __restrict__ tiles,                 \
+                                                    const GemmSeg* __restrict__ segs, int ntiles,       \
+                                                    Bases bs) {                                         \
+    const int t = xcd_tile(ntiles);                                                                     \
+    if (t >= 0) gemm_tile<T>(tiles + t, segs, bs);                                                      \
   }
 NSS_GEMM_KERNEL(k_hv_w_f64, double)
 NSS_GEMM_KERNEL(k_hv_y_f64, double)
@@ -183,18 +389,26 @@ NSS_GEMM_KERNEL(k_gemm_f64, double)
 NSS_GEMM_KERNEL(k_hv_w_f32, float)
 NSS_GEMM_KERNEL(k_hv_y_f32, float)
 
+typedef void (*gemm_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases);
+
+static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t k32, gemm_kernel_t k64) {
+  const int n = (int)p.htiles.size();
+  const unsigned grid = 8u * (unsigned)((n + 7) / 8);
+  hipLaunchKernelGGL(p.tile == 64 ? k64 : k32, dim3(grid), dim3(256), 0, s, p.tiles.p, p.segs.p, n, b);
+  NSS_HIP(hipGetLastError());
+}
+
 void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind) {
   if (p.htiles.empty()) return;
-  auto k = kind == GEMM_HV_W ? k_hv_w_f64 : kind == GEMM_HV_Y ? k_hv_y_f64 : k_gemm_f64;
-  hipLaunchKernelGGL(k, dim3((unsigned)p.htiles.size()), dim3(256), 0, s, p.tiles.p, p.segs.p, b);
-  NSS_HIP(hipGetLastError());
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f64, k_hv_w_f64_t64);
+  else if (kind == GEMM_HV_Y) launch_plan(p, b, s, k_hv_y_f64, k_hv_y_f64_t64);
+  else launch_plan(p, b, s, k_gemm_f64, k_gemm_f64_t64);
 }
 
 void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind) {
   if (p.htiles.empty()) return;
-  auto k = kind == GEMM_HV_W ? k_hv_w_f32 : k_hv_y_f32;
-  hipLaunchKernelGGL(k, dim3((unsigned)p.htiles.size()), dim3(256), 0, s, p.tiles.p, p.segs.p, b);
-  NSS_HIP(hipGetLastError());
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f32, k_hv_w_f32_t64);
+  else launch_plan(p, b, s, k_hv_y_f32, k_hv_y_f32_t64);
 }
 
 // ------------------------------------------------------------ vectors

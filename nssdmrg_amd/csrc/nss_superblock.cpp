@@ -79,6 +79,14 @@ DevicePool::~DevicePool() {
 }
 
 // ---------------------------------------------------------------- plans
+int gemm_default_tile() {
+  static const int t = [] {
+    const char* e = std::getenv("NSS_GEMM_TILE");
+    return (e && std::atoi(e) == 64) ? 64 : 32;
+  }();
+  return t;
+}
+
 void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
                           const std::vector<GemmSeg>& sg, int beta) {
   if (M <= 0 || N <= 0) return;
@@ -90,8 +98,8 @@ void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int 
     ++nseg;
     flops += 2.0 * M * (double)N * g.K;
   }
-  for (int r = 0; r < M; r += GEMM_BM)
-    for (int cc = 0; cc < N; cc += GEMM_BN) {
+  for (int r = 0; r < M; r += tile)
+    for (int cc = 0; cc < N; cc += tile) {
       GemmTile t{};
       t.c = c;
       t.coff = coff;
@@ -277,8 +285,10 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, cons
 
 static hipEvent_t next_event(nss_ctx* ctx) {
   if (ctx->ev_used == ctx->ev.size()) {
+    // Timing-only events: no system-scope fence (no L2 writeback between the
+    // W and Y launches, and tighter bracketing of each launch).
     hipEvent_t e;
-    NSS_HIP(hipEventCreate(&e));
+    NSS_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     ctx->ev.push_back(e);
   }
   return ctx->ev[ctx->ev_used++];
