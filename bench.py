@@ -6,8 +6,12 @@ quantum numbers, OBC, L=64 sites in the finite sweeps, m=256 kept states,
 plain FP64 H_sb.psi.  The reference's finite algorithm sweeps a superblock of
 2*LDMRG+2 sites (src/DMRG_MAIN.f90:168-241, fNlat=2*LDMRG), so L=64 is
 LDMRG=31 (the iDMRG growth ends at 62 sites).  HFMODE=T (the reference
-default), LANC_NEIGEN=1, LANC_TOLERANCE=1e-12, LANC_DIM_THRESHOLD=128,
-start vectors SplitMix64 uniform[-1,1) with seed 1.
+default), LANC_NEIGEN=1, LANC_TOLERANCE=1e-12, LANC_DIM_THRESHOLD=128.
+Lanczos start: SplitMix64 uniform[-1,1) with seed 1 for the iDMRG steps and
+the first finite step of each sweep direction; every later finite step starts
+from the previous ground state carried into the new basis (wavefunction
+prediction).  The same sweeps with a random start in every step (the
+reference's behaviour) are timed beside it under "fp64_random_start".
 
 A step is one finite sweep (left->right, right->left, back to the middle).
 The iDMRG growth and --warmup sweeps run untimed; --steps sweeps are timed.
@@ -46,7 +50,7 @@ WORKLOADS = {
 }
 
 
-def make_engine(N, ctx, w, nsweeps, precision, verbose=0):
+def make_engine(N, ctx, w, nsweeps, precision, verbose=0, predict=True):
     ldmrg = (w["L"] - 2) // 2
     if w["model"] == "electron":
         site, hij, tq = N.electron_site(), np.diag([-1.0, -1.0]), (0.5, 0.5)
@@ -55,12 +59,12 @@ def make_engine(N, ctx, w, nsweeps, precision, verbose=0):
         hij, tq = np.diag([1.0, 0.5]), (0.0, 0.0)
     cfg = N.DMRGConfig(site=site, hij=hij, ldmrg=ldmrg, mdmrg=w["m"], msweep=[w["m"]] * nsweeps,
                        target_qn=tq, uloc=w["U"], hfmode=True, neigen=1, lanc_dim_threshold=128,
-                       lanc_tol=1e-12, seed=w["seed"], precision=precision, verbose=verbose)
+                       lanc_tol=1e-12, seed=w["seed"], precision=precision, verbose=verbose, predict=predict)
     return N.DMRG(ctx, cfg)
 
 
-def run_path(N, ctx, w, warmup, steps, precision, verbose):
-    d = make_engine(N, ctx, w, warmup + steps, precision, verbose)
+def run_path(N, ctx, w, warmup, steps, precision, verbose, predict=True):
+    d = make_engine(N, ctx, w, warmup + steps, precision, verbose, predict)
     t0 = time.perf_counter()
     d.finite_begin()
     t_idmrg = time.perf_counter() - t0
@@ -141,6 +145,8 @@ def main():
     ap.add_argument("--workload", default="hubbard_L64_m256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-MFMA comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-random-start", action="store_true",
+                    help="skip the FP64 run whose every Lanczos starts from the seeded random vector")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="end with a plain device copy of MIB MiB (counter calibration under rocprofv3 --pmc)")
@@ -177,7 +183,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": a.workload, "model": "hubbard" if w["model"] == "electron" else w["model"],
                    "L": w["L"], "ldmrg": (w["L"] - 2) // 2, "m": w["m"], "U": w["U"], "bc": "obc",
-                   "hfmode": True, "neigen": 1, "lanc_tol": 1e-12, "seed": w["seed"]},
+                   "hfmode": True, "neigen": 1, "lanc_tol": 1e-12, "seed": w["seed"],
+                   "lanczos_start": "predicted wavefunction after the first step of each sweep direction"},
         "roofline": roofline,
         "sweep_wall_s": r64["walls"], "idmrg_wall_s": r64["t_idmrg"],
         "hv_calls": st["calls"], "hv_gpu_s": hv_time,
@@ -189,6 +196,21 @@ def main():
     out["hbm_copy_probe_gbs"] = copy_probe(1024, 10)
     if os.environ.get("NSS_PROFILE") == "1":
         out["phase_timers_s"] = ctx.timers()
+    if not a.no_random_start:
+        print("[bench] FP64 path, random Lanczos start in every step (no wavefunction prediction)", flush=True)
+        dr, rr = run_path(N, ctx, w, a.warmup, a.steps, "fp64", a.verbose, predict=False)
+        sr = rr["stats"]
+        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(steps, rr["steps"]))
+        out["fp64_random_start"] = {
+            "ms_per_step": 1e3 * float(np.mean(rr["walls"])), "hv_calls": sr["calls"],
+            "hv_gflops": sr["flops"] / max(1e-30, sr["t_w"] + sr["t_y"]) / 1e9,
+            "max_abs_dE0_per_site_vs_predicted": de,
+            "time_split_s": {k: float(sum(s_[k] for s_ in rr["steps"])) for k in
+                             ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
+        }
+        print(f"[bench] random start done: {out['fp64_random_start']['ms_per_step']:.1f} ms/sweep, "
+              f"{sr['calls']} H.v, max|dE0/L| vs predicted {de:.2e}", flush=True)
+        dr.close()
     if not a.no_mixed:
         print("[bench] FP32-MFMA path", flush=True)
         d32, r32 = run_path(N, ctx, w, a.warmup, a.steps, "mixed", a.verbose)

@@ -95,8 +95,9 @@ int nss_sb_flops(nss_ctx* ctx, double* flops);
  * src/DMRG_SUPERBLOCK.f90:412-425): Lanczos with full re-orthogonalisation
  * on the GPU, Krylov dimension <= kmax (restarted from the Ritz vectors),
  * residual ||H x - e x|| <= tol * max(1,|e|).  Start vector: SplitMix64
- * uniform[-1,1) from seed (v0_mode 0) or constant 1/sqrt(n) (v0_mode 1,
- * LANC_V0_dble=T).  precision NSS_FP32_MIXED runs FP32 H*v first and ends
+ * uniform[-1,1) from seed (v0_mode 0), constant 1/sqrt(n) (v0_mode 1,
+ * LANC_V0_dble=T), or the first n entries of evecs on input (v0_mode 2, e.g.
+ * a predicted wavefunction; a zero vector falls back to v0_mode 0).  precision NSS_FP32_MIXED runs FP32 H*v first and ends
  * with an FP64 restart.  evecs: host, n x neigen, vector-contiguous. */
 int nss_lanczos(nss_ctx* ctx, int neigen, int kmax, int maxiter, double tol, uint64_t seed,
                 int v0_mode, int precision, double* evals, double* evecs, int* numop);
@@ -178,6 +179,14 @@ typedef struct {
   double deg_thr;             /* DEG_EVALS_THRESHOLD */
   int32_t precision;          /* NSS_FP64 | NSS_FP32_MIXED */
   int32_t verbose;
+  /* 1: with neigen == 1, a finite step that follows a step in the same sweep
+   * direction starts its Lanczos from the previous ground state carried into
+   * the new superblock basis (wavefunction transformation through the
+   * kept-state matrices of both blocks); 0, or neigen > 1: every step starts
+   * from v0_mode, as the reference does (src/DMRG_SUPERBLOCK.f90:412-425).
+   * Converged energies, entropies and kept counts agree to the Lanczos
+   * tolerance either way. */
+  int32_t predict;
 } nss_params;
 
 typedef struct {

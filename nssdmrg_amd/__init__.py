@@ -283,6 +283,7 @@ class DMRGConfig:
     deg_thr: float = 0.1
     precision: str = "fp64"
     verbose: int = 0
+    predict: bool = False   # start finite-step Lanczos from the transformed previous ground state
 
 
 class DMRG:
@@ -314,6 +315,7 @@ class DMRG:
         p.seed, p.v0_mode, p.deg_thr = cfg.seed, cfg.v0_mode, cfg.deg_thr
         p.precision = F.NSS_FP32_MIXED if cfg.precision in ("fp32", "mixed") else F.NSS_FP64
         p.verbose = cfg.verbose
+        p.predict = int(bool(cfg.predict))
         h = C.c_void_p()
         F.check(F.lib().nss_dmrg_create(ctx.handle, C.byref(m), C.byref(p), C.byref(h)))
         self._h = h

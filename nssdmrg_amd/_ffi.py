@@ -40,7 +40,7 @@ class Params(C.Structure):
                 ("neigen", C.c_int32), ("lanc_dim_threshold", C.c_int32),
                 ("lanc_kmax", C.c_int32), ("lanc_maxiter", C.c_int32), ("lanc_tol", C.c_double),
                 ("seed", C.c_uint64), ("v0_mode", C.c_int32), ("deg_thr", C.c_double),
-                ("precision", C.c_int32), ("verbose", C.c_int32)]
+                ("precision", C.c_int32), ("verbose", C.c_int32), ("predict", C.c_int32)]
 
 
 class StepInfo(C.Structure):

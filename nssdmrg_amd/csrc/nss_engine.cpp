@@ -56,10 +56,30 @@ struct BlockOp {
 };
 using OpPtr = std::shared_ptr<BlockOp>;
 
+// Piece of an enlarged basis: block sector a times site state s, starting at
+// row `off` of its enlarged sector (a outer, s inner: the reference's
+// Kronecker order restricted to the sector, src/DMRG_CONNECT.f90:75-88).
+struct Piece {
+  int a, s, off;
+};
+
+// How a block's basis comes from the enlargement of its parent: sector k
+// holds the kept states of enlarged sector src[k] as the columns of
+// U_k (dim[k] x m[k], row-major at uoff[k], descending rho eigenvalue).  A
+// block that was not renormalized (the enlarged block itself) carries
+// identity blocks.
+struct Basis {
+  std::vector<int> src, dim, m;
+  std::vector<int64_t> uoff;
+  DevArr<double> U;
+};
+
 struct Block {
   int length = 0;
   std::vector<Sector> sec;
   std::map<std::string, OpPtr> ops;
+  std::vector<std::vector<Piece>> pieces;  // set on enlarged blocks: pieces of each sector
+  std::shared_ptr<Basis> from;             // set on renormalized / stored blocks
   int dim() const {
     int d = 0;
     for (auto& s : sec) d += s.dim;
@@ -243,7 +263,6 @@ struct Engine {
   Block enlarge(const Block& b, bool is_left) {
     Block e;
     e.length = b.length + 1;
-    struct Piece { int a, s, off; };
     std::vector<std::vector<Piece>> pieces;
     for (size_t a = 0; a < b.sec.size(); ++a)
       for (int s = 0; s < site.dim; ++s) {
@@ -369,7 +388,29 @@ struct Engine {
     }
     for (auto& kv : e.ops) make_fp32(*kv.second);
     NSS_HIP(hipStreamSynchronize(ctx->stream));
+    e.pieces = std::move(pieces);
     return e;
+  }
+
+  // Identity basis record of a block that was not renormalized.
+  std::shared_ptr<Basis> identity_basis(const Block& b) {
+    auto B = std::make_shared<Basis>();
+    int64_t tot = 0;
+    for (size_t k = 0; k < b.sec.size(); ++k) {
+      const int D = b.sec[k].dim;
+      B->src.push_back((int)k);
+      B->dim.push_back(D);
+      B->m.push_back(D);
+      B->uoff.push_back(tot);
+      tot += (int64_t)D * D;
+    }
+    std::vector<double> h(std::max<int64_t>(1, tot), 0.0);
+    for (size_t k = 0; k < b.sec.size(); ++k)
+      for (int i = 0; i < B->dim[k]; ++i) h[B->uoff[k] + (int64_t)i * B->dim[k] + i] = 1.0;
+    B->U.resize(ctx->pool, h.size());
+    NSS_HIP(hipMemcpyAsync(B->U.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return B;
   }
 
   // ------------------------------------------------------------ superblock
@@ -606,9 +647,143 @@ struct Engine {
     launch_gemm_f64(p2, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
     for (size_t i = 0; i < outs.size(); ++i) N.ops[names[i]] = outs[i];
     NSS_HIP(hipStreamSynchronize(ctx->stream));
+    auto B = std::make_shared<Basis>();
+    B->src = src_sector;
+    for (size_t k = 0; k < N.sec.size(); ++k) {
+      B->dim.push_back(E.sec[src_sector[k]].dim);
+      B->m.push_back(N.sec[k].dim);
+    }
+    B->uoff = uoff;
+    B->U = std::move(du);
+    N.from = B;
     ms_out = ms;
     trunc = te;
     return N;
+  }
+
+  // ------------------------------------------------------------ prediction
+  // Ground state of the previous finite step and the basis maps around it.
+  struct PrevStep {
+    bool valid = false;
+    int label = 0, neig = 0;
+    int64_t n = 0;
+    std::vector<int> sbL, sbR, Dl, Dr;
+    std::vector<std::vector<Piece>> piecesL, piecesR;  // its enlarged blocks
+    std::shared_ptr<Basis> unenlL, unenlR;             // blocks that were enlarged
+    std::shared_ptr<Basis> newL, newR;                 // blocks after renormalization
+    DevArr<double> psi, sum;
+  } prev;
+
+  // White's wavefunction transformation between two finite steps in the
+  // same direction.  Left->right (label 1): the previous ground state
+  // Psi[(a,s1),(s2,b)] becomes
+  //     Psi'[(k,s2),(c,s3)] = sum U_L[(a,s1),k] Psi[(a,s1),(s2,b)] U_R[(c,s3),b]
+  // with U_L the kept states of the block just renormalized and U_R the
+  // matrix that built the previous right block from the enlargement of the
+  // current one; right->left (label 2) is the mirror image.  Both products
+  // are sector-blocked grouped GEMMs: T = U_L^T Psi_q (or Psi_q U_R), then
+  // one product per piece of the far enlarged sector into its output block.
+  // Writes the predicted vector (the sum over the previous eigenvectors when
+  // LANC_NEIGEN > 1) to out; returns false when no prediction applies.
+  bool predict_psi(int label, double* out, int64_t n_new) {
+    const Basis* rot = label == 1 ? prev.newL.get() : prev.newR.get();
+    const Basis* far = label == 1 ? prev.unenlR.get() : prev.unenlL.get();
+    if (!rot || !far) return false;
+    const Block& near_enl = label == 1 ? left : right;
+    const Block& far_enl = label == 1 ? right : left;
+    const std::vector<std::vector<Piece>>& prev_near_pieces = label == 1 ? prev.piecesL : prev.piecesR;
+    const std::vector<std::vector<Piece>>& prev_far_pieces = label == 1 ? prev.piecesR : prev.piecesL;
+    // the far block's map must refer to the enlargement used now
+    for (size_t b = 0; b < far->src.size(); ++b)
+      if (far->src[b] >= (int)far_enl.sec.size() || far_enl.sec[far->src[b]].dim != far->dim[b]) return false;
+    std::vector<int> k_of_e(prev_near_pieces.size(), -1);
+    for (size_t k = 0; k < rot->src.size(); ++k)
+      if (rot->src[k] < (int)k_of_e.size()) k_of_e[rot->src[k]] = (int)k;
+    std::map<std::pair<int, int>, std::pair<int, int>> piece_at;  // (block sector, site state) -> (sector, offset)
+    for (size_t c = 0; c < near_enl.pieces.size(); ++c)
+      for (const Piece& pc : near_enl.pieces[c]) piece_at[{pc.a, pc.s}] = {(int)c, pc.off};
+    const std::vector<int>& sb_near = label == 1 ? sbL : sbR;
+    const std::vector<int>& sb_far = label == 1 ? sbR : sbL;
+    std::vector<int> q_of_near(near_enl.sec.size(), -1);
+    for (size_t q = 0; q < sb_near.size(); ++q) q_of_near[sb_near[q]] = (int)q;
+    std::vector<int64_t> noff(lastDl.size() + 1, 0), poff(prev.Dl.size() + 1, 0);
+    for (size_t q = 0; q < lastDl.size(); ++q) noff[q + 1] = noff[q] + (int64_t)lastDl[q] * lastDr[q];
+    for (size_t q = 0; q < prev.Dl.size(); ++q) poff[q + 1] = poff[q] + (int64_t)prev.Dl[q] * prev.Dr[q];
+    if (noff.back() != n_new || poff.back() != prev.n) return false;
+    hipStream_t st = ctx->stream;
+    const double* x = prev.psi.p;
+    if (prev.neig > 1) {  // sum of the previous eigenvectors: w = x_0 - X_{1..} (-1)
+      prev.sum.resize(ctx->pool, prev.n + prev.neig);
+      std::vector<double> m1(prev.neig, -1.0);
+      NSS_HIP(hipMemcpyAsync(prev.sum.p + prev.n, m1.data(), prev.neig * sizeof(double), hipMemcpyHostToDevice, st));
+      NSS_HIP(hipMemcpyAsync(prev.sum.p, prev.psi.p, prev.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      launch_gemv_n(prev.psi.p + prev.n, prev.n, prev.neig - 1, prev.sum.p + prev.n, prev.sum.p, prev.n, 0, st);
+      x = prev.sum.p;
+    }
+    GemmPlan p1, p2;
+    std::vector<int64_t> toff(prev.Dl.size(), -1);
+    int64_t ttot = 0;
+    for (size_t q = 0; q < prev.Dl.size(); ++q) {
+      const int k = k_of_e[label == 1 ? prev.sbL[q] : prev.sbR[q]];
+      if (k < 0) continue;
+      toff[q] = ttot;
+      ttot += (int64_t)rot->m[k] * (label == 1 ? prev.Dr[q] : prev.Dl[q]);
+    }
+    DevArr<double> T;
+    T.resize(ctx->pool, std::max<int64_t>(1, ttot));
+    int pieces_done = 0;
+    for (size_t q = 0; q < prev.Dl.size(); ++q) {
+      if (toff[q] < 0) continue;
+      const int eL = prev.sbL[q], eR = prev.sbR[q];
+      const int k = k_of_e[label == 1 ? eL : eR];
+      const int mk = rot->m[k], Dl = prev.Dl[q], Dr = prev.Dr[q];
+      const double* Uk = rot->U.p + rot->uoff[k];
+      const double* Psi = x + poff[q];
+      double* Tq = T.p + toff[q];
+      GemmSeg g{};
+      g.abase = g.bbase = BASE_ABS;
+      g.alpha = 1.0;
+      if (label == 1) {  // T_q = U_k^T Psi_q  (m_k x Dr)
+        g.a = Uk; g.lda = mk; g.flags = SEG_TA;
+        g.b = Psi; g.ldb = Dr; g.K = Dl;
+        p1.add_output(BASE_ABS, Tq, 0, Dr, mk, Dr, {g});
+      } else {           // T_q = Psi_q U_k  (Dl x m_k)
+        g.a = Psi; g.lda = Dr;
+        g.b = Uk; g.ldb = mk; g.K = Dr;
+        p1.add_output(BASE_ABS, Tq, 0, mk, Dl, mk, {g});
+      }
+      for (const Piece& pc : prev_far_pieces[label == 1 ? eR : eL]) {
+        auto it = piece_at.find({k, pc.s});
+        if (it == piece_at.end()) continue;
+        const int c = it->second.first, o = it->second.second;
+        const int q2 = q_of_near[c];
+        if (q2 < 0 || sb_far[q2] != far->src[pc.a]) continue;
+        const int mb = far->m[pc.a], Db = far->dim[pc.a];
+        const double* Ub = far->U.p + far->uoff[pc.a];
+        GemmSeg h{};
+        h.abase = h.bbase = BASE_ABS;
+        h.alpha = 1.0;
+        h.K = mb;
+        if (label == 1) {  // rows [o, o+m_k) of sector q2: T_q[:, cols of piece] U_b^T
+          h.a = Tq + pc.off; h.lda = Dr;
+          h.b = Ub; h.ldb = mb; h.flags = SEG_TB;
+          p2.add_output(BASE_ABS, out + noff[q2] + (int64_t)o * lastDr[q2], 0, lastDr[q2], mk, Db, {h});
+        } else {           // cols [o, o+m_k) of sector q2: U_a T_q[rows of piece, :]
+          h.a = Ub; h.lda = mb;
+          h.b = Tq + (int64_t)pc.off * mk; h.ldb = mk;
+          p2.add_output(BASE_ABS, out + noff[q2] + o, 0, lastDr[q2], Db, mk, {h});
+        }
+        ++pieces_done;
+      }
+    }
+    if (pieces_done == 0) return false;
+    NSS_HIP(hipMemsetAsync(out, 0, n_new * sizeof(double), st));
+    p1.upload(ctx->pool, st);
+    p2.upload(ctx->pool, st);
+    launch_gemm_f64(p1, Bases{nullptr, nullptr, nullptr}, st, GEMM_GENERIC);
+    launch_gemm_f64(p2, Bases{nullptr, nullptr, nullptr}, st, GEMM_GENERIC);
+    NSS_HIP(hipStreamSynchronize(st));
+    return true;
   }
 
   // ------------------------------------------------------------ step
@@ -636,6 +811,7 @@ struct Engine {
     I.label = label;
     I.sweep = sweep + (type == 'f' ? 1 : 0);
     PhaseTimer pt(ctx, "step.enlarge");
+    std::shared_ptr<Basis> fromL0 = left.from, fromR0 = right.from;
     left = enlarge(left, true);
     right = enlarge(right, false);
     pt.next("step.setup");
@@ -660,8 +836,15 @@ struct Engine {
       lo = dense_diag(ctx, ctx->sb, neig, psi.p);
       I.numop = 0;
     } else {
-      lo = lanczos_device(ctx, ctx->sb, neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed,
-                          par.v0_mode, par.precision, psi.p);
+      int v0 = par.v0_mode;
+      // one eigenpair only: with several, a start biased towards the previous
+      // states can let a higher state pass the residual test before a missed
+      // lower one surfaces, so those steps keep the random start
+      if (type == 'f' && par.predict && neig == 1 && prev.valid && prev.label == label &&
+          predict_psi(label, psi.p, msb))
+        v0 = 2;
+      lo = lanczos_device(ctx, ctx->sb, neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed, v0,
+                          par.precision, psi.p);
       I.numop = lo.numop + lo.numop_fp32;
       I.numop_fp32 = lo.numop_fp32;
     }
@@ -699,6 +882,28 @@ struct Engine {
       I.m_right = mr;
       I.trunc_left = tl;
       I.trunc_right = tr;
+    } else {
+      left.from = identity_basis(left);
+      right.from = identity_basis(right);
+    }
+    prev.valid = type == 'f' && par.predict;
+    if (prev.valid) {
+      prev.label = label;
+      prev.neig = neig;
+      prev.n = msb;
+      prev.sbL = sbL;
+      prev.sbR = sbR;
+      prev.Dl = lastDl;
+      prev.Dr = lastDr;
+      prev.piecesL = lastL.pieces;
+      prev.piecesR = lastR.pieces;
+      prev.unenlL = fromL0;
+      prev.unenlR = fromR0;
+      prev.newL = left.from;
+      prev.newR = right.from;
+      prev.psi.resize(ctx->pool, (size_t)neig * msb);
+      NSS_HIP(hipMemcpyAsync(prev.psi.p, psi.p, (size_t)neig * msb * sizeof(double), hipMemcpyDeviceToDevice,
+                             ctx->stream));
     }
     pt.next("step.tail");
     const auto t5 = clk::now();

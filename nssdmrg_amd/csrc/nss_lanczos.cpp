@@ -255,8 +255,21 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   L.scal.resize(ctx->pool, 4);
   DevArr<double> v0;
   v0.resize(ctx->pool, n);
-  if (v0_mode == 1) launch_fill_const(v0.p, n, 1.0 / std::sqrt((double)n), ctx->stream);
-  else launch_fill_splitmix(v0.p, n, seed, ctx->stream);
+  bool filled = false;
+  if (v0_mode == 2) {
+    // caller-supplied start vector in xdev (a predicted wavefunction); a
+    // vanishing or non-finite one falls back to the seeded random start
+    NSS_HIP(hipMemcpyAsync(v0.p, xdev, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_norm(v0.p, n, L.part.p, L.scal.p, ctx->stream);
+    double nrm = 0.0;
+    NSS_HIP(hipMemcpyAsync(&nrm, L.scal.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    filled = std::isfinite(nrm) && nrm > 1e-8;
+  }
+  if (!filled) {
+    if (v0_mode == 1) launch_fill_const(v0.p, n, 1.0 / std::sqrt((double)n), ctx->stream);
+    else launch_fill_splitmix(v0.p, n, seed, ctx->stream);
+  }
   LanczosOut out;
   if (precision == NSS_FP32_MIXED) {
     // FP32-MFMA H*v to an FP32-attainable residual, then the FP64 restart

@@ -843,9 +843,13 @@ extern "C" int nss_lanczos(nss_ctx* ctx, int neigen, int kmax, int maxiter, doub
   SbDevice* sb = ctx->sb;
   NSS_REQUIRE(sb->n > 0 && !sb->Y64.htiles.empty(), NSS_E_STATE, "nss_lanczos: no superblock uploaded");
   NSS_REQUIRE(neigen >= 1 && neigen <= sb->n, NSS_E_ARG, "nss_lanczos: neigen out of range");
+  NSS_REQUIRE(v0_mode >= 0 && v0_mode <= 2, NSS_E_ARG, "nss_lanczos: v0_mode must be 0, 1 or 2");
+  NSS_REQUIRE(v0_mode != 2 || evecs, NSS_E_ARG, "nss_lanczos: v0_mode 2 reads the start vector from evecs");
   NSS_HIP(hipSetDevice(ctx->device));
   DevArr<double> x;
   x.resize(ctx->pool, (size_t)neigen * sb->n);
+  if (v0_mode == 2)
+    NSS_HIP(hipMemcpyAsync(x.p, evecs, sb->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   LanczosOut r = lanczos_device(ctx, sb, neigen, kmax, maxiter, tol, seed, v0_mode, precision, x.p);
   for (int i = 0; i < neigen; ++i) evals[i] = r.evals[i];
   if (evecs)

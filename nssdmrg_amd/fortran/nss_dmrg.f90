@@ -41,7 +41,7 @@ module nss_dmrg
      integer(c_int64_t) :: seed = 0
      integer(c_int32_t) :: v0_mode = 0
      real(c_double) :: deg_thr = 0.1d0
-     integer(c_int32_t) :: precision = 0, verbose = 0
+     integer(c_int32_t) :: precision = 0, verbose = 0, predict = 0
   end type nss_params
 
   type, bind(C), public :: nss_step_info
