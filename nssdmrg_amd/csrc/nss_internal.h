@@ -228,6 +228,12 @@ struct nss_ctx {
   hipStream_t stream = nullptr;
   nss::DevicePool* pool = nullptr;
   void* rocblas = nullptr;  // rocblas_handle, created lazily
+  // auxiliary streams (default priority), each with its own rocblas handle,
+  // created lazily: independent batches of the rho eigensolve run on them
+  // concurrently with the context stream
+  static constexpr int NAUX = 3;
+  hipStream_t aux[NAUX] = {};
+  void* aux_rocblas[NAUX] = {};
   nss::SbDevice* sb = nullptr;
   // H*v timing with HIP events on the context stream: 3 events per call
   // (start, after the W launch, after the Y launch).

@@ -381,6 +381,29 @@ void nss_release_rocblas(nss_ctx* ctx) {
     (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
     ctx->rocblas = nullptr;
   }
+  for (int k = 0; k < nss_ctx::NAUX; ++k) {
+    if (ctx->aux_rocblas[k]) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->aux_rocblas[k]));
+    ctx->aux_rocblas[k] = nullptr;
+    if (ctx->aux[k]) {
+      (void)hipStreamSynchronize(ctx->aux[k]);
+      (void)hipStreamDestroy(ctx->aux[k]);
+    }
+    ctx->aux[k] = nullptr;
+  }
+}
+
+// Auxiliary stream k with its rocblas handle bound to it (created on first use).
+static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
+  if (!ctx->aux[k]) NSS_HIP(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
+  if (!ctx->aux_rocblas[k]) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(NSS_E_HIP, "rocblas_create_handle failed");
+    ctx->aux_rocblas[k] = h;
+    if (rocblas_set_stream(h, ctx->aux[k]) != rocblas_status_success)
+      throw Error(NSS_E_HIP, "rocblas_set_stream failed");
+  }
+  *st = ctx->aux[k];
+  return static_cast<rocblas_handle>(ctx->aux_rocblas[k]);
 }
 
 // rho blocks of both sides in one batch: items (side, sector) share one
@@ -499,56 +522,67 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     launch_jacobi_eig(sb->rho.p, djoff.p, djdim.p, djeoff.p, sb->rho.p, dD.p, nb, jn, ctx->stream);
   }
   // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
-  // size class (dims rounded up to a multiple of 64, padded slabs)
+  // size class (dims rounded up to a multiple of 64, padded slabs).  The
+  // classes run on the auxiliary streams, concurrently with the Jacobi
+  // launch on the context stream (rho is complete: the symmetry check above
+  // synchronised the context stream; every batch writes disjoint rows of
+  // rho and of the eigenvalue array).
   pt.next("rdm.rocsolver");
   std::map<int, std::vector<int>> classes;
   for (int i = 0; i < ni; ++i)
     if (items[i].kind == 2) classes[(items[i].d + 63) / 64 * 64].push_back(i);
-  if (!classes.empty()) {
-    rocblas_handle hnd = static_cast<rocblas_handle>(ctx->rocblas);
-    if (!hnd) {
-      if (rocblas_create_handle(&hnd) != rocblas_status_success)
-        throw Error(NSS_E_HIP, "rocblas_create_handle failed");
-      ctx->rocblas = hnd;
-    }
-    if (rocblas_set_stream(hnd, ctx->stream) != rocblas_status_success)
-      throw Error(NSS_E_HIP, "rocblas_set_stream failed");
+  struct ClassBatch {
+    int ncls = 0, nb = 0;
+    hipStream_t st = nullptr;
+    std::vector<int64_t> co, ceo;
+    std::vector<int32_t> cd, ci;
+    DevArr<int64_t> dco, dceo;
+    DevArr<int32_t> dcd, dci;
+    DevArr<double> slab, sD, sE;
+  };
+  std::vector<ClassBatch> batches(classes.size());
+  bool used_aux[nss_ctx::NAUX] = {};
+  {
+    size_t c = 0;
     for (auto& kv : classes) {
-      const int ncls = kv.first;
-      const int nb = (int)kv.second.size();
-      std::vector<int64_t> co, ceo;
-      std::vector<int32_t> cd;
+      ClassBatch& B = batches[c];
+      const int k = (int)(c % nss_ctx::NAUX);
+      rocblas_handle hnd = aux_handle(ctx, k, &B.st);
+      used_aux[k] = true;
+      ++c;
+      B.ncls = kv.first;
+      B.nb = (int)kv.second.size();
       for (int i : kv.second) {
-        co.push_back(items[i].roff);
-        ceo.push_back(items[i].eoff);
-        cd.push_back(items[i].d);
+        B.co.push_back(items[i].roff);
+        B.ceo.push_back(items[i].eoff);
+        B.cd.push_back(items[i].d);
       }
-      DevArr<int64_t> dco, dceo;
-      DevArr<int32_t> dcd, dci;
-      DevArr<double> slab, sD, sE;
-      dco.resize(ctx->pool, nb);
-      dceo.resize(ctx->pool, nb);
-      dcd.resize(ctx->pool, nb);
-      dci.resize(ctx->pool, nb);
-      slab.resize(ctx->pool, (size_t)nb * ncls * ncls);
-      sD.resize(ctx->pool, (size_t)nb * ncls);
-      sE.resize(ctx->pool, (size_t)nb * ncls);
-      NSS_HIP(hipMemcpyAsync(dco.p, co.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
-      NSS_HIP(hipMemcpyAsync(dceo.p, ceo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
-      NSS_HIP(hipMemcpyAsync(dcd.p, cd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-      launch_pad_blocks(sb->rho.p, dco.p, dcd.p, nb, ncls, slab.p, ctx->stream);
+      const int nb = B.nb, ncls = B.ncls;
+      B.ci.assign(nb, 0);
+      B.dco.resize(ctx->pool, nb);
+      B.dceo.resize(ctx->pool, nb);
+      B.dcd.resize(ctx->pool, nb);
+      B.dci.resize(ctx->pool, nb);
+      B.slab.resize(ctx->pool, (size_t)nb * ncls * ncls);
+      B.sD.resize(ctx->pool, (size_t)nb * ncls);
+      B.sE.resize(ctx->pool, (size_t)nb * ncls);
+      NSS_HIP(hipMemcpyAsync(B.dco.p, B.co.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, B.st));
+      NSS_HIP(hipMemcpyAsync(B.dceo.p, B.ceo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, B.st));
+      NSS_HIP(hipMemcpyAsync(B.dcd.p, B.cd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, B.st));
+      launch_pad_blocks(sb->rho.p, B.dco.p, B.dcd.p, nb, ncls, B.slab.p, B.st);
       const rocblas_status st = rocsolver_dsyevd_strided_batched(
-          hnd, rocblas_evect_original, rocblas_fill_upper, ncls, slab.p, ncls, (rocblas_stride)ncls * ncls, sD.p,
-          ncls, sE.p, ncls, dci.p, nb);
+          hnd, rocblas_evect_original, rocblas_fill_upper, ncls, B.slab.p, ncls, (rocblas_stride)ncls * ncls, B.sD.p,
+          ncls, B.sE.p, ncls, B.dci.p, nb);
       if (st != rocblas_status_success) throw Error(NSS_E_HIP, "rocsolver_dsyevd_strided_batched failed");
-      launch_unpad_eig(slab.p, sD.p, ncls, dco.p, dcd.p, dceo.p, nb, sb->rho.p, dD.p, ctx->stream);
-      std::vector<int32_t> ci(nb);
-      NSS_HIP(hipMemcpyAsync(ci.data(), dci.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-      NSS_HIP(hipStreamSynchronize(ctx->stream));
-      for (int c = 0; c < nb; ++c)
-        if (ci[c] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
+      launch_unpad_eig(B.slab.p, B.sD.p, ncls, B.dco.p, B.dcd.p, B.dceo.p, nb, sb->rho.p, dD.p, B.st);
+      NSS_HIP(hipMemcpyAsync(B.ci.data(), B.dci.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, B.st));
     }
   }
+  for (int k = 0; k < nss_ctx::NAUX; ++k)
+    if (used_aux[k]) NSS_HIP(hipStreamSynchronize(ctx->aux[k]));
+  for (const ClassBatch& B : batches)
+    for (int c = 0; c < B.nb; ++c)
+      if (B.ci[c] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
   pt.next("rdm.download");
   std::vector<double> h(std::max<int64_t>(1, tot)), hD(std::max<int64_t>(1, etot));
   if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
