@@ -509,15 +509,16 @@ struct Engine {
 
   // ------------------------------------------------------------ renormalize
   struct Spectrum {
-    std::vector<std::vector<double>> ev, vec;  // per SB sector (ascending)
+    std::vector<std::vector<double>> ev;       // per SB sector (ascending)
+    std::vector<RhoVec> dv;                    // eigenvectors on the device, per SB sector
     std::vector<double> all_desc;
     std::vector<std::pair<int, int>> order;    // (block, position) in descending order
   };
 
-  static Spectrum sort_spectrum(std::vector<std::vector<double>>&& ev, std::vector<std::vector<double>>&& vec) {
+  static Spectrum sort_spectrum(std::vector<std::vector<double>>&& ev, std::vector<RhoVec>&& dv) {
     Spectrum s;
     s.ev = std::move(ev);
-    s.vec = std::move(vec);
+    s.dv = std::move(dv);
     std::vector<double> all;
     std::vector<std::pair<int, int>> pos;
     for (size_t b = 0; b < s.ev.size(); ++b)
@@ -588,19 +589,39 @@ struct Engine {
       uoff[k] = utot;
       utot += (int64_t)E.sec[src_sector[k]].dim * N.sec[k].dim;
     }
-    std::vector<double> hu(std::max<int64_t>(1, utot), 0.0);
-    for (size_t k = 0; k < N.sec.size(); ++k) {
-      const int b = blk_of_new[k];
-      const int D = E.sec[src_sector[k]].dim, m = N.sec[k].dim;
-      const std::vector<double>& V = sp.vec[b];
-      for (int c = 0; c < m; ++c) {
-        const int col = D - 1 - c;  // largest first
-        for (int r = 0; r < D; ++r) hu[uoff[k] + (size_t)r * m + c] = V[(size_t)r * D + col];
-      }
-    }
+    // gathered on the device from the eigenvector blocks of the rho solve
     DevArr<double> du;
-    du.resize(ctx->pool, hu.size());
-    NSS_HIP(hipMemcpyAsync(du.p, hu.data(), hu.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    du.resize(ctx->pool, std::max<int64_t>(1, utot));
+    std::vector<int32_t> hcols;
+    std::vector<GatherTask> gt;
+    int max_el = 0;
+    for (size_t k = 0; k < N.sec.size(); ++k) {
+      const RhoVec& V = sp.dv[blk_of_new[k]];
+      const int D = E.sec[src_sector[k]].dim, m = N.sec[k].dim;
+      NSS_REQUIRE(V.base && V.d == D, NSS_E_STATE, "renormalize: rho eigenvectors missing");
+      GatherTask t{};
+      t.dst = du.p + uoff[k];
+      t.src = V.base;
+      t.si = V.si;
+      t.sj = V.sj;
+      t.cols = reinterpret_cast<const int32_t*>(hcols.size());  // offset, rebased below
+      t.rows = D;
+      t.ncol = m;
+      for (int c = 0; c < m; ++c) hcols.push_back(V.order[D - 1 - c]);  // largest first
+      gt.push_back(t);
+      max_el = std::max(max_el, D * m);
+    }
+    DevArr<int32_t> dcols;
+    DevArr<GatherTask> dgt;
+    dcols.resize(ctx->pool, std::max<size_t>(1, hcols.size()));
+    for (GatherTask& t : gt) t.cols = dcols.p + reinterpret_cast<size_t>(t.cols);
+    dgt.resize(ctx->pool, std::max<size_t>(1, gt.size()));
+    if (!gt.empty()) {
+      NSS_HIP(hipMemcpyAsync(dcols.p, hcols.data(), hcols.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                             ctx->stream));
+      NSS_HIP(hipMemcpyAsync(dgt.p, gt.data(), gt.size() * sizeof(GatherTask), hipMemcpyHostToDevice, ctx->stream));
+      launch_gather_cols(dgt.p, (int)gt.size(), max_el, ctx->stream);
+    }
     // map enlarged sector -> new sector
     std::vector<int> new_of_enl(E.sec.size(), -1);
     for (size_t k = 0; k < N.sec.size(); ++k) new_of_enl[src_sector[k]] = (int)k;
@@ -853,11 +874,12 @@ struct Engine {
     for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = lo.evals[i];
     // sb_get_rdm
     pt.next("step.rdm");
-    std::vector<std::vector<double>> evs[2], vecs[2];
-    sb_rdm_eigh2(ctx, ctx->sb, psi.p, 3, evs, vecs);
+    std::vector<std::vector<double>> evs[2], unused[2];
+    std::vector<RhoVec> dvs[2];
+    sb_rdm_eigh2(ctx, ctx->sb, psi.p, 3, evs, unused, dvs);
     pt.next("step.spectrum");
-    Spectrum sl = sort_spectrum(std::move(evs[0]), std::move(vecs[0]));
-    Spectrum sr = sort_spectrum(std::move(evs[1]), std::move(vecs[1]));
+    Spectrum sl = sort_spectrum(std::move(evs[0]), std::move(dvs[0]));
+    Spectrum sr = sort_spectrum(std::move(evs[1]), std::move(dvs[1]));
     const auto t4 = clk::now();
     I.t_rdm = sec(t3, t4);
     I.entropy_left = entropy(sl.all_desc);

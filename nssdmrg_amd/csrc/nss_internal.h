@@ -332,7 +332,29 @@ const std::vector<int64_t>& sb_off(const SbDevice* sb);
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
                  std::vector<std::vector<double>>& evals, std::vector<std::vector<double>>& evecs);
 // both sides at once (sides bit 0: left, bit 1: right)
+// Device view of one rho block's eigenvectors: component i of eigenvector
+// j at base[i*si + j*sj]; order[k] = index j of the k-th smallest eigenvalue.
+struct RhoVec {
+  const double* base = nullptr;
+  int d = 0;
+  int64_t si = 0, sj = 0;
+  std::vector<int> order;
+};
+// rho_L / rho_R blocks of psi (sides: bit 0 left, bit 1 right) and their
+// eigenpairs, eigenvalues ascending on the host.  With dvecs == nullptr the
+// eigenvectors come to the host (evecs: row-major, eigenvectors as columns);
+// otherwise they stay in the superblock's rho buffer (valid until the next
+// call) and dvecs[side][q] describes them.
 void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
-                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2]);
+                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
+                  std::vector<RhoVec>* dvecs = nullptr);
+struct GatherTask {  // dst[r*ncol + c] = src[r*si + cols[c]*sj]
+  double* dst;
+  const double* src;
+  int64_t si, sj;
+  const int32_t* cols;
+  int32_t rows, ncol;
+};
+void launch_gather_cols(const GatherTask* tasks, int ntasks, int max_elems, hipStream_t s);
 
 }  // namespace nss

@@ -903,6 +903,24 @@ __global__ __launch_bounds__(256) void k_unpad_eig(const double* __restrict__ ve
     for (int k = threadIdx.x; k < n; k += 256) w[eoff[b] + k] = vals[(size_t)b * ncls + ncls - n + k];
 }
 
+// Column gather for block rotations: task t copies the selected columns of
+// a strided source into a dense row-major (rows x ncol) destination.
+__global__ __launch_bounds__(256) void k_gather_cols(const GatherTask* __restrict__ tasks) {
+  const GatherTask t = tasks[blockIdx.y];
+  const int64_t ne = (int64_t)t.rows * t.ncol;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < ne; e += (int64_t)gridDim.x * 256) {
+    const int r = (int)(e / t.ncol), c = (int)(e % t.ncol);
+    t.dst[e] = t.src[r * t.si + (int64_t)t.cols[c] * t.sj];
+  }
+}
+
+void launch_gather_cols(const GatherTask* tasks, int ntasks, int max_elems, hipStream_t s) {
+  if (ntasks <= 0 || max_elems <= 0) return;
+  const unsigned gx = (unsigned)std::min<int64_t>(256, ((int64_t)max_elems + 255) / 256);
+  hipLaunchKernelGGL(k_gather_cols, dim3(gx, ntasks), dim3(256), 0, s, tasks);
+  NSS_HIP(hipGetLastError());
+}
+
 void launch_pad_blocks(const double* src, const int64_t* off, const int32_t* dim, int nb, int ncls, double* dst,
                        hipStream_t s) {
   const unsigned gx = (unsigned)std::min<int64_t>(64, ((int64_t)ncls * ncls + 255) / 256);

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <mutex>
+#include <string>
 
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -392,6 +393,16 @@ void nss_release_rocblas(nss_ctx* ctx) {
   }
 }
 
+// Largest rho block sent to the batched Jacobi kernel (larger ones go to
+// rocSOLVER on the auxiliary streams): NSS_JACOBI_MAX, default 64.
+static int jacobi_limit() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_JACOBI_MAX");
+    return e ? std::max(0, std::atoi(e)) : 64;
+  }();
+  return v;
+}
+
 // Auxiliary stream k with its rocblas handle bound to it (created on first use).
 static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
   if (!ctx->aux[k]) NSS_HIP(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
@@ -410,7 +421,8 @@ static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
 // rho buffer, one GEMM launch, one symmetry check, one Jacobi launch and one
 // strided-batched rocSOLVER call per size class.
 void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
-                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2]) {
+                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
+                  std::vector<RhoVec>* dvecs) {
   const int nq = (int)sb->Dl.size();
   struct Item {
     int side, q, d;
@@ -484,7 +496,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   // limit with the batched GPU Jacobi kernel, larger ones with rocSOLVER
   // dsyevd; device results overwrite rho in place
   pt.next("rdm.jacobi");
-  const int jmax = jacobi_max_dim(ctx->device);
+  const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff;
   std::vector<int32_t> jdim;
@@ -584,40 +596,85 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     for (int c = 0; c < B.nb; ++c)
       if (B.ci[c] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
   pt.next("rdm.download");
-  std::vector<double> h(std::max<int64_t>(1, tot)), hD(std::max<int64_t>(1, etot));
-  if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<double> hD(std::max<int64_t>(1, etot));
   if (etot) NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  // eigenvectors to the host: all of rho (host_vectors) or only the tiny
+  // host-solved blocks (device views)
+  std::vector<double> h;
+  if (!dvecs) {
+    h.resize(std::max<int64_t>(1, tot));
+    if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    h.resize(std::max<int64_t>(1, tot));  // only the kind-0 ranges are filled
+    for (const Item& it : items)
+      if (it.kind == 0)
+        NSS_HIP(hipMemcpyAsync(h.data() + it.roff, sb->rho.p + it.roff, (size_t)it.d * it.d * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   pt.next("rdm.host");
+  if (dvecs) {
+    dvecs[0].assign(nq, {});
+    dvecs[1].assign(nq, {});
+  }
+  bool uploaded = false;
   for (const Item& it : items) {
     const int d = it.d;
     std::vector<double>& ev = evals[it.side][it.q];
-    std::vector<double>& vec = evecs[it.side][it.q];
     const double* a = h.data() + it.roff;
     ev.resize(d);
-    vec.resize((size_t)d * d);
+    RhoVec* rv = dvecs ? &dvecs[it.side][it.q] : nullptr;
+    std::vector<double>* vec = dvecs ? nullptr : &evecs[it.side][it.q];
+    if (vec) vec->resize((size_t)d * d);
+    if (rv) {
+      rv->base = sb->rho.p + it.roff;
+      rv->d = d;
+      rv->order.resize(d);
+      for (int j = 0; j < d; ++j) rv->order[j] = j;
+    }
     if (it.kind == 2) {
+      // rocSOLVER: eigenvector j contiguous (component i at j*d + i), ascending
       std::copy(hD.begin() + it.eoff, hD.begin() + it.eoff + d, ev.begin());
-      // column-major eigenvectors -> row-major with eigenvectors as columns
-      for (int i = 0; i < d; ++i)
-        for (int j = 0; j < d; ++j) vec[(size_t)i * d + j] = a[(size_t)j * d + i];
+      if (vec)
+        for (int i = 0; i < d; ++i)
+          for (int j = 0; j < d; ++j) (*vec)[(size_t)i * d + j] = a[(size_t)j * d + i];
+      if (rv) {
+        rv->si = 1;
+        rv->sj = d;
+      }
     } else if (it.kind == 1) {
-      // Jacobi: unsorted eigenpairs, row-major columns -> ascending order
+      // Jacobi: row-major, eigenvectors as columns, unsorted -> ascending order
       std::vector<int> idx(d);
       for (int i = 0; i < d; ++i) idx[i] = i;
       const double* w = hD.data() + it.eoff;
       std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
       for (int j = 0; j < d; ++j) {
         ev[j] = w[idx[j]];
-        for (int i = 0; i < d; ++i) vec[(size_t)i * d + j] = a[(size_t)i * d + idx[j]];
+        if (vec)
+          for (int i = 0; i < d; ++i) (*vec)[(size_t)i * d + j] = a[(size_t)i * d + idx[j]];
+      }
+      if (rv) {
+        rv->si = d;
+        rv->sj = 1;
+        rv->order = idx;
       }
     } else {
-      std::copy(a, a + (size_t)d * d, vec.begin());
-      sym_eig(d, vec.data(), ev.data());
+      std::vector<double> v(a, a + (size_t)d * d);
+      sym_eig(d, v.data(), ev.data());
+      if (vec) *vec = v;
+      if (rv) {  // back into rho, row-major with eigenvectors as columns
+        std::copy(v.begin(), v.end(), h.begin() + it.roff);
+        NSS_HIP(hipMemcpyAsync(sb->rho.p + it.roff, h.data() + it.roff, (size_t)d * d * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+        uploaded = true;
+        rv->si = d;
+        rv->sj = 1;
+      }
     }
     for (double& e : ev)
       if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
   }
+  if (uploaded) NSS_HIP(hipStreamSynchronize(ctx->stream));  // h must outlive the copies
 }
 
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
