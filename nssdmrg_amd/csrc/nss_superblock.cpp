@@ -403,6 +403,27 @@ static int jacobi_limit() {
   return v;
 }
 
+// Padded size of a block in the batched rocSOLVER rho eigensolves.  Up to
+// 256 (rocSOLVER's single-kernel tridiagonalisation) padding costs flops, so
+// classes are multiples of 64 (NSS_RHO_CLASS overrides the granularity);
+// above it the one-stage tridiagonalisation launches a few kernels per column
+// of every class, so all larger blocks share one class, the largest of them
+// rounded up to 64 (NSS_RHO_CLASS_LARGE > 0 sets a granularity instead).
+static int rho_class(int d, int dlarge) {
+  static const int small = [] {
+    const char* e = std::getenv("NSS_RHO_CLASS");
+    const int g = e ? std::atoi(e) : 64;
+    return g >= 16 ? g : 64;
+  }();
+  static const int large = [] {
+    const char* e = std::getenv("NSS_RHO_CLASS_LARGE");
+    return e ? std::max(0, std::atoi(e)) : 0;
+  }();
+  if (d <= 256) return (d + small - 1) / small * small;
+  if (large > 0) return (d + large - 1) / large * large;
+  return (dlarge + 63) / 64 * 64;
+}
+
 // Auxiliary stream k with its rocblas handle bound to it (created on first use).
 static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
   if (!ctx->aux[k]) NSS_HIP(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
@@ -541,8 +562,11 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   // rho and of the eigenvalue array).
   pt.next("rdm.rocsolver");
   std::map<int, std::vector<int>> classes;
+  int dlarge = 0;  // largest block above the single-kernel range
+  for (const Item& it : items)
+    if (it.kind == 2 && it.d > 256) dlarge = std::max(dlarge, it.d);
   for (int i = 0; i < ni; ++i)
-    if (items[i].kind == 2) classes[(items[i].d + 63) / 64 * 64].push_back(i);
+    if (items[i].kind == 2) classes[rho_class(items[i].d, dlarge)].push_back(i);
   struct ClassBatch {
     int ncls = 0, nb = 0;
     hipStream_t st = nullptr;
