@@ -232,6 +232,15 @@ int nss_dmrg_kept_qn(nss_dmrg* d, int i, int side, double min_weight, int max, d
 int nss_dmrg_last_sb_dims(nss_dmrg* d, int* nq, int* nterms, int64_t* n);
 int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, int32_t* dst, int32_t* src,
                             double* HL, double* HR, double* A, double* B, double* psi);
+/* Measure_DMRG (src/DMRG_MEASURE.f90:148-287): <psi| O(pos) |psi> on the
+ * ground state of the last step, for nops site operators (each dim x dim,
+ * row-major, in the model's site basis: spin-1/2 |up>,|dn>; spin-1
+ * |+1>,|0>,|-1>; electron |0>,|up>,|dw>,|up dw>) at npos chain positions
+ * (1-based, 1..L_left+L_right of the last superblock).  Each operator is
+ * carried through the stored blocks' kept-state matrices on the device
+ * (Build_Op_dmrg / Advance_Op_dmrg / Average_Op_dmrg).  Operators must
+ * conserve the quantum numbers.  out[iop*npos + ipos]. */
+int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int npos, const int32_t* pos, double* out);
 int nss_dmrg_destroy(nss_dmrg* d);
 
 #ifdef __cplusplus

@@ -345,6 +345,25 @@ class DMRG:
             out.append(d)
         return out
 
+    def measure(self, ops, pos=None) -> np.ndarray:
+        """Measure_DMRG(ops, pos, avOp) (src/DMRG_MEASURE.f90:148-287) on the
+        ground state of the last step.  ops: one (dim x dim) site operator or
+        a list of them, in the site basis (spin-1/2 |up>,|dn>; spin-1
+        |+1>,|0>,|-1>; electron |0>,|up>,|dw>,|up dw>), conserving the
+        quantum numbers; pos: 1-based chain positions (default: every site
+        of the last superblock).  Returns an array (len(ops), len(pos)), or
+        (len(pos),) for a single operator."""
+        single = np.ndim(ops) == 2
+        mats = [np.asarray(ops, dtype=np.float64)] if single else [np.asarray(o, dtype=np.float64) for o in ops]
+        if pos is None:
+            pos = np.arange(1, self.steps()[-1]["L"] + 1)
+        p = np.ascontiguousarray(pos, dtype=np.int32)
+        a = np.ascontiguousarray(np.stack(mats), dtype=np.float64)
+        out = np.zeros((len(mats), len(p)))
+        F.check(F.lib().nss_dmrg_measure(self._h, len(mats), F.dptr(a), len(p),
+                                         p.ctypes.data_as(C.POINTER(C.c_int32)), F.dptr(out)))
+        return out[0] if single else out
+
     def kept_qn(self, i: int, side: str = "left", min_weight: float = 0.0) -> dict:
         """QN -> number of kept states with rho eigenvalue > min_weight."""
         n = C.c_int32()

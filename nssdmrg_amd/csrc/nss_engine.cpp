@@ -80,6 +80,9 @@ struct Block {
   std::map<std::string, OpPtr> ops;
   std::vector<std::vector<Piece>> pieces;  // set on enlarged blocks: pieces of each sector
   std::shared_ptr<Basis> from;             // set on renormalized / stored blocks
+  // the block whose enlargement this block's basis comes from (the history
+  // Measure_DMRG walks); null for the one-site block
+  std::shared_ptr<const Block> parent;
   int dim() const {
     int d = 0;
     for (auto& s : sec) d += s.dim;
@@ -260,22 +263,99 @@ struct Engine {
   };
 
   // enlarge_block, src/DMRG_CONNECT.f90:18-291 (OBC link 'n').
+  // Sectors and pieces of the enlarged basis of b (block sector outer, site
+  // state inner, grouped by QN in first-appearance order).
+  void enlarged_basis(const Block& b, std::vector<Sector>& esec, std::vector<std::vector<Piece>>& pieces) const {
+    esec.clear();
+    pieces.clear();
+    for (size_t a = 0; a < b.sec.size(); ++a)
+      for (int s = 0; s < site.dim; ++s) {
+        const QN q = b.sec[a].q + site.qn[s];
+        int k = -1;
+        for (size_t i = 0; i < esec.size(); ++i)
+          if (esec[i].q == q) { k = (int)i; break; }
+        if (k < 0) {
+          esec.push_back({q, 0});
+          pieces.emplace_back();
+          k = (int)esec.size() - 1;
+        }
+        pieces[k].push_back({(int)a, s, esec[k].dim});
+        esec[k].dim += b.sec[a].dim;
+      }
+  }
+
+  // One operator of the enlarged basis as a sum of Kronecker terms X (x) Y
+  // (X a block operator or the identity, Y a site matrix): the new operator
+  // is zeroed on the stream and its k_kron tasks are appended.
+  OpPtr kron_op(const Block& b, const std::vector<Sector>& esec, const std::vector<std::vector<Piece>>& pieces,
+                const std::vector<KTerm>& terms, QN dq, std::vector<KronTask>& tasks,
+                std::vector<KronContrib>& contribs) {
+    const int d = site.dim;
+    OpPtr op = new_op(dq, esec, esec);
+    for (const OpBlock& ob : op->blks) {
+      const int ldd = esec[ob.cs].dim;
+      for (const Piece& p : pieces[ob.rs])
+        for (const Piece& pc : pieces[ob.cs]) {
+          const int c0 = (int)contribs.size();
+          const int Da = b.sec[p.a].dim, Dc = b.sec[pc.a].dim;
+          for (const KTerm& t : terms) {
+            const double yv = t.y[p.s * d + pc.s];
+            if (yv == 0.0) continue;
+            KronContrib c{};
+            c.scale = t.coef * yv;
+            if (t.sign_row) c.scale *= fsign(b.sec[p.a].q);
+            if (t.sign_col) c.scale *= fsign(b.sec[pc.a].q);
+            if (t.x == nullptr) {
+              if (p.a != pc.a) continue;
+              c.x = nullptr;
+            } else if (!t.xtrans) {
+              const OpBlock* xb = t.x->row(p.a);
+              if (!xb || xb->cs != pc.a) continue;
+              c.x = t.x->data.p + xb->off;
+              c.ldx = Dc;
+              c.trans = 0;
+            } else {
+              const OpBlock* xb = t.x->row(pc.a);
+              if (!xb || xb->cs != p.a) continue;
+              c.x = t.x->data.p + xb->off;
+              c.ldx = Da;  // stored (pc.a x p.a) row-major: element (r,c) at [c*Da + r]
+              c.trans = 1;
+            }
+            contribs.push_back(c);
+          }
+          const int nc = (int)contribs.size() - c0;
+          if (nc == 0) continue;  // stays zero
+          double* dst = op->data.p + ob.off + (size_t)p.off * ldd + pc.off;
+          for (int r = 0; r < Da; r += KRON_TILE)
+            for (int cc = 0; cc < Dc; cc += KRON_TILE)
+              tasks.push_back({dst, ldd, Da, Dc, c0, nc, r, cc});
+        }
+    }
+    NSS_HIP(hipMemsetAsync(op->data.p, 0, op->data.n * sizeof(double), ctx->stream));
+    return op;
+  }
+
+  // Uploads the task lists and runs k_kron (synchronous).
+  void run_kron(const std::vector<KronTask>& tasks, const std::vector<KronContrib>& contribs) {
+    DevArr<KronTask> dt;
+    DevArr<KronContrib> dc;
+    dt.resize(ctx->pool, std::max<size_t>(1, tasks.size()));
+    dc.resize(ctx->pool, std::max<size_t>(1, contribs.size()));
+    if (!tasks.empty()) {
+      NSS_HIP(hipMemcpyAsync(dt.p, tasks.data(), tasks.size() * sizeof(KronTask), hipMemcpyHostToDevice,
+                             ctx->stream));
+      NSS_HIP(hipMemcpyAsync(dc.p, contribs.data(), contribs.size() * sizeof(KronContrib),
+                             hipMemcpyHostToDevice, ctx->stream));
+      launch_kron(dt.p, (int)tasks.size(), dc.p, ctx->stream);
+    }
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+  }
+
   Block enlarge(const Block& b, bool is_left) {
     Block e;
     e.length = b.length + 1;
     std::vector<std::vector<Piece>> pieces;
-    for (size_t a = 0; a < b.sec.size(); ++a)
-      for (int s = 0; s < site.dim; ++s) {
-        const QN q = b.sec[a].q + site.qn[s];
-        int k = e.find(q);
-        if (k < 0) {
-          e.sec.push_back({q, 0});
-          pieces.emplace_back();
-          k = (int)e.sec.size() - 1;
-        }
-        pieces[k].push_back({(int)a, s, e.sec[k].dim});
-        e.sec[k].dim += b.sec[a].dim;
-      }
+    enlarged_basis(b, e.sec, pieces);
     const int d = site.dim;
     auto sop = [&](const char* k) -> const SiteOp& { return site.ops.at(k); };
     std::vector<double> I(d * d, 0.0);
@@ -331,61 +411,10 @@ struct Engine {
     std::vector<KronTask> tasks;
     std::vector<KronContrib> contribs;
     for (auto& kv : defs) {
-      QN dq = kv.first == "H" ? z : site.ops.at(kv.first).dq;
-      OpPtr op = new_op(dq, e.sec, e.sec);
-      for (const OpBlock& ob : op->blks) {
-        const int ldd = e.sec[ob.cs].dim;
-        for (const Piece& p : pieces[ob.rs])
-          for (const Piece& pc : pieces[ob.cs]) {
-            const int c0 = (int)contribs.size();
-            const int Da = b.sec[p.a].dim, Dc = b.sec[pc.a].dim;
-            for (const KTerm& t : kv.second) {
-              const double yv = t.y[p.s * d + pc.s];
-              if (yv == 0.0) continue;
-              KronContrib c{};
-              c.scale = t.coef * yv;
-              if (t.sign_row) c.scale *= fsign(b.sec[p.a].q);
-              if (t.sign_col) c.scale *= fsign(b.sec[pc.a].q);
-              if (t.x == nullptr) {
-                if (p.a != pc.a) continue;
-                c.x = nullptr;
-              } else if (!t.xtrans) {
-                const OpBlock* xb = t.x->row(p.a);
-                if (!xb || xb->cs != pc.a) continue;
-                c.x = t.x->data.p + xb->off;
-                c.ldx = Dc;
-                c.trans = 0;
-              } else {
-                const OpBlock* xb = t.x->row(pc.a);
-                if (!xb || xb->cs != p.a) continue;
-                c.x = t.x->data.p + xb->off;
-                c.ldx = Da;  // stored (pc.a x p.a) row-major: element (r,c) at [c*Da + r]
-                c.trans = 1;
-              }
-              contribs.push_back(c);
-            }
-            const int nc = (int)contribs.size() - c0;
-            if (nc == 0) continue;  // stays zero
-            double* dst = op->data.p + ob.off + (size_t)p.off * ldd + pc.off;
-            for (int r = 0; r < Da; r += KRON_TILE)
-              for (int cc = 0; cc < Dc; cc += KRON_TILE)
-                tasks.push_back({dst, ldd, Da, Dc, c0, nc, r, cc});
-          }
-      }
-      NSS_HIP(hipMemsetAsync(op->data.p, 0, op->data.n * sizeof(double), ctx->stream));
-      e.ops[kv.first] = op;
+      const QN dq = kv.first == "H" ? z : site.ops.at(kv.first).dq;
+      e.ops[kv.first] = kron_op(b, e.sec, pieces, kv.second, dq, tasks, contribs);
     }
-    DevArr<KronTask> dt;
-    DevArr<KronContrib> dc;
-    dt.resize(ctx->pool, std::max<size_t>(1, tasks.size()));
-    dc.resize(ctx->pool, std::max<size_t>(1, contribs.size()));
-    if (!tasks.empty()) {
-      NSS_HIP(hipMemcpyAsync(dt.p, tasks.data(), tasks.size() * sizeof(KronTask), hipMemcpyHostToDevice,
-                             ctx->stream));
-      NSS_HIP(hipMemcpyAsync(dc.p, contribs.data(), contribs.size() * sizeof(KronContrib),
-                             hipMemcpyHostToDevice, ctx->stream));
-      launch_kron(dt.p, (int)tasks.size(), dc.p, ctx->stream);
-    }
+    run_kron(tasks, contribs);
     for (auto& kv : e.ops) make_fp32(*kv.second);
     NSS_HIP(hipStreamSynchronize(ctx->stream));
     e.pieces = std::move(pieces);
@@ -682,6 +711,180 @@ struct Engine {
     return N;
   }
 
+  std::map<int, Block> bl1, bl2;
+
+  // ------------------------------------------------------------ measurement
+  // Measure_DMRG (src/DMRG_MEASURE.f90:148-706) on the ground state of the
+  // last step: a site operator O at chain position pos is built as
+  // Id (x) O in the enlarged basis of the block of length i (Build_Op_dmrg,
+  // :288-396), carried to the superblock basis by "rotate with the block's
+  // kept-state matrix, enlarge with (x) Id" (Advance_Op_dmrg, :398-496), and
+  // averaged as <psi| O (x) 1 |psi> or <psi| 1 (x) O |psi> over the
+  // superblock sectors (Average_Op_dmrg + OdotV_direct, :595-706).  O must
+  // conserve the quantum numbers (the reference keeps the QN-diagonal blocks
+  // of O only).  Everything stays on the device.
+
+  // O (site basis) in the basis of the one-site block b1.
+  OpPtr site_op_in_block(const std::vector<double>& O, const Block& b1) {
+    const int d = site.dim;
+    std::vector<int> ssec(d), spos(d), cnt(b1.sec.size(), 0);
+    for (int st = 0; st < d; ++st) {
+      ssec[st] = b1.find(site.qn[st]);
+      NSS_REQUIRE(ssec[st] >= 0, NSS_E_STATE, "measure: one-site block does not match the site");
+      spos[st] = cnt[ssec[st]]++;
+    }
+    OpPtr op = new_op({0, 0}, b1.sec, b1.sec);
+    std::vector<double> h(op->data.n, 0.0);
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        const double v = O[i * d + j];
+        if (v == 0.0) continue;
+        const OpBlock* ob = op->row(ssec[i]);
+        NSS_REQUIRE(ob && ob->cs == ssec[j], NSS_E_ARG, "measure: operator does not conserve the quantum numbers");
+        h[ob->off + (size_t)spos[i] * b1.sec[ssec[j]].dim + spos[j]] = v;
+      }
+    NSS_HIP(hipMemcpyAsync(op->data.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return op;
+  }
+
+  // U^T O U per sector of the block whose basis record is B (sectors nsec).
+  OpPtr rotate_op(const BlockOp& O, const Basis& B, const std::vector<Sector>& nsec) {
+    OpPtr R = new_op({0, 0}, nsec, nsec);
+    NSS_HIP(hipMemsetAsync(R->data.p, 0, R->data.n * sizeof(double), ctx->stream));
+    int64_t mt = 0;
+    std::vector<int64_t> moff(R->blks.size(), -1);
+    for (size_t i = 0; i < R->blks.size(); ++i) {
+      const int k = R->blks[i].rs;
+      moff[i] = mt;
+      mt += (int64_t)B.dim[k] * B.m[k];
+    }
+    DevArr<double> mid;
+    mid.resize(ctx->pool, std::max<int64_t>(1, mt));
+    GemmPlan p1, p2;
+    for (size_t i = 0; i < R->blks.size(); ++i) {
+      const OpBlock& nb = R->blks[i];
+      const int k = nb.rs, e = B.src[k], D = B.dim[k], m = B.m[k];
+      const OpBlock* ob = O.row(e);
+      if (!ob || ob->cs != e) continue;  // zero block
+      const double* Uk = B.U.p + B.uoff[k];
+      GemmSeg g1{};
+      g1.a = O.data.p + ob->off; g1.lda = D;
+      g1.b = Uk; g1.ldb = m;
+      g1.K = D; g1.alpha = 1.0; g1.abase = g1.bbase = BASE_ABS;
+      p1.add_output(BASE_ABS, mid.p + moff[i], 0, m, D, m, {g1});
+      GemmSeg g2{};
+      g2.a = Uk; g2.lda = m; g2.flags = SEG_TA;
+      g2.b = mid.p + moff[i]; g2.ldb = m;
+      g2.K = D; g2.alpha = 1.0; g2.abase = g2.bbase = BASE_ABS;
+      p2.add_output(BASE_ABS, R->data.p + nb.off, 0, m, m, m, {g2});
+    }
+    p1.upload(ctx->pool, ctx->stream);
+    p2.upload(ctx->pool, ctx->stream);
+    launch_gemm_f64(p1, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
+    launch_gemm_f64(p2, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return R;
+  }
+
+  // <psi| O (x) 1 |psi> (left) or <psi| 1 (x) O |psi> (right), O in the basis
+  // of the last superblock's enlarged block on that side.
+  double expect(const BlockOp& O, bool left_side) {
+    const int nq = (int)lastDl.size();
+    int64_t n = 0;
+    std::vector<int64_t> off(nq);
+    for (int q = 0; q < nq; ++q) {
+      off[q] = n;
+      n += (int64_t)lastDl[q] * lastDr[q];
+    }
+    DevArr<double> X, part;
+    X.resize(ctx->pool, std::max<int64_t>(1, n));
+    part.resize(ctx->pool, gemv_t_chunks(n) + 2);
+    NSS_HIP(hipMemsetAsync(X.p, 0, n * sizeof(double), ctx->stream));
+    GemmPlan p;
+    for (int q = 0; q < nq; ++q) {
+      const int Dl = lastDl[q], Dr = lastDr[q];
+      const OpBlock* ob = O.row(left_side ? sbL[q] : sbR[q]);
+      if (!ob || ob->cs != (left_side ? sbL[q] : sbR[q])) continue;
+      GemmSeg g{};
+      g.alpha = 1.0;
+      g.abase = g.bbase = BASE_ABS;
+      if (left_side) {  // X_q = O Psi_q
+        g.a = O.data.p + ob->off; g.lda = Dl;
+        g.b = psi.p + off[q]; g.ldb = Dr;
+        g.K = Dl;
+      } else {          // X_q = Psi_q O^T
+        g.a = psi.p + off[q]; g.lda = Dr;
+        g.b = O.data.p + ob->off; g.ldb = Dr; g.flags = SEG_TB;
+        g.K = Dr;
+      }
+      p.add_output(BASE_ABS, X.p + off[q], 0, Dr, Dl, Dr, {g});
+    }
+    p.upload(ctx->pool, ctx->stream);
+    launch_gemm_f64(p, Bases{nullptr, nullptr, nullptr}, ctx->stream, GEMM_GENERIC);
+    launch_gemv_t(psi.p, n, 1, X.p, n, part.p, part.p + gemv_t_chunks(n), ctx->stream);
+    double v = 0.0;
+    NSS_HIP(hipMemcpyAsync(&v, part.p + gemv_t_chunks(n), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return v;
+  }
+
+  double measure_one(const std::vector<double>& O, int pos) {
+    const int L = lastL.length, R = lastR.length;
+    NSS_REQUIRE(L > 0 && !lastDl.empty(), NSS_E_STATE, "measure: run DMRG first");
+    NSS_REQUIRE(pos >= 1 && pos <= L + R, NSS_E_ARG, "measure: position outside [1, L+R]");
+    const bool left_side = pos <= L;
+    const int i = left_side ? pos : L + R + 1 - pos;  // site index inside its block
+    const int Lb = left_side ? L : R;
+    // history of the last superblock's block on that side: bl[k] has length k
+    std::map<int, const Block*> bl;
+    for (const Block* b = (left_side ? lastL : lastR).parent.get(); b; b = b->parent.get()) bl[b->length] = b;
+    const QN z{0, 0};
+    std::vector<double> I(site.dim * site.dim, 0.0);
+    for (int k = 0; k < site.dim; ++k) I[k * site.dim + k] = 1.0;
+    OpPtr cur;
+    const Block* cb = nullptr;  // block whose enlargement the operator lives in (nullptr: one-site block)
+    int it0;
+    if (i == 1) {
+      NSS_REQUIRE(bl.count(1), NSS_E_STATE, "measure: block history missing");
+      cur = site_op_in_block(O, *bl.at(1));
+      it0 = 1;
+    } else {
+      NSS_REQUIRE(bl.count(i - 1), NSS_E_STATE, "measure: block history missing");
+      cb = bl.at(i - 1);
+      std::vector<Sector> esec;
+      std::vector<std::vector<Piece>> pieces;
+      enlarged_basis(*cb, esec, pieces);
+      std::vector<KronTask> tasks;
+      std::vector<KronContrib> contribs;
+      for (int a = 0; a < site.dim; ++a)
+        for (int b = 0; b < site.dim; ++b)
+          NSS_REQUIRE(O[a * site.dim + b] == 0.0 || site.qn[a] == site.qn[b], NSS_E_ARG,
+                      "measure: operator does not conserve the quantum numbers");
+      cur = kron_op(*cb, esec, pieces, {KTerm{nullptr, 0, 0, 0, O, z, 1.0}}, z, tasks, contribs);
+      run_kron(tasks, contribs);
+      it0 = i;
+    }
+    for (int it = it0; it <= Lb - 1; ++it) {
+      NSS_REQUIRE(bl.count(it), NSS_E_STATE, "measure: block history missing");
+      const Block& b = *bl.at(it);
+      if (it > 1) {  // rotate from the enlargement of bl[it-1] into bl[it]
+        NSS_REQUIRE(b.from, NSS_E_STATE, "measure: block without a basis record");
+        cur = rotate_op(*cur, *b.from, b.sec);
+      }
+      std::vector<Sector> esec;
+      std::vector<std::vector<Piece>> pieces;
+      enlarged_basis(b, esec, pieces);
+      std::vector<KronTask> tasks;
+      std::vector<KronContrib> contribs;
+      // the source operator must stay alive until k_kron has read it
+      OpPtr next = kron_op(b, esec, pieces, {KTerm{cur.get(), 0, 0, 0, I, z, 1.0}}, z, tasks, contribs);
+      run_kron(tasks, contribs);
+      cur = std::move(next);
+    }
+    return expect(*cur, left_side);
+  }
+
   // ------------------------------------------------------------ prediction
   // Ground state of the previous finite step and the basis maps around it.
   struct PrevStep {
@@ -833,8 +1036,11 @@ struct Engine {
     I.sweep = sweep + (type == 'f' ? 1 : 0);
     PhaseTimer pt(ctx, "step.enlarge");
     std::shared_ptr<Basis> fromL0 = left.from, fromR0 = right.from;
+    auto parL = std::make_shared<const Block>(left), parR = std::make_shared<const Block>(right);
     left = enlarge(left, true);
     right = enlarge(right, false);
+    left.parent = parL;
+    right.parent = parR;
     pt.next("step.setup");
     const auto t1 = clk::now();
     I.t_enlarge = sec(t0, t1);
@@ -898,6 +1104,8 @@ struct Engine {
       double tl = 0.0, tr = 0.0;
       Block nl = renormalize(left, sbL, sl, mstates, estates, ml, tl, I.cut_left, st.kept_left);
       Block nr = renormalize(right, sbR, sr, mstates, estates, mr, tr, I.cut_right, st.kept_right);
+      nl.parent = left.parent;
+      nr.parent = right.parent;
       left = std::move(nl);
       right = std::move(nr);
       I.m_left = ml;
@@ -942,12 +1150,21 @@ struct Engine {
   }
 
   void infinite() {
-    while (left.length < par.ldmrg) step('i', 0, 0);
+    bl1.clear();
+    bl2.clear();
+    bl1[left.length] = left;
+    bl2[right.length] = right;
+    while (left.length < par.ldmrg) {
+      step('i', 0, 0);
+      bl1[left.length] = left;
+      bl2[right.length] = right;
+    }
   }
 
   // finite_DMRG, src/DMRG_MAIN.f90:168-241, split into the iDMRG growth and
-  // one call per sweep.
-  std::map<int, Block> bl1, bl2;
+  // one call per sweep.  bl1 / bl2 hold the left / right blocks by length:
+  // after a run, bl?[k+1] is the renormalization of the enlarged bl?[k], the
+  // chain that Measure_DMRG walks.
   bool finite_ready = false;
 
   void finite_begin() {
@@ -1181,6 +1398,20 @@ extern "C" int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, in
   if (psi) {
     NSS_HIP(hipMemcpyAsync(psi, E->psi.p, nn * sizeof(double), hipMemcpyDeviceToHost, s));
     NSS_HIP(hipStreamSynchronize(s));
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int npos, const int32_t* pos,
+                                double* out) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && ops && pos && out && nops > 0 && npos > 0, NSS_E_ARG, "nss_dmrg_measure: bad arguments");
+  nss::Engine* E = d->e;
+  NSS_HIP(hipSetDevice(E->ctx->device));
+  const int dd = E->site.dim;
+  for (int o = 0; o < nops; ++o) {
+    std::vector<double> O(ops + (size_t)o * dd * dd, ops + (size_t)(o + 1) * dd * dd);
+    for (int p = 0; p < npos; ++p) out[(size_t)o * npos + p] = E->measure_one(O, pos[p]);
   }
   NSS_API_END
 }

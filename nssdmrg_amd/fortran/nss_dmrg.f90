@@ -143,6 +143,15 @@ module nss_dmrg
        type(c_ptr), value :: d
        integer(c_int) :: r
      end function c_nss_dmrg_destroy
+     function c_nss_dmrg_measure(d, nops, ops, npos, pos, out) bind(C, name="nss_dmrg_measure") result(r)
+       import :: c_ptr, c_int, c_int32_t, c_double
+       type(c_ptr), value :: d
+       integer(c_int), value :: nops, npos
+       real(c_double), intent(in) :: ops(*)
+       integer(c_int32_t), intent(in) :: pos(*)
+       real(c_double), intent(out) :: out(*)
+       integer(c_int) :: r
+     end function c_nss_dmrg_measure
      function c_strlen(s) bind(C, name="strlen") result(n)
        import :: c_ptr, c_size_t
        type(c_ptr), value :: s
@@ -156,7 +165,7 @@ module nss_dmrg
   real(c_double), allocatable, target, save :: esweep_(:)
 
   public :: nss_gpu_init, nss_gpu_finalize, nss_gpu_set_superblock, spMatVec_gpu_main, nss_gpu_lanczos
-  public :: init_dmrg_gpu, run_dmrg_gpu, finalize_dmrg_gpu, dmrg_gpu_nsteps, dmrg_gpu_step
+  public :: init_dmrg_gpu, run_dmrg_gpu, finalize_dmrg_gpu, dmrg_gpu_nsteps, dmrg_gpu_step, dmrg_gpu_measure
   public :: nss_check
 
 contains
@@ -304,6 +313,31 @@ contains
     type(nss_step_info) :: info
     call nss_check(c_nss_dmrg_step_info(dmrg_, int(i - 1, c_int), info), "dmrg_gpu_step")
   end function dmrg_gpu_step
+
+  !> Measure_DMRG(Op, pos, avOp) on the ground state of the last step:
+  !> ops(:,:,k) site operators (site basis, QN conserving), pos 1-based chain
+  !> positions; avOp(k, ipos) = <psi|O_k(pos(ipos))|psi>.
+  subroutine dmrg_gpu_measure(ops, pos, avOp)
+    real(8), intent(in) :: ops(:, :, :)
+    integer, intent(in) :: pos(:)
+    real(8), allocatable, intent(out) :: avOp(:, :)
+    real(c_double), allocatable :: opsT(:), out(:)
+    integer(c_int32_t), allocatable :: p(:)
+    integer :: k, d, nops, np
+    d = size(ops, 1)
+    nops = size(ops, 3)
+    np = size(pos)
+    allocate(opsT(d * d * nops), out(nops * np), p(np))
+    do k = 1, nops   ! row-major for the C side
+       opsT((k - 1) * d * d + 1:k * d * d) = reshape(transpose(ops(:, :, k)), [d * d])
+    enddo
+    p = int(pos, c_int32_t)
+    call nss_check(c_nss_dmrg_measure(dmrg_, int(nops, c_int), opsT, int(np, c_int), p, out), "dmrg_gpu_measure")
+    allocate(avOp(nops, np))
+    do k = 1, nops
+       avOp(k, :) = out((k - 1) * np + 1:k * np)
+    enddo
+  end subroutine dmrg_gpu_measure
 
   subroutine finalize_dmrg_gpu()
     if (c_associated(dmrg_)) call nss_check(c_nss_dmrg_destroy(dmrg_), "finalize_dmrg_gpu")

@@ -662,6 +662,29 @@ def chain_hamiltonian(dot: Site, L: int, hij) -> tuple[np.ndarray, np.ndarray]:
     return b.ops["H"], b.qn
 
 
+def ed_site_expectations(dot: Site, L: int, hij, target: tuple, ops) -> np.ndarray:
+    """<psi0| O_i |psi0>, i = 1..L, for each site operator in ops, on the exact
+    ground state of the full OBC chain in the target sector.  The chain basis
+    is built site by site in the left-block order, so site 1 is the leading
+    Kronecker factor.  This is the untruncated limit of Measure_DMRG
+    (src/DMRG_MEASURE.f90:148-706); the operators are diagonal in the
+    occupation basis, so no Jordan-Wigner string enters.  Returns
+    (len(ops), L)."""
+    h, qn = chain_hamiltonian(dot, L, hij)
+    sel = np.all(np.isclose(qn, np.array(target)[None, :]), axis=1)
+    e, v = np.linalg.eigh(h[np.ix_(sel, sel)])
+    psi = np.zeros(h.shape[0])
+    psi[sel] = v[:, 0]
+    d = dot.dim
+    T = psi.reshape([d] * L)
+    out = np.zeros((len(ops), L))
+    for k, O in enumerate(ops):
+        for i in range(L):
+            Tm = np.moveaxis(T, i, 0).reshape(d, -1)
+            out[k, i] = float(np.sum(Tm * (np.asarray(O) @ Tm)))
+    return out
+
+
 def ed_ground(dot: Site, L: int, hij, target: tuple, k: int = 1) -> np.ndarray:
     h, qn = chain_hamiltonian(dot, L, hij)
     sel = np.all(np.isclose(qn, np.array(target)[None, :]), axis=1)

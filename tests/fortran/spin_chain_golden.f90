@@ -22,12 +22,14 @@ program spin_chain_golden
   type(nss_model) :: model
   type(nss_params) :: par
   type(nss_step_info) :: info
-  real(8) :: Hij(2, 2), ref(3), err, maxerr
+  real(8) :: Hij(2, 2), ref(3), err, maxerr, szref, sz(2, 2, 1)
+  real(8), allocatable :: avo(:, :)
   integer :: i, u, ios, nbad
-  character(len=512) :: golden
+  character(len=512) :: golden, golden_sz
 
   ! ---------------------------------------------------------------- part 1
   call get_command_argument(1, golden)
+  call get_command_argument(2, golden_sz)
   Hij = 0d0
   Hij(1, 1) = 1d0      ! Jp
   Hij(2, 2) = 0.5d0    ! Jx/2
@@ -51,6 +53,25 @@ program spin_chain_golden
      maxerr = max(maxerr, err)
   enddo
   close(u)
+  ! Measure_DMRG(S_z, pos=1..40) against the reference's sz.check (1e-6)
+  if (len_trim(golden_sz) > 0) then
+     sz = 0d0
+     sz(1, 1, 1) = 0.5d0
+     sz(2, 2, 1) = -0.5d0
+     call dmrg_gpu_measure(sz, [(i, i = 1, 40)], avo)
+     open(newunit=u, file=trim(golden_sz), status="old", action="read")
+     maxerr = 0d0
+     do i = 1, 40
+        read(u, *) szref
+        maxerr = max(maxerr, abs(avo(1, i) - szref))
+     enddo
+     close(u)
+     write(*, "(A,ES10.3)") "golden S_z profile: max |d<Sz>| =", maxerr
+     if (maxerr > 1d-6) then
+        write(*, "(A)") "FAIL golden S_z"
+        stop 1
+     endif
+  endif
   call finalize_dmrg_gpu()
   write(*, "(A,I3,A,ES10.3)") "golden iDMRG steps:", dmrg_steps_checked(), "  max |dE/L| =", maxerr
   if (maxerr > 1d-10 .or. nbad > 0) then

@@ -165,6 +165,60 @@ def test_finite_prediction_matches_oracle(ctx, model, neigen):
         assert n_pred == n_plain
 
 
+SZ12 = np.diag([0.5, -0.5])
+NUP = np.diag([0.0, 1.0, 0.0, 1.0])
+NDW = np.diag([0.0, 0.0, 1.0, 1.0])
+
+
+def test_measure_golden_hubbard_L40_M40(ctx, golden_dir):
+    # hm_1d.f90: Measure_DMRG([dens, docc, s2z], pos=1..Ldmrg) after run_DMRG,
+    # asserted to 1e-6 against n.check, d.check, s2z.check
+    d, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=40, mdmrg=40,
+                target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=128, uloc=0.0, hfmode=True)
+    ops = [NUP + NDW, NUP @ NDW, (NUP - NDW) @ (NUP - NDW)]
+    avo = d.measure(ops, pos=np.arange(1, 41))
+    for k, name in enumerate(["n", "d", "s2z"]):
+        ref = np.loadtxt(os.path.join(golden_dir, f"hubbard_U0_L40_M40_{name}.check"))
+        assert np.max(np.abs(avo[k] - ref)) < 1e-6, name
+
+
+def test_measure_golden_spin_L40_M20(ctx, golden_dir):
+    d, st = run(ctx, "i", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=40, mdmrg=20,
+                target_qn=(0.0, 0.0), neigen=2, lanc_dim_threshold=128)
+    ref = np.loadtxt(os.path.join(golden_dir, "spin12_L40_M20_sz.check"))
+    assert np.max(np.abs(d.measure(SZ12, pos=np.arange(1, 41)) - ref)) < 1e-6
+
+
+@pytest.mark.parametrize("model,kind", [("spin12", "i"), ("spin12", "f"), ("hubbard", "i"), ("hubbard", "f")])
+def test_measure_matches_ed_untruncated(ctx, model, kind):
+    # no truncation: the measured site profile equals the exact one
+    print(f"case measure {model} {kind}", flush=True)
+    if model == "spin12":
+        L = 5
+        kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), target_qn=(0.1, 0.0))
+        osite, ops, target = O.spin_site(2), [SZ12], (1.0,)
+    else:
+        # half filling keeps the target int(L/2) consistent through the growth;
+        # double occupancy and (n_up-n_dw)^2 have nontrivial OBC profiles
+        L = 3 if kind == "i" else 2
+        kw = dict(site=N.electron_site(), hij=np.array(HUB_HIJ), target_qn=(0.5, 0.5), uloc=4.0, hfmode=True)
+        osite, ops, target = O.electron_site(4.0, True), [NUP @ NDW, (NUP - NDW) @ (NUP - NDW)], (3.0, 3.0)
+    d, st = run(ctx, kind, ldmrg=L, mdmrg=64, msweep=[64], neigen=1, lanc_dim_threshold=64, **kw)
+    nsites = st[-1]["L"]
+    got = d.measure(ops)
+    assert np.ptp(got) > 1e-3  # a nontrivial profile
+    ref = O.ed_site_expectations(osite, nsites, np.array(HUB_HIJ if model == "hubbard" else SPIN_HIJ), target, ops)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) < 1e-9, np.max(np.abs(got - ref))
+
+
+def test_measure_rejects_qn_breaking_operator(ctx):
+    d, st = run(ctx, "i", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=4, mdmrg=16, target_qn=(0.0, 0.0),
+                neigen=1, lanc_dim_threshold=64)
+    with pytest.raises(N.NssError):
+        d.measure(np.array([[0.0, 1.0], [0.0, 0.0]]), pos=[2])
+
+
 def test_mixed_precision_refined_matches_fp64(ctx):
     kw = dict(site=N.spin_site(3), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=48, msweep=[48],
               target_qn=(0.0, 0.0), neigen=1, lanc_dim_threshold=128)

@@ -118,6 +118,33 @@ def test_hubbard_untruncated_equals_ed(U, hf):
         assert abs(r.energies[0] - ed[0]) < 1e-10
 
 
+def test_ed_site_expectations_sum_rules_and_kron():
+    # spin-1/2, L=8, Sz_total = 1: sum_i <Sz_i> = 1, <Sz_i^2> = 1/4,
+    # reflection symmetry of the OBC ground state
+    sz = np.diag([0.5, -0.5])
+    v = O.ed_site_expectations(O.spin_site(2), 8, SPIN_HIJ, (1.0,), [sz, sz @ sz])
+    assert abs(v[0].sum() - 1.0) < 1e-12
+    assert np.allclose(v[1], 0.25, atol=1e-13)
+    assert np.allclose(v[0], v[0][::-1], atol=1e-12)
+    assert np.ptp(v[0]) > 1e-3  # a nontrivial profile
+    # Hubbard U=4, L=6, (N_up, N_dw) = (2, 2): particle-number sum rules,
+    # docc <= min(n_up, n_dw), and an explicit Kronecker check of one site
+    site = O.electron_site(4.0, True)
+    nup, ndw = np.diag([0.0, 1.0, 0.0, 1.0]), np.diag([0.0, 0.0, 1.0, 1.0])
+    w = O.ed_site_expectations(site, 6, HUB_HIJ, (2.0, 2.0), [nup, ndw, nup @ ndw])
+    assert abs(w[0].sum() - 2.0) < 1e-12 and abs(w[1].sum() - 2.0) < 1e-12
+    assert np.all(w[2] <= np.minimum(w[0], w[1]) + 1e-12)
+    assert np.allclose(w[0], w[0][::-1], atol=1e-12)
+    h, qn = O.chain_hamiltonian(site, 6, HUB_HIJ)
+    sel = np.all(np.isclose(qn, [2.0, 2.0]), axis=1)
+    e, u = np.linalg.eigh(h[np.ix_(sel, sel)])
+    psi = np.zeros(h.shape[0])
+    psi[sel] = u[:, 0]
+    i = 2
+    Oi = np.kron(np.kron(np.eye(4 ** i), nup @ ndw), np.eye(4 ** (6 - 1 - i)))
+    assert abs(psi @ Oi @ psi - w[2][i]) < 1e-12
+
+
 def test_hubbard_two_site_closed_form():
     # E = (U - sqrt(U^2+16 t^2))/2 for 2 sites, 1 up + 1 down; HF shift -U/2*N.
     U = 3.0
