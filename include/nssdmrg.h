@@ -241,6 +241,16 @@ int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, int32_t* dst,
  * (Build_Op_dmrg / Advance_Op_dmrg / Average_Op_dmrg).  Operators must
  * conserve the quantum numbers.  out[iop*npos + ipos]. */
 int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int npos, const int32_t* pos, double* out);
+/* Block restart files in the reference's text format (write_block /
+ * read_block, src/BLOCKS/BLOCKS.f90:418-537).  load: sets the left and right
+ * blocks an infinite run continues from (infinite_DMRG loading
+ * block_left/right.restart, src/DMRG_MAIN.f90:151-152); operator keys of
+ * both generations are accepted (S_z or S_z_n, C_l1_s1 or C_l1_s1_n, ...).
+ * save: writes the two blocks that entered the last step (SAVE_BLOCK at
+ * length Ldmrg-1, src/DMRG_MAIN.f90:308-317) with the current keys and an
+ * empty rotation-matrix list. */
+int nss_dmrg_load_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
+int nss_dmrg_save_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
 int nss_dmrg_destroy(nss_dmrg* d);
 
 #ifdef __cplusplus

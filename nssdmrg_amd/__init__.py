@@ -13,6 +13,7 @@ Every call runs the HIP library; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import re
 from dataclasses import dataclass, field
 
@@ -363,6 +364,15 @@ class DMRG:
         F.check(F.lib().nss_dmrg_measure(self._h, len(mats), F.dptr(a), len(p),
                                          p.ctypes.data_as(C.POINTER(C.c_int32)), F.dptr(out)))
         return out[0] if single else out
+
+    def load_blocks(self, left_path: str, right_path: str):
+        """read_block of block_left/right.restart (reference text format) as
+        the blocks the next infinite run continues from."""
+        F.check(F.lib().nss_dmrg_load_blocks(self._h, os.fsencode(left_path), os.fsencode(right_path)))
+
+    def save_blocks(self, left_path: str, right_path: str):
+        """write_block of the two blocks that entered the last step."""
+        F.check(F.lib().nss_dmrg_save_blocks(self._h, os.fsencode(left_path), os.fsencode(right_path)))
 
     def kept_qn(self, i: int, side: str = "left", min_weight: float = 0.0) -> dict:
         """QN -> number of kept states with rho eigenvalue > min_weight."""

@@ -18,6 +18,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
+#include <string>
 #include <memory>
 #include <numeric>
 
@@ -885,6 +887,157 @@ struct Engine {
     return expect(*cur, left_side);
   }
 
+  // ------------------------------------------------------------ restart I/O
+  // write_block / read_block (src/BLOCKS/BLOCKS.f90:418-537) in the
+  // reference's list-directed text format: length, Dim, SiteType, OpName,
+  // the sectors list (QN and 1-based state map per sector,
+  // src/LIST/LIST_SECTORS.f90:639-705), the operators list (key, type and
+  // sparse rows per operator, src/LIST/LIST_OPERATORS.f90:602-659,
+  // src/MATRIX/MATRIX_SPARSE.f90:703-766) and the rotation-matrix list.
+  // Keys of both generations are read (S_z / S_z_n, C_l1_s1 / C_l1_s1_n,
+  // P / P_n); the current ones are written.
+
+  struct Tokens {
+    std::vector<std::string> t;
+    size_t i = 0;
+    const std::string& next() {
+      NSS_REQUIRE(i < t.size(), NSS_E_ARG, "read_block: unexpected end of file");
+      return t[i++];
+    }
+    long inum() { return std::strtol(next().c_str(), nullptr, 10); }
+    double rnum() {
+      std::string v = next();
+      for (char& c : v)
+        if (c == 'D' || c == 'd') c = 'E';
+      return std::strtod(v.c_str(), nullptr);
+    }
+  };
+
+  Block load_block(const std::string& path) {
+    std::ifstream in(path);
+    NSS_REQUIRE(in.good(), NSS_E_ARG, ("read_block: cannot open " + path).c_str());
+    Tokens tk;
+    for (std::string w; in >> w;) tk.t.push_back(w);
+    Block b;
+    b.length = (int)tk.inum();
+    const long dim = tk.inum();
+    const std::string stype = tk.next();
+    (void)tk.next();  // OpName
+    NSS_REQUIRE((stype == "FERMION") == site.fermion, NSS_E_ARG, "read_block: site type does not match the model");
+    const long nlists = tk.inum();
+    NSS_REQUIRE(nlists == 1, NSS_E_ARG, "read_block: one sectors list expected");
+    const long nsec = tk.inum();
+    const long qdim = tk.inum();
+    std::vector<std::pair<int, int>> pos(dim, {-1, -1});  // old state -> (sector, index)
+    for (long k = 0; k < nsec; ++k) {
+      const long dq = tk.inum();
+      NSS_REQUIRE(dq == qdim && (qdim == 1 || qdim == 2), NSS_E_ARG, "read_block: bad QN size");
+      QN q;
+      q.a = (int)std::lround(2.0 * tk.rnum());
+      q.b = qdim == 2 ? (int)std::lround(2.0 * tk.rnum()) : 0;
+      const long nm = tk.inum();
+      b.sec.push_back({q, (int)nm});
+      for (long j = 0; j < nm; ++j) {
+        const long st = tk.inum() - 1;
+        NSS_REQUIRE(st >= 0 && st < dim && pos[st].first < 0, NSS_E_ARG, "read_block: bad sector map");
+        pos[st] = {(int)k, (int)j};
+      }
+    }
+    for (long st = 0; st < dim; ++st) NSS_REQUIRE(pos[st].first >= 0, NSS_E_ARG, "read_block: state in no sector");
+    static const std::map<std::string, std::string> keymap = {
+        {"H", "H"},         {"S_z", "Sz"},        {"S_z_n", "Sz"},      {"S_p", "Sp"},
+        {"S_p_n", "Sp"},    {"C_l1_s1", "Cup"},   {"C_l1_s1_n", "Cup"}, {"C_l1_s2", "Cdn"},
+        {"C_l1_s2_n", "Cdn"}};
+    const long nops = tk.inum();
+    for (long o = 0; o < nops; ++o) {
+      const std::string key = tk.next();
+      (void)tk.next();  // type
+      const long nr = tk.inum(), nc = tk.inum();
+      NSS_REQUIRE(nr == dim && nc == dim, NSS_E_ARG, "read_block: operator shape differs from Dim");
+      struct E { int r, c; double v; };
+      std::vector<E> ent;
+      for (long r = 0; r < nr; ++r) {
+        const long nz = tk.inum();
+        for (long z = 0; z < nz; ++z) {
+          const long c = tk.inum() - 1;
+          const double v = tk.rnum();
+          NSS_REQUIRE(c >= 0 && c < nc, NSS_E_ARG, "read_block: column out of range");
+          if (v != 0.0) ent.push_back({(int)r, (int)c, v});
+        }
+      }
+      auto it = keymap.find(key);
+      if (it == keymap.end()) continue;  // P (implicit here) and any other key
+      const QN dq = site.ops.at(it->second).dq;
+      OpPtr op = new_op(dq, b.sec, b.sec);
+      std::vector<double> h(op->data.n, 0.0);
+      for (const E& e : ent) {
+        const auto pr = pos[e.r], pc = pos[e.c];
+        const OpBlock* ob = op->row(pr.first);
+        NSS_REQUIRE(ob && ob->cs == pc.first, NSS_E_ARG,
+                    ("read_block: operator " + key + " does not have the QN shift of " + it->second).c_str());
+        h[ob->off + (size_t)pr.second * b.sec[pc.first].dim + pc.second] = e.v;
+      }
+      NSS_HIP(hipMemcpyAsync(op->data.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+      NSS_HIP(hipStreamSynchronize(ctx->stream));
+      make_fp32(*op);
+      b.ops[it->second] = op;
+    }
+    for (const char* need : site.fermion ? std::initializer_list<const char*>{"H", "Cup", "Cdn"}
+                                         : std::initializer_list<const char*>{"H", "Sz", "Sp"})
+      NSS_REQUIRE(b.ops.count(need), NSS_E_ARG, (std::string("read_block: operator missing: ") + need).c_str());
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    return b;  // rotation matrices are not needed to continue a run (measurement needs the history)
+  }
+
+  void save_block(const Block& b, const std::string& path) {
+    std::ofstream out(path);
+    NSS_REQUIRE(out.good(), NSS_E_ARG, ("write_block: cannot open " + path).c_str());
+    out.precision(17);
+    const int dim = b.dim();
+    const int qdim = site.fermion ? 2 : 1;
+    out << b.length << "\n" << dim << "\n" << (site.fermion ? "FERMION" : "SPIN") << "\n"
+        << (site.fermion ? "C" : "S") << "\n1\n" << b.sec.size() << "\n" << qdim << "\n";
+    std::vector<int> off(b.sec.size() + 1, 0);
+    for (size_t k = 0; k < b.sec.size(); ++k) off[k + 1] = off[k] + b.sec[k].dim;
+    for (size_t k = 0; k < b.sec.size(); ++k) {
+      out << qdim << "\n" << 0.5 * b.sec[k].q.a;
+      if (qdim == 2) out << " " << 0.5 * b.sec[k].q.b;
+      out << "\n" << b.sec[k].dim << "\n";
+      for (int j = 0; j < b.sec[k].dim; ++j) out << off[k] + j + 1 << (j + 1 < b.sec[k].dim ? " " : "\n");
+    }
+    struct W { std::string key, type, name; };
+    std::vector<W> ws = site.fermion
+        ? std::vector<W>{{"H", "bosonic", "H"}, {"C_l1_s1_n", "fermionic", "Cup"}, {"C_l1_s2_n", "fermionic", "Cdn"},
+                         {"P_n", "psign", ""}}
+        : std::vector<W>{{"H", "bosonic", "H"}, {"S_z_n", "bosonic", "Sz"}, {"S_p_n", "bosonic", "Sp"}};
+    out << ws.size() << "\n";
+    for (const W& w : ws) {
+      std::vector<double> dense((size_t)dim * dim, 0.0);
+      if (w.name.empty()) {  // fermionic parity, diagonal
+        for (size_t k = 0; k < b.sec.size(); ++k)
+          for (int j = off[k]; j < off[k + 1]; ++j) dense[(size_t)j * dim + j] = fsign(b.sec[k].q);
+      } else {
+        const BlockOp& op = *b.ops.at(w.name);
+        std::vector<double> h(op.data.n);
+        NSS_HIP(hipMemcpy(h.data(), op.data.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (const OpBlock& ob : op.blks)
+          for (int r = 0; r < b.sec[ob.rs].dim; ++r)
+            for (int c = 0; c < b.sec[ob.cs].dim; ++c)
+              dense[(size_t)(off[ob.rs] + r) * dim + off[ob.cs] + c] = h[ob.off + (size_t)r * b.sec[ob.cs].dim + c];
+      }
+      out << w.key << "\n" << w.type << "\n" << dim << " " << dim << "\n";
+      for (int r = 0; r < dim; ++r) {
+        int nz = 0;
+        for (int c = 0; c < dim; ++c) nz += dense[(size_t)r * dim + c] != 0.0;
+        out << nz << "\n";
+        for (int c = 0; c < dim; ++c)
+          if (dense[(size_t)r * dim + c] != 0.0) out << c + 1 << " " << dense[(size_t)r * dim + c] << "\n";
+      }
+    }
+    out << 0 << "\n";  // rotation matrices: not written (read_block then warns that no measurement is possible)
+    NSS_REQUIRE(out.good(), NSS_E_HIP, ("write_block: write failed for " + path).c_str());
+  }
+
   // ------------------------------------------------------------ prediction
   // Ground state of the previous finite step and the basis maps around it.
   struct PrevStep {
@@ -1399,6 +1552,29 @@ extern "C" int nss_dmrg_last_sb_export(nss_dmrg* d, int32_t* Dl, int32_t* Dr, in
     NSS_HIP(hipMemcpyAsync(psi, E->psi.p, nn * sizeof(double), hipMemcpyDeviceToHost, s));
     NSS_HIP(hipStreamSynchronize(s));
   }
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_load_blocks(nss_dmrg* d, const char* left_path, const char* right_path) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && left_path && right_path, NSS_E_ARG, "nss_dmrg_load_blocks: bad arguments");
+  nss::Engine* E = d->e;
+  NSS_HIP(hipSetDevice(E->ctx->device));
+  nss::Block l = E->load_block(left_path), r = E->load_block(right_path);
+  NSS_REQUIRE(l.length == r.length, NSS_E_ARG, "infinite_DMRG error: L.length != R.length");
+  E->left = std::move(l);
+  E->right = std::move(r);
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_save_blocks(nss_dmrg* d, const char* left_path, const char* right_path) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && left_path && right_path, NSS_E_ARG, "nss_dmrg_save_blocks: bad arguments");
+  nss::Engine* E = d->e;
+  NSS_REQUIRE(E->lastL.parent && E->lastR.parent, NSS_E_STATE, "nss_dmrg_save_blocks: run a step first");
+  NSS_HIP(hipSetDevice(E->ctx->device));
+  E->save_block(*E->lastL.parent, left_path);
+  E->save_block(*E->lastR.parent, right_path);
   NSS_API_END
 }
 

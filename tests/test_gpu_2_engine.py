@@ -219,6 +219,51 @@ def test_measure_rejects_qn_breaking_operator(ctx):
         d.measure(np.array([[0.0, 1.0], [0.0, 0.0]]), pos=[2])
 
 
+@pytest.mark.parametrize("model", ["spin12", "hubbard"])
+def test_restart_fixture_last_step(ctx, golden_dir, model):
+    # infinite_DMRG with block_left/right.restart present: the reference's
+    # length-39 blocks, one step to L=40, against the final energy.check
+    if model == "spin12":
+        kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=40, mdmrg=20, target_qn=(0.0, 0.0), neigen=2,
+                  lanc_dim_threshold=128)
+        tag = "spin12_L40_M20"  # row: length, E0/L, E1/L, length
+        ref = np.loadtxt(os.path.join(golden_dir, "spin12_L40_M20_final_energy.check"))[1:3]
+    else:
+        kw = dict(site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=40, mdmrg=40, target_qn=(0.5, 0.5),
+                  neigen=1, lanc_dim_threshold=128, uloc=0.0, hfmode=True)
+        tag = "hubbard_U0_L40_M40"
+        ref = np.atleast_1d(np.loadtxt(os.path.join(golden_dir, "hubbard_U0_L40_M40_final_energy.check")))
+    d = N.DMRG(ctx, N.DMRGConfig(**kw))
+    d.load_blocks(os.path.join(golden_dir, f"{tag}_block_left.restart"),
+                  os.path.join(golden_dir, f"{tag}_block_right.restart"))
+    st = d.run("i")
+    assert len(st) == 1 and st[0]["left_len"] == 40
+    got = st[0]["e_per_site"][:len(ref)]
+    assert np.max(np.abs(got - ref)) < 1e-10, (got, ref)
+
+
+@pytest.mark.parametrize("model", ["spin12", "hubbard"])
+def test_restart_round_trip(ctx, tmp_path, model):
+    # write_block of the blocks entering the last step, read_block into a new
+    # run: the same last step again
+    if model == "spin12":
+        kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=16, target_qn=(0.0, 0.0), neigen=1,
+                  lanc_dim_threshold=64)
+    else:
+        kw = dict(site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=6, mdmrg=24, target_qn=(0.5, 0.5), neigen=1,
+                  lanc_dim_threshold=64, uloc=3.0, hfmode=True)
+    d1, st1 = run(ctx, "i", **kw)
+    lp, rp = str(tmp_path / "block_left.restart"), str(tmp_path / "block_right.restart")
+    d1.save_blocks(lp, rp)
+    d2 = N.DMRG(ctx, N.DMRGConfig(**kw))
+    d2.load_blocks(lp, rp)
+    st2 = d2.run("i")
+    assert len(st2) == 1
+    assert abs(st2[0]["energies"][0] - st1[-1]["energies"][0]) < 1e-11
+    assert abs(st2[0]["entropy_left"] - st1[-1]["entropy_left"]) < 1e-10
+    assert st2[0]["m_sb"] == st1[-1]["m_sb"]
+
+
 def test_mixed_precision_refined_matches_fp64(ctx):
     kw = dict(site=N.spin_site(3), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=48, msweep=[48],
               target_qn=(0.0, 0.0), neigen=1, lanc_dim_threshold=128)
