@@ -602,22 +602,27 @@ __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, 
   if (threadIdx.x == 0) partial[(size_t)j * nch + c] = r;
 }
 
-// h[j] = sum_c partial[j*nch + c]   (one block per j)
-__global__ __launch_bounds__(256) void k_lz_hreduce(const double* __restrict__ partial, int64_t nch,
-                                                    double* __restrict__ h) {
-  __shared__ double sh[4];
-  const int j = blockIdx.x;
-  const double r = coop_sum(partial + (size_t)j * nch, nch, sh);
-  if (threadIdx.x == 0) h[j] = r;
-}
-
-// w -= V[0..m) h; sq[block] = sum of squares of the block's new rows
+// h[j] = sum_c partial[j*nch + c] (every block, one thread per column, a
+// fixed order; block 0 publishes h); w -= V[0..m) h; sq[block] = sum of
+// squares of the block's new rows
 __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ V, int64_t ldv, int m,
-                                                     const double* __restrict__ h, double* __restrict__ w,
+                                                     const double* __restrict__ partial, int64_t nch,
+                                                     double* __restrict__ h, double* __restrict__ w,
                                                      int64_t n, double* __restrict__ sq) {
   extern __shared__ double hs[];
   __shared__ double red[4];
-  for (int j = threadIdx.x; j < m; j += 256) hs[j] = h[j];
+  for (int j = threadIdx.x; j < m; j += 256) {
+    const double* p = partial + (size_t)j * nch;
+    double a0 = 0.0, a1 = 0.0;
+    int64_t c = 0;
+    for (; c + 2 <= nch; c += 2) {
+      a0 += p[c];
+      a1 += p[c + 1];
+    }
+    if (c < nch) a0 += p[c];
+    hs[j] = a0 + a1;
+    if (blockIdx.x == 0) h[j] = a0 + a1;
+  }
   __syncthreads();
   double ss = 0.0;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
@@ -669,9 +674,8 @@ void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n,
                       double* hout, double* sq, hipStream_t s) {
   const int64_t nch = lz_nch(n);
   hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch);
-  hipLaunchKernelGGL(k_lz_hreduce, dim3((unsigned)m), dim3(256), 0, s, partial, nch, hout);
   hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
-                     ldv, m, hout, w, n, sq);
+                     ldv, m, partial, nch, hout, w, n, sq);
   NSS_HIP(hipGetLastError());
 }
 
