@@ -116,7 +116,7 @@ struct Bases {
   void* y;
 };
 
-// Tile edge for new plans: 32 unless the environment sets NSS_GEMM_TILE=64.
+// Forced tile edge (NSS_GEMM_TILE = 32 or 64), 0 for the size-based choice.
 int gemm_default_tile();
 
 // One grouped launch: tiles over outputs, segments over K-contributions.
@@ -126,11 +126,14 @@ struct GemmPlan {
   DevArr<GemmTile> tiles;
   DevArr<GemmSeg> segs;
   double flops = 0.0;
-  // Output tile edge: 32 -> k_*  (32x32 tile, K chain split over the four
-  // waves of a workgroup, reduced through LDS); 64 -> k_*_t64 (64x64 tile,
-  // four waves share LDS-staged operands).  Set before add_output.
-  int tile = gemm_default_tile();
-  void clear() { htiles.clear(); hsegs.clear(); flops = 0.0; }
+  // Output tile edge, chosen in upload(): 32 -> k_* (32x32 tile, K chain
+  // split over the four waves of a workgroup, reduced through LDS; for
+  // launches too small to fill the chip); 64 -> k_*_t64 (64x64 tile, four
+  // waves share LDS-staged operands; for large launches).  NSS_GEMM_TILE
+  // forces either.
+  int tile = 32;
+  std::vector<GemmTile> outs;  // one per add_output (row0 = col0 = 0)
+  void clear() { htiles.clear(); hsegs.clear(); outs.clear(); flops = 0.0; }
   // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
   // the same list of segments.
   void add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,

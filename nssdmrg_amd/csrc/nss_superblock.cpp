@@ -83,7 +83,8 @@ DevicePool::~DevicePool() {
 int gemm_default_tile() {
   static const int t = [] {
     const char* e = std::getenv("NSS_GEMM_TILE");
-    return (e && std::atoi(e) == 64) ? 64 : 32;
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 32 || v == 64) ? v : 0;
   }();
   return t;
 }
@@ -99,25 +100,38 @@ void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int 
     ++nseg;
     flops += 2.0 * M * (double)N * g.K;
   }
-  for (int r = 0; r < M; r += tile)
-    for (int cc = 0; cc < N; cc += tile) {
-      GemmTile t{};
-      t.c = c;
-      t.coff = coff;
-      t.ldc = ldc;
-      t.M = M;
-      t.N = N;
-      t.row0 = r;
-      t.col0 = cc;
-      t.seg0 = seg0;
-      t.nseg = nseg;
-      t.cbase = cbase;
-      t.beta = beta;
-      htiles.push_back(t);
-    }
+  GemmTile t{};
+  t.c = c;
+  t.coff = coff;
+  t.ldc = ldc;
+  t.M = M;
+  t.N = N;
+  t.seg0 = seg0;
+  t.nseg = nseg;
+  t.cbase = cbase;
+  t.beta = beta;
+  outs.push_back(t);
 }
 
+// Launches with at least this many 32x32 tiles (two 64x64 tiles per CU) use
+// the 64x64 LDS-staged kernel: on large launches its operand sharing across
+// the four waves wins (spin-1 m=1024: 29.7 vs 21.9 TFLOP/s FP64); on smaller
+// ones the split K chain of the 32x32 kernel does.
+static constexpr int64_t GEMM_T64_MIN_TILES = 2048;
+
 void GemmPlan::upload(DevicePool* pool, hipStream_t s) {
+  int64_t n32 = 0;
+  for (const GemmTile& o : outs) n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
+  tile = gemm_default_tile() ? gemm_default_tile() : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
+  htiles.clear();
+  for (const GemmTile& o : outs)
+    for (int r = 0; r < o.M; r += tile)
+      for (int cc = 0; cc < o.N; cc += tile) {
+        GemmTile t = o;
+        t.row0 = r;
+        t.col0 = cc;
+        htiles.push_back(t);
+      }
   tiles.resize(pool, std::max<size_t>(1, htiles.size()));
   segs.resize(pool, std::max<size_t>(1, hsegs.size()));
   if (!htiles.empty())
