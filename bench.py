@@ -169,7 +169,10 @@ def main():
     calls = max(1, st["calls"])
     y_avg_s = st["t_y"] / calls
     y_flops = st["flops_y"] / calls
-    roofline = {"bound": "mfma", "kernel": "k_hv_y_f64", "achieved": y_flops / y_avg_s / 1e12,
+    roofline = {"bound": "mfma", "kernel": "k_hv_y_f64",
+                "kernel_variants": "k_hv_y_f64 (32x32 split-K) or k_hv_y_f64_t64 (64x64 LDS-staged), "
+                                   "chosen per superblock by its tile count; the figures cover both",
+                "achieved": y_flops / y_avg_s / 1e12,
                 "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
                 "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA,
                 "traffic": profiled_traffic(a.workload, "k_hv_y_f64"),
