@@ -20,11 +20,6 @@
 
 namespace nss {
 
-// rho blocks up to this size are diagonalised on the host (Householder +
-// implicit QL); larger ones on the device (batched Jacobi, or rocSOLVER
-// dsyevd beyond the Jacobi kernel's LDS limit).
-constexpr int HOST_EIG_MAX = 12;
-
 static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
@@ -462,7 +457,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   struct Item {
     int side, q, d;
     int64_t roff, eoff;
-    int kind;  // 0 host, 1 jacobi, 2 rocsolver
+    int kind;  // 1 jacobi, 2 rocsolver
   };
   std::vector<Item> items;
   int64_t tot = 0;
@@ -473,7 +468,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     for (int q = 0; q < nq; ++q) {
       if ((int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
       const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-      items.push_back({side, q, d, tot, -1, 0});
+      items.push_back({side, q, d, tot, -1, 1});
       tot += (int64_t)d * d;
     }
   }
@@ -527,17 +522,17 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   for (int c = 0; c < nchk; ++c)
     if (!(asym[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
-  // eigen-decompositions: tiny blocks on the host, blocks up to the LDS
-  // limit with the batched GPU Jacobi kernel, larger ones with rocSOLVER
-  // dsyevd; device results overwrite rho in place
+  // eigen-decompositions: blocks up to the LDS limit (the 1x1 and tiny
+  // ones included) with the batched GPU Jacobi kernel, larger ones with
+  // rocSOLVER dsyevd; device results overwrite rho in place
   pt.next("rdm.jacobi");
   const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
+  if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff;
   std::vector<int32_t> jdim;
   int jn = 0;
   for (Item& it : items) {
-    if (it.d <= HOST_EIG_MAX) continue;
     it.eoff = etot;
     etot += it.d;
     if (ctx->prof) {
@@ -636,18 +631,12 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   pt.next("rdm.download");
   std::vector<double> hD(std::max<int64_t>(1, etot));
   if (etot) NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  // eigenvectors to the host: all of rho (host_vectors) or only the tiny
-  // host-solved blocks (device views)
+  // eigenvectors to the host only when asked for (host_vectors); otherwise
+  // they stay in rho and are handed out as device views
   std::vector<double> h;
   if (!dvecs) {
     h.resize(std::max<int64_t>(1, tot));
     if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  } else {
-    h.resize(std::max<int64_t>(1, tot));  // only the kind-0 ranges are filled
-    for (const Item& it : items)
-      if (it.kind == 0)
-        NSS_HIP(hipMemcpyAsync(h.data() + it.roff, sb->rho.p + it.roff, (size_t)it.d * it.d * sizeof(double),
-                               hipMemcpyDeviceToHost, ctx->stream));
   }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   pt.next("rdm.host");
@@ -655,11 +644,10 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     dvecs[0].assign(nq, {});
     dvecs[1].assign(nq, {});
   }
-  bool uploaded = false;
   for (const Item& it : items) {
     const int d = it.d;
     std::vector<double>& ev = evals[it.side][it.q];
-    const double* a = h.data() + it.roff;
+    const double* a = dvecs ? nullptr : h.data() + it.roff;
     ev.resize(d);
     RhoVec* rv = dvecs ? &dvecs[it.side][it.q] : nullptr;
     std::vector<double>* vec = dvecs ? nullptr : &evecs[it.side][it.q];
@@ -680,7 +668,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
         rv->si = 1;
         rv->sj = d;
       }
-    } else if (it.kind == 1) {
+    } else {
       // Jacobi: row-major, eigenvectors as columns, unsorted -> ascending order
       std::vector<int> idx(d);
       for (int i = 0; i < d; ++i) idx[i] = i;
@@ -696,23 +684,10 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
         rv->sj = 1;
         rv->order = idx;
       }
-    } else {
-      std::vector<double> v(a, a + (size_t)d * d);
-      sym_eig(d, v.data(), ev.data());
-      if (vec) *vec = v;
-      if (rv) {  // back into rho, row-major with eigenvectors as columns
-        std::copy(v.begin(), v.end(), h.begin() + it.roff);
-        NSS_HIP(hipMemcpyAsync(sb->rho.p + it.roff, h.data() + it.roff, (size_t)d * d * sizeof(double),
-                               hipMemcpyHostToDevice, ctx->stream));
-        uploaded = true;
-        rv->si = d;
-        rv->sj = 1;
-      }
     }
     for (double& e : ev)
       if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
   }
-  if (uploaded) NSS_HIP(hipStreamSynchronize(ctx->stream));  // h must outlive the copies
 }
 
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
