@@ -260,7 +260,7 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
   return t < ntiles ? t : -1;
 }
 
-template <typename T>
+template <typename T, int NBUF>
 __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
                                             int ntiles, const Bases& bs) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -319,26 +319,40 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
     alpha = v.alpha;
     return true;
   };
-  // Two fragment buffers in ping-pong, each refilled right after its chunk
-  // has gone to the MFMA pipe, so loads run one chunk ahead.  Every refill
-  // issues its loads (an exhausted buffer re-reads valid addresses with
-  // alpha = 0), which keeps the loop straight-line with a single exit: the
-  // accumulators stay in place and each buffer waits only on its own loads.
-  int m0[2] = {0, 1}, m1[2] = {0, 1};
-  T al0 = T(0), al1 = T(0);
-  if (advance(m0, al0)) {
-    Frag32<T> f0, f1;
-    load_frag32<T>(v, m0[0], fk, f0);
-    advance(m1, al1);
-    load_frag32<T>(v, m1[0], fk, f1);
+  // NBUF fragment buffers in rotation, each refilled right after its chunk
+  // has gone to the MFMA pipe, so loads run NBUF-1 chunks ahead (two
+  // buffers measured fastest: the chains of small sectors are only a few
+  // chunks long per wave, and every buffer costs a full round of loads).  Every
+  // refill issues its loads (an exhausted buffer re-reads valid addresses
+  // with alpha = 0), which keeps the loop straight-line with a single exit:
+  // the accumulators stay in place and each buffer waits only on its own
+  // loads.  Buffers take the wave's chunks in order, so once buffer 0 finds
+  // the chain exhausted every later buffer does too.
+  int m[NBUF][2];
+  T al[NBUF];
+#pragma unroll
+  for (int b = 0; b < NBUF; ++b) {
+    m[b][0] = 0;
+    m[b][1] = 1;
+    al[b] = T(0);
+  }
+  if (advance(m[0], al[0])) {
+    Frag32<T> f[NBUF];
+    load_frag32<T>(v, m[0][0], fk, f[0]);
+#pragma unroll
+    for (int b = 1; b < NBUF; ++b) {
+      advance(m[b], al[b]);
+      load_frag32<T>(v, m[b][0], fk, f[b]);
+    }
     bool h0;
     do {
-      mma_chunk32<T>(f0, m0[0], m0[1], al0, fk, acc);
-      h0 = advance(m0, al0);
-      load_frag32<T>(v, m0[0], fk, f0);
-      mma_chunk32<T>(f1, m1[0], m1[1], al1, fk, acc);
-      advance(m1, al1);
-      load_frag32<T>(v, m1[0], fk, f1);
+#pragma unroll
+      for (int b = 0; b < NBUF; ++b) {
+        mma_chunk32<T>(f[b], m[b][0], m[b][1], al[b], fk, acc);
+        const bool hb = advance(m[b], al[b]);
+        if (b == 0) h0 = hb;
+        load_frag32<T>(v, m[b][0], fk, f[b]);
+      }
     } while (h0);
   }
 
@@ -375,7 +389,7 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
 #define NSS_GEMM_KERNEL(name, T)                                                                        \
   __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,                       \
                                               const GemmSeg* __restrict__ segs, int ntiles, Bases bs) { \
-    gemm_tile32<T>(tiles, segs, ntiles, bs);                                                            \
+    gemm_tile32<T, 2>(tiles, segs, ntiles, bs);                                                         \
   }                                                                                                     \
   __global__ __launch_bounds__(256) void name##_t64(const GemmTile* __restrict__ tiles,                 \
                                                     const GemmSeg* __restrict__ segs, int ntiles,       \
@@ -385,6 +399,7 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   }
 NSS_GEMM_KERNEL(k_hv_w_f64, double)
 NSS_GEMM_KERNEL(k_hv_y_f64, double)
+
 NSS_GEMM_KERNEL(k_gemm_f64, double)
 NSS_GEMM_KERNEL(k_hv_w_f32, float)
 NSS_GEMM_KERNEL(k_hv_y_f32, float)

@@ -597,6 +597,10 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
       ++c;
       B.ncls = kv.first;
       B.nb = (int)kv.second.size();
+      if (ctx->prof) {
+        ctx->timers["rdm.class_" + std::to_string(B.ncls) + ".calls"] += 1;
+        ctx->timers["rdm.class_" + std::to_string(B.ncls) + ".blocks"] += B.nb;
+      }
       for (int i : kv.second) {
         B.co.push_back(items[i].roff);
         B.ceo.push_back(items[i].eoff);

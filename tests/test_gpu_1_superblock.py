@@ -216,3 +216,24 @@ def test_rdm_eigh_large_blocks(ctx):
             assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-14), (side, k)
             assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14
             assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12
+
+
+def test_rdm_eigh_rank_deficient_blocks(ctx):
+    # rank-deficient rho blocks on both eigensolver paths (Jacobi up to 64,
+    # rocSOLVER above): large null spaces, rank-1 blocks, a 2x2 and a full block
+    dims = [(135, 5), (5, 135), (2, 2), (64, 64), (135, 1), (1, 136)]
+    sb = _random_sb(7, dims, 6)
+    upload(ctx, sb)
+    psi = O.splitmix64_uniform_fast(sb.n, 5)
+    psi /= np.linalg.norm(psi)
+    for side in ("left", "right"):
+        evs, vecs = ctx.rdm_eigh(psi, side)
+        for k, (ev, vec) in enumerate(zip(evs, vecs)):
+            dl, dr, off = sb.Dl[k], sb.Dr[k], sb.offset[k]
+            p = psi[off:off + dl * dr].reshape(dl, dr)
+            rho = p @ p.T if side == "left" else p.T @ p
+            ref = np.linalg.eigvalsh(rho)
+            assert np.all(np.diff(ev) >= 0), (side, k)
+            assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-15), (side, k)
+            assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14, (side, k)
+            assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-13, (side, k)
