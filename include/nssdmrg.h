@@ -218,6 +218,19 @@ int nss_dmrg_run(nss_dmrg* d, int type);
 int nss_dmrg_finite_begin(nss_dmrg* d);
 int nss_dmrg_sweep(nss_dmrg* d, int im);
 int nss_dmrg_nsteps(nss_dmrg* d);
+/* Independent chains solved together (no counterpart in the reference,
+ * whose runs are one chain per process): the n engines share one context
+ * and LDMRG (and NSWEEP for 'f'), and step in lockstep; each step's ground
+ * states come from one batched Lanczos whose H*v over all chains is one
+ * grouped W launch and one grouped Y launch.  Per engine the results are
+ * those of nss_dmrg_run on it alone (same tolerances; the Lanczos iterates
+ * differ).  Steps on the dense path, with LANC_NEIGEN > 1 or in mixed
+ * precision keep the engine's own solver inside the lockstep.  The step
+ * records (nss_dmrg_step_info) stay per engine; the batch's H*v time is
+ * booked on the first engine's steps. */
+int nss_dmrg_batch_run(int n, nss_dmrg* const* engines, int type);
+int nss_dmrg_batch_finite_begin(int n, nss_dmrg* const* engines);
+int nss_dmrg_batch_sweep(int n, nss_dmrg* const* engines, int im);
 int nss_dmrg_step_info(nss_dmrg* d, int i, nss_step_info* out);
 /* Kept-state counts per QN of step i (side 0 left, 1 right), counting only
  * kept states whose rho eigenvalue exceeds min_weight (0: all kept states;

@@ -22,7 +22,7 @@ import numpy as np
 from . import _ffi as F
 from ._ffi import NssError
 
-__all__ = ["Context", "DMRG", "NssError", "kept_count", "host_sym_eig", "host_tridiag_eig",
+__all__ = ["Context", "DMRG", "DMRGBatch", "NssError", "kept_count", "host_sym_eig", "host_tridiag_eig",
            "read_input", "init_dmrg", "run_DMRG", "finalize_dmrg", "spin_site", "electron_site",
            "device_ok"]
 
@@ -241,6 +241,30 @@ def _parse_value(s: str):
                     vals.append(p)
     return vals[0] if len(vals) == 1 else vals
 
+
+
+class DMRGBatch:
+    """Independent chains solved together (nss_dmrg_batch_*): the engines step
+    in lockstep and each step's ground states come from one batched Lanczos
+    whose H*v over all chains is one grouped launch per phase.  The engines
+    must share the context and LDMRG (and NSWEEP for finite runs); their step
+    records stay per engine (``DMRG.steps``)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        if not self.engines:
+            raise ValueError("DMRGBatch needs at least one engine")
+        self._arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+
+    def run(self, kind: str = "i"):
+        F.check(F.lib().nss_dmrg_batch_run(len(self.engines), self._arr, ord(kind)))
+        return [e.steps() for e in self.engines]
+
+    def finite_begin(self):
+        F.check(F.lib().nss_dmrg_batch_finite_begin(len(self.engines), self._arr))
+
+    def sweep(self, im: int):
+        F.check(F.lib().nss_dmrg_batch_sweep(len(self.engines), self._arr, im))
 
 def read_input(path: str) -> dict:
     """Reads a DMRG.conf (KEY=value !comment) as parse_input_variable does."""

@@ -92,6 +92,9 @@ SIGNATURES = {
     "nss_dmrg_run": (C.c_int, [_P, C.c_int]),
     "nss_dmrg_finite_begin": (C.c_int, [_P]),
     "nss_dmrg_sweep": (C.c_int, [_P, C.c_int]),
+    "nss_dmrg_batch_run": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_int]),
+    "nss_dmrg_batch_finite_begin": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "nss_dmrg_batch_sweep": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_int]),
     "nss_dmrg_nsteps": (C.c_int, [_P]),
     "nss_dmrg_step_info": (C.c_int, [_P, C.c_int, C.POINTER(StepInfo)]),
     "nss_dmrg_kept_qn": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int, _D, _I32, _I32]),

@@ -194,6 +194,14 @@ struct Engine {
   std::vector<SbTerm> lastTerms;
   DevArr<double> psi;
   bool want32 = false;
+  // superblock plan: the context's (standalone engines, so that the
+  // low-level entry points see the engine's last superblock) or an own one
+  // (members of a batch, whose superblocks coexist)
+  SbDevice* own_sb = nullptr;
+  SbDevice* sbd() const { return own_sb ? own_sb : ctx->sb; }
+  ~Engine() {
+    if (own_sb) sb_destroy(own_sb);
+  }
 
   // ------------------------------------------------------------ blocks
   OpPtr new_op(QN dq, const std::vector<Sector>& rsec, const std::vector<Sector>& csec) {
@@ -460,7 +468,7 @@ struct Engine {
     }
     NSS_REQUIRE(!sbL.empty(), NSS_E_STATE, "sb_get_states ERROR: total_states=0. There are no SB states.");
     const int nq = (int)sbL.size();
-    sb_set_layout(ctx->sb, Dl, Dr, nullptr);
+    sb_set_layout(sbd(), Dl, Dr, nullptr);
     lastDl = Dl;
     lastDr = Dr;
     std::map<QN, int> index;
@@ -534,7 +542,7 @@ struct Engine {
         }
       }
     }
-    sb_set_terms(ctx, ctx->sb, HL, HR, terms, want32);
+    sb_set_terms(ctx, sbd(), HL, HR, terms, want32);
     lastTerms = terms;
   }
 
@@ -1172,23 +1180,46 @@ struct Engine {
     return t;
   }
 
-  void step(char type, int label, int sweep) {
-    using clk = std::chrono::steady_clock;
-    auto sec = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
-    const auto t0 = clk::now();
-    int mstates = par.mdmrg;
-    double estates = par.edmrg;
-    if (type == 'f') {
-      mstates = msweep.at(sweep);
-      estates = esweep.empty() ? 0.0 : esweep.at(sweep);
-    }
+  // One DMRG step (step_dmrg, src/DMRG_MAIN.f90:265-460) in three phases:
+  // begin_step (enlargement, superblock, start vector), the diagonalisation
+  // (diag, or a batched Lanczos over several engines), end_step (rho,
+  // truncation, renormalization, the step record).
+  struct StepState {
+    char type = 'i';
+    int label = 0, sweep = 0, mstates = 0, neig = 1, v0 = 0;
+    double estates = 0.0;
+    bool dense = false;
+    int64_t msb = 0;
     Step st;
-    nss_step_info& I = st.info;
+    std::shared_ptr<Basis> fromL0, fromR0;
+    std::chrono::steady_clock::time_point t0, t2, t3;
+    LanczosOut lo;
+  };
+
+  static double secs(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+  }
+
+  StepState begin_step(char type, int label, int sweep) {
+    using clk = std::chrono::steady_clock;
+    StepState S;
+    S.t0 = clk::now();
+    S.type = type;
+    S.label = label;
+    S.sweep = sweep;
+    S.mstates = par.mdmrg;
+    S.estates = par.edmrg;
+    if (type == 'f') {
+      S.mstates = msweep.at(sweep);
+      S.estates = esweep.empty() ? 0.0 : esweep.at(sweep);
+    }
+    nss_step_info& I = S.st.info;
     I.type = type;
     I.label = label;
     I.sweep = sweep + (type == 'f' ? 1 : 0);
     PhaseTimer pt(ctx, "step.enlarge");
-    std::shared_ptr<Basis> fromL0 = left.from, fromR0 = right.from;
+    S.fromL0 = left.from;
+    S.fromR0 = right.from;
     auto parL = std::make_shared<const Block>(left), parR = std::make_shared<const Block>(right);
     left = enlarge(left, true);
     right = enlarge(right, false);
@@ -1196,51 +1227,68 @@ struct Engine {
     right.parent = parR;
     pt.next("step.setup");
     const auto t1 = clk::now();
-    I.t_enlarge = sec(t0, t1);
+    I.t_enlarge = secs(S.t0, t1);
     I.left_len = left.length;
     I.right_len = right.length;
     I.L = left.length + right.length;
     setup_superblock(left, right, target_of(I.L));
-    const int64_t msb = sb_n(ctx->sb);
-    I.m_sb = msb;
-    I.nsect = sb_nq(ctx->sb);
+    S.msb = sb_n(sbd());
+    I.m_sb = S.msb;
+    I.nsect = sb_nq(sbd());
     I.nterms = (int)lastTerms.size();
-    I.hv_flops = sb_flops(ctx->sb);
-    const int neig = (int)std::min<int64_t>(msb, par.neigen);
-    I.neigen = neig;
-    psi.resize(ctx->pool, (size_t)neig * msb);
-    LanczosOut lo;
-    pt.next("step.diag");
-    const auto t2 = clk::now();
-    if (neig == msb || msb <= par.lanc_dim_threshold) {
-      lo = dense_diag(ctx, ctx->sb, neig, psi.p);
-      I.numop = 0;
-    } else {
-      int v0 = par.v0_mode;
+    I.hv_flops = sb_flops(sbd());
+    S.neig = (int)std::min<int64_t>(S.msb, par.neigen);
+    I.neigen = S.neig;
+    psi.resize(ctx->pool, (size_t)S.neig * S.msb);
+    pt.next("step.predict");
+    S.t2 = clk::now();
+    S.dense = S.neig == S.msb || S.msb <= par.lanc_dim_threshold;
+    if (!S.dense) {
+      S.v0 = par.v0_mode;
       // one eigenpair only: with several, a start biased towards the previous
       // states can let a higher state pass the residual test before a missed
       // lower one surfaces, so those steps keep the random start
-      if (type == 'f' && par.predict && neig == 1 && prev.valid && prev.label == label &&
-          predict_psi(label, psi.p, msb))
-        v0 = 2;
-      lo = lanczos_device(ctx, ctx->sb, neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed, v0,
-                          par.precision, psi.p);
-      I.numop = lo.numop + lo.numop_fp32;
-      I.numop_fp32 = lo.numop_fp32;
+      if (type == 'f' && par.predict && S.neig == 1 && prev.valid && prev.label == label &&
+          predict_psi(label, psi.p, S.msb))
+        S.v0 = 2;
     }
-    const auto t3 = clk::now();
-    I.t_diag = sec(t2, t3);
-    for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = lo.evals[i];
+    return S;
+  }
+
+  void diag(StepState& S) {
+    PhaseTimer pt(ctx, "step.diag");
+    nss_step_info& I = S.st.info;
+    if (S.dense) {
+      S.lo = dense_diag(ctx, sbd(), S.neig, psi.p);
+      I.numop = 0;
+    } else {
+      S.lo = lanczos_device(ctx, sbd(), S.neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed, S.v0,
+                            par.precision, psi.p);
+      I.numop = S.lo.numop + S.lo.numop_fp32;
+      I.numop_fp32 = S.lo.numop_fp32;
+    }
+  }
+
+  void end_step(StepState& S) {
+    using clk = std::chrono::steady_clock;
+    const char type = S.type;
+    const int label = S.label, neig = S.neig;
+    const int64_t msb = S.msb;
+    Step& st = S.st;
+    nss_step_info& I = st.info;
+    S.t3 = clk::now();
+    I.t_diag = secs(S.t2, S.t3);
+    for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = S.lo.evals[i];
     // sb_get_rdm
-    pt.next("step.rdm");
+    PhaseTimer pt(ctx, "step.rdm");
     std::vector<std::vector<double>> evs[2], unused[2];
     std::vector<RhoVec> dvs[2];
-    sb_rdm_eigh2(ctx, ctx->sb, psi.p, 3, evs, unused, dvs);
+    sb_rdm_eigh2(ctx, sbd(), psi.p, 3, evs, unused, dvs);
     pt.next("step.spectrum");
     Spectrum sl = sort_spectrum(std::move(evs[0]), std::move(dvs[0]));
     Spectrum sr = sort_spectrum(std::move(evs[1]), std::move(dvs[1]));
     const auto t4 = clk::now();
-    I.t_rdm = sec(t3, t4);
+    I.t_rdm = secs(S.t3, t4);
     I.entropy_left = entropy(sl.all_desc);
     I.entropy_right = entropy(sr.all_desc);
     bool ren;
@@ -1255,8 +1303,8 @@ struct Engine {
     if (ren) {
       int ml = 0, mr = 0;
       double tl = 0.0, tr = 0.0;
-      Block nl = renormalize(left, sbL, sl, mstates, estates, ml, tl, I.cut_left, st.kept_left);
-      Block nr = renormalize(right, sbR, sr, mstates, estates, mr, tr, I.cut_right, st.kept_right);
+      Block nl = renormalize(left, sbL, sl, S.mstates, S.estates, ml, tl, I.cut_left, st.kept_left);
+      Block nr = renormalize(right, sbR, sr, S.mstates, S.estates, mr, tr, I.cut_right, st.kept_right);
       nl.parent = left.parent;
       nr.parent = right.parent;
       left = std::move(nl);
@@ -1280,8 +1328,8 @@ struct Engine {
       prev.Dr = lastDr;
       prev.piecesL = lastL.pieces;
       prev.piecesR = lastR.pieces;
-      prev.unenlL = fromL0;
-      prev.unenlR = fromR0;
+      prev.unenlL = S.fromL0;
+      prev.unenlR = S.fromR0;
       prev.newL = left.from;
       prev.newR = right.from;
       prev.psi.resize(ctx->pool, (size_t)neig * msb);
@@ -1290,9 +1338,9 @@ struct Engine {
     }
     pt.next("step.tail");
     const auto t5 = clk::now();
-    I.t_renorm = sec(t4, t5);
+    I.t_renorm = secs(t4, t5);
     I.t_hv = nss_hv_timer_flush(ctx);
-    I.t_step = sec(t0, t5);
+    I.t_step = secs(S.t0, t5);
     if (par.verbose > 0) {
       std::printf("step %c%d len %d+%d  m_sb %lld nsect %d  E/L %.15f  S %.12f  m %d,%d  nop %d  t %.3fs\n", type,
                   label, I.left_len, I.right_len, (long long)msb, I.nsect, I.energies[0] / I.L, I.entropy_left,
@@ -1300,6 +1348,12 @@ struct Engine {
       std::fflush(stdout);
     }
     steps.push_back(std::move(st));
+  }
+
+  void step(char type, int label, int sweep) {
+    StepState S = begin_step(type, label, sweep);
+    diag(S);
+    end_step(S);
   }
 
   void infinite() {
@@ -1357,6 +1411,108 @@ struct Engine {
   void finite() {
     finite_begin();
     for (int im = 0; im < (int)msweep.size(); ++im) sweep(im);
+  }
+};
+
+// Independent chains solved together.  The engines' steps run in lockstep
+// (equal LDMRG gives equal step sequences): every engine enlarges and sets
+// up its superblock, one batched Lanczos solves all ground states (the H*v
+// of all chains as one grouped launch per phase), then every engine
+// truncates.  Steps on the dense path, with LANC_NEIGEN > 1 or in mixed
+// precision use the engine's own solver inside the same lockstep.
+struct Batch {
+  nss_ctx* ctx = nullptr;
+  std::vector<Engine*> es;
+
+  void step(char type, int label, int sweep) {
+    std::vector<Engine::StepState> S;
+    S.reserve(es.size());
+    for (Engine* e : es) S.push_back(e->begin_step(type, label, sweep));
+    std::vector<LanczosJob> jobs;
+    std::vector<size_t> who;
+    for (size_t i = 0; i < es.size(); ++i) {
+      Engine* e = es[i];
+      if (S[i].dense || S[i].neig != 1 || e->par.precision != NSS_FP64) {
+        e->diag(S[i]);
+        continue;
+      }
+      LanczosJob J;
+      J.sb = e->sbd();
+      J.xdev = e->psi.p;
+      J.v0_mode = S[i].v0;
+      J.seed = e->par.seed;
+      jobs.push_back(J);
+      who.push_back(i);
+    }
+    {
+      PhaseTimer pt(ctx, "step.diag");
+      const nss_params& p0 = es[0]->par;
+      lanczos_batch(ctx, jobs, p0.lanc_kmax, p0.lanc_maxiter, p0.lanc_tol);
+    }
+    for (size_t k = 0; k < jobs.size(); ++k) {
+      Engine::StepState& St = S[who[k]];
+      St.lo = jobs[k].out;
+      St.st.info.numop = jobs[k].out.numop;
+    }
+    for (size_t i = 0; i < es.size(); ++i) es[i]->end_step(S[i]);
+  }
+
+  void finite_begin() {
+    for (Engine* e : es) {
+      e->bl1.clear();
+      e->bl2.clear();
+      e->bl1[1] = e->left;
+      e->bl2[1] = e->right;
+    }
+    while (es[0]->left.length < es[0]->par.ldmrg) {
+      step('i', 0, 0);
+      for (Engine* e : es) {
+        e->bl1[e->left.length] = e->left;
+        e->bl2[e->right.length] = e->right;
+      }
+    }
+    for (Engine* e : es) e->finite_ready = true;
+  }
+
+  void infinite() {
+    for (Engine* e : es) {
+      e->bl1.clear();
+      e->bl2.clear();
+      e->bl1[e->left.length] = e->left;
+      e->bl2[e->right.length] = e->right;
+    }
+    while (es[0]->left.length < es[0]->par.ldmrg) {
+      step('i', 0, 0);
+      for (Engine* e : es) {
+        e->bl1[e->left.length] = e->left;
+        e->bl2[e->right.length] = e->right;
+      }
+    }
+  }
+
+  // the loops of Engine::sweep, applied to every engine
+  void sweep(int im) {
+    for (Engine* e : es) {
+      NSS_REQUIRE(e->finite_ready, NSS_E_STATE, "finite sweep before nss_dmrg_batch_finite_begin");
+      NSS_REQUIRE(im >= 0 && im < (int)e->msweep.size(), NSS_E_ARG, "sweep index outside MSWEEP");
+    }
+    Engine* e0 = es[0];
+    const int fn = 2 * e0->par.ldmrg;
+    while (e0->left.length < fn) {
+      for (Engine* e : es) e->right = e->bl2.at(fn - e->left.length);
+      step('f', 1, im);
+      for (Engine* e : es) e->bl1[e->left.length] = e->left;
+    }
+    while (e0->right.length < fn) {
+      for (Engine* e : es) e->left = e->bl1.at(fn - e->right.length);
+      step('f', 2, im);
+      for (Engine* e : es) e->bl2[e->right.length] = e->right;
+    }
+    while (e0->left.length <= e0->par.ldmrg) {
+      for (Engine* e : es) e->right = e->bl2.at(fn - e->left.length);
+      step('f', 1, im);
+      for (Engine* e : es) e->bl1[e->left.length] = e->left;
+    }
   }
 };
 
@@ -1449,6 +1605,55 @@ extern "C" int nss_dmrg_sweep(nss_dmrg* d, int im) {
   NSS_REQUIRE(d && d->e, NSS_E_ARG, "nss_dmrg_sweep: NULL handle");
   NSS_HIP(hipSetDevice(d->e->ctx->device));
   d->e->sweep(im);
+  NSS_API_END
+}
+
+// Batch entry points: the engines share one context and LDMRG; each gets a
+// superblock of its own (the context's stays free for the low-level calls).
+static Batch make_batch(int n, nss_dmrg* const* ds, bool finite) {
+  NSS_REQUIRE(n >= 1 && ds, NSS_E_ARG, "nss_dmrg_batch: need at least one engine");
+  Batch B;
+  for (int i = 0; i < n; ++i) {
+    NSS_REQUIRE(ds[i] && ds[i]->e, NSS_E_ARG, "nss_dmrg_batch: NULL engine");
+    Engine* e = ds[i]->e;
+    if (!B.ctx) B.ctx = e->ctx;
+    NSS_REQUIRE(e->ctx == B.ctx, NSS_E_ARG, "nss_dmrg_batch: engines on different contexts");
+    NSS_REQUIRE(e->par.ldmrg == ds[0]->e->par.ldmrg, NSS_E_ARG, "nss_dmrg_batch: engines with different LDMRG");
+    if (finite)
+      NSS_REQUIRE(e->msweep.size() == ds[0]->e->msweep.size(), NSS_E_ARG, "nss_dmrg_batch: different NSWEEP");
+    for (Engine* o : B.es) NSS_REQUIRE(o != e, NSS_E_ARG, "nss_dmrg_batch: an engine appears twice");
+    if (!e->own_sb) e->own_sb = sb_create(e->ctx);
+    B.es.push_back(e);
+  }
+  NSS_HIP(hipSetDevice(B.ctx->device));
+  return B;
+}
+
+extern "C" int nss_dmrg_batch_run(int n, nss_dmrg* const* ds, int type) {
+  NSS_API_BEGIN
+  const bool fin = type == 'f' || type == 'F';
+  NSS_REQUIRE(fin || type == 'i' || type == 'I', NSS_E_ARG, "ERROR DMRG: unsupported DMRGtype: DMRGtype !=['i','f']");
+  Batch B = make_batch(n, ds, fin);
+  if (!fin) {
+    B.infinite();
+  } else {
+    B.finite_begin();
+    for (int im = 0; im < (int)B.es[0]->msweep.size(); ++im) B.sweep(im);
+  }
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_batch_finite_begin(int n, nss_dmrg* const* ds) {
+  NSS_API_BEGIN
+  Batch B = make_batch(n, ds, true);
+  B.finite_begin();
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_batch_sweep(int n, nss_dmrg* const* ds, int im) {
+  NSS_API_BEGIN
+  Batch B = make_batch(n, ds, true);
+  B.sweep(im);
   NSS_API_END
 }
 

@@ -175,6 +175,24 @@ void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int6
                           double* beta_out, double* alpha_out, double* partial, hipStream_t s);
 // v = w / beta with beta = sqrt(sum sq) written to bout (device).
 void launch_normalize(const double* w, double* v, int64_t n, const double* sq, double* bout, hipStream_t s);
+// Batched Lanczos over independent superblocks ("chains"): one launch per
+// vector operation covers every active chain.  Chain c owns the blocks
+// [blk0, blk0 + nch) of a launch (bchain[b] = c), one per lz_nch chunk.
+struct LzChainDev {
+  double* V;        // (kmax + 1) x n Krylov basis
+  double* w;        // H v output, then the orthogonalised residual
+  double* x;        // H v input: copy of the newest basis vector
+  double* slots;    // iteration j at slots + j * sstride (single-chain slot layout, width 2 kb1 + 1)
+  double* part;     // (kb1 + 1) x nch partial sums
+  int64_t n;
+  int64_t sstride;
+  int32_t nch, blk0, active, kmax;
+};
+int64_t lz_nch(int64_t n);
+// One Lanczos iteration j after H v (w = H v_j), every active chain: the
+// three-term step, one classical Gram-Schmidt pass against V[0..j], and
+// v_{j+1} = w / |w| (also copied to x).  kb1 = batch kmax + 1.
+void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int j, int kb1, hipStream_t s);
 // out[0] = sqrt(<x,x>)
 void launch_norm(const double* x, int64_t n, double* partial, double* out, hipStream_t s);
 // y[i] = x[i] * (1/den[0])
@@ -299,6 +317,18 @@ struct LanczosOut {
 // Eigenvectors stay on the device in xdev (neigen x n).
 LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int maxiter, double tol,
                           uint64_t seed, int v0_mode, int precision, double* xdev);
+// One job of a batched ground-state solve: an uploaded superblock, the
+// start vector (v0_mode 2: in xdev) and the resulting eigenvector (xdev).
+struct LanczosJob {
+  SbDevice* sb = nullptr;
+  double* xdev = nullptr;
+  int v0_mode = 0;
+  uint64_t seed = 0;
+  LanczosOut out;
+};
+// Lowest eigenpair of every job, the H*v of all chains in one grouped launch
+// per phase (nss_lanczos.cpp).
+void lanczos_batch(::nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax, int maxiter, double tol);
 // Dense path: H_sb assembled on the device from unit vectors, diagonalised on the host.
 LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev);
 
@@ -325,6 +355,13 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL,
                   const std::vector<OpRef>& HR, const std::vector<SbTerm>& terms, bool want_fp32);
 void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);          // device, async
 void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);     // device, async
+// Batch H*v: merged W and Y plans of several superblocks whose vectors sit at
+// xoff[c] of shared X / Y arenas and whose term scratch sits at woff[c] of a
+// shared W arena (sb_wsize elements each); hv_plans launches them (timed).
+int64_t sb_wsize(const SbDevice* sb);
+void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s);
+void hv_plans(::nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b);
 int64_t sb_n(const SbDevice* sb);
 double sb_flops(const SbDevice* sb);
 int sb_nq(const SbDevice* sb);

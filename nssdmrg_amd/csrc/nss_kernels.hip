@@ -699,6 +699,155 @@ void launch_normalize(const double* w, double* v, int64_t n, const double* sq, d
   NSS_HIP(hipGetLastError());
 }
 
+// ---- batched Lanczos: one launch per vector operation for every active chain
+// Block b of a launch works on chain bchain[b], chunk b - blk0 of that chain
+// (LZ_CHUNK elements).  The per-chain slot of iteration j lies at
+// slots + j * sstride with the single-chain layout: [0, kb1) Gram-Schmidt
+// coefficients, kb1 + i the recurrence terms, 2 kb1 beta_j.  Reductions are
+// fixed-order (partials per chunk, then a block-cooperative sum).
+
+// w -= beta_{j-1} v_{j-1} (j > 0); part[chunk] = <v_j, w>; chunk 0 copies beta_{j-1}
+__global__ __launch_bounds__(256) void k_lzb_alpha(const LzChainDev* __restrict__ ch,
+                                                   const int32_t* __restrict__ bchain, int j, int kb1) {
+  __shared__ double sh[4];
+  const LzChainDev C = ch[bchain[blockIdx.x]];
+  if (!C.active) return;
+  const int64_t chunk = blockIdx.x - C.blk0;
+  const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
+  const double* vj = C.V + (size_t)j * C.n;
+  const double* vp = C.V + (size_t)(j > 0 ? j - 1 : 0) * C.n;
+  const double b = j > 0 ? C.slots[(size_t)(j - 1) * C.sstride + 2 * kb1] : 0.0;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    double x = C.w[i];
+    if (j > 0) {
+      x -= b * vp[i];
+      C.w[i] = x;
+    }
+    acc += vj[i] * x;
+  }
+  const double r = block_sum256(acc, sh);
+  if (threadIdx.x == 0) {
+    C.part[chunk] = r;
+    if (chunk == 0 && j > 0) C.slots[(size_t)j * C.sstride + kb1 + j - 1] = b;
+  }
+}
+
+// alpha = sum of the chain's partials; w -= alpha v_j; chunk 0 stores alpha
+__global__ __launch_bounds__(256) void k_lzb_sub(const LzChainDev* __restrict__ ch,
+                                                 const int32_t* __restrict__ bchain, int j, int kb1) {
+  __shared__ double sh[4];
+  __shared__ double ash;
+  const LzChainDev C = ch[bchain[blockIdx.x]];
+  if (!C.active) return;
+  const int64_t chunk = blockIdx.x - C.blk0;
+  const double a0 = coop_sum(C.part, C.nch, sh);
+  if (threadIdx.x == 0) {
+    ash = a0;
+    if (chunk == 0) C.slots[(size_t)j * C.sstride + kb1 + j] = a0;
+  }
+  __syncthreads();
+  const double a = ash;
+  const double* vj = C.V + (size_t)j * C.n;
+  const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) C.w[i] -= a * vj[i];
+}
+
+// part[r * nch + chunk] = <V_r, w> over the chunk, r = blockIdx.y <= j
+__global__ __launch_bounds__(256) void k_lzb_gemvt(const LzChainDev* __restrict__ ch,
+                                                   const int32_t* __restrict__ bchain) {
+  __shared__ double sh[4];
+  const LzChainDev C = ch[bchain[blockIdx.x]];
+  if (!C.active) return;
+  const int64_t chunk = blockIdx.x - C.blk0;
+  const int r = blockIdx.y;
+  const double* v = C.V + (size_t)r * C.n;
+  const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * C.w[i];
+  const double res = block_sum256(acc, sh);
+  if (threadIdx.x == 0) C.part[(size_t)r * C.nch + chunk] = res;
+}
+
+// h_r = sum_c part[r * nch + c] (r <= j, every block, one thread per r, a
+// fixed order; chunk 0 publishes h); w -= V[0..j] h; the chunk's sum of
+// squares of the new w to part[kb1 * nch + chunk]
+__global__ __launch_bounds__(256) void k_lzb_update(const LzChainDev* __restrict__ ch,
+                                                    const int32_t* __restrict__ bchain, int j, int kb1) {
+  extern __shared__ double hs[];
+  __shared__ double red[4];
+  const LzChainDev C = ch[bchain[blockIdx.x]];
+  if (!C.active) return;
+  const int64_t chunk = blockIdx.x - C.blk0;
+  const int m = j + 1;
+  for (int r = threadIdx.x; r < m; r += 256) {
+    const double* p = C.part + (size_t)r * C.nch;
+    double a0 = 0.0, a1 = 0.0;
+    int c = 0;
+    for (; c + 2 <= C.nch; c += 2) {
+      a0 += p[c];
+      a1 += p[c + 1];
+    }
+    if (c < C.nch) a0 += p[c];
+    hs[r] = a0 + a1;
+    if (chunk == 0) C.slots[(size_t)j * C.sstride + r] = a0 + a1;
+  }
+  __syncthreads();
+  const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
+  double ss = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int r = 0;
+    for (; r + 4 <= m; r += 4) {
+      a0 += C.V[(size_t)r * C.n + i] * hs[r];
+      a1 += C.V[(size_t)(r + 1) * C.n + i] * hs[r + 1];
+      a2 += C.V[(size_t)(r + 2) * C.n + i] * hs[r + 2];
+      a3 += C.V[(size_t)(r + 3) * C.n + i] * hs[r + 3];
+    }
+    for (; r < m; ++r) a0 += C.V[(size_t)r * C.n + i] * hs[r];
+    const double x = C.w[i] - ((a0 + a1) + (a2 + a3));
+    C.w[i] = x;
+    ss += x * x;
+  }
+  const double rs = block_sum256(ss, red);
+  if (threadIdx.x == 0) C.part[(size_t)kb1 * C.nch + chunk] = rs;
+}
+
+// beta_j = sqrt(sum of the chain's squares); v_{j+1} = w / beta_j, also to
+// x (the next H*v input); chunk 0 stores beta_j
+__global__ __launch_bounds__(256) void k_lzb_normalize(const LzChainDev* __restrict__ ch,
+                                                       const int32_t* __restrict__ bchain, int j, int kb1) {
+  __shared__ double sh[4];
+  __shared__ double bsh;
+  const LzChainDev C = ch[bchain[blockIdx.x]];
+  if (!C.active) return;
+  const int64_t chunk = blockIdx.x - C.blk0;
+  const double ssum = coop_sum(C.part + (size_t)kb1 * C.nch, C.nch, sh);
+  if (threadIdx.x == 0) {
+    bsh = sqrt(ssum);
+    if (chunk == 0) C.slots[(size_t)j * C.sstride + 2 * kb1] = bsh;
+  }
+  __syncthreads();
+  const double inv = 1.0 / bsh;
+  double* v = C.V + (size_t)(j + 1) * C.n;
+  const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double y = C.w[i] * inv;
+    v[i] = y;
+    C.x[i] = y;
+  }
+}
+
+void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int j, int kb1, hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_lzb_alpha, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
+  hipLaunchKernelGGL(k_lzb_sub, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
+  hipLaunchKernelGGL(k_lzb_gemvt, dim3(nblk, j + 1), dim3(256), 0, s, ch, bchain);
+  hipLaunchKernelGGL(k_lzb_update, dim3(nblk), dim3(256), sizeof(double) * (size_t)(j + 1), s, ch, bchain, j, kb1);
+  hipLaunchKernelGGL(k_lzb_normalize, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
+  NSS_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(256) void k_sumsq_partial(const double* __restrict__ x, int64_t n,
                                                        double* __restrict__ partial) {
   __shared__ double sh[4];

@@ -294,6 +294,194 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   return out;
 }
 
+// Lowest eigenpair of several independent superblocks at once: plain
+// Lanczos with full re-orthogonalisation, every chain in lockstep.  Each
+// iteration is one merged W launch and one merged Y launch over all active
+// chains (their vectors in shared X / Y arenas) and five batched vector
+// launches; the host reads every chain's new slots in one copy at the
+// checks (iterations 1..8, then every 4th).  A chain leaves the batch when
+// its residual |beta_{m-1} y_{m-1}| meets tol max(1, |theta|) (the
+// single-chain criterion), at an invariant subspace, or at its kmax; one
+// that is still unconverged there is finished by the single-chain
+// thick-restart solver from its current Ritz vector.
+void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, int maxiter, double tol) {
+  const int K = (int)jobs.size();
+  if (K == 0) return;
+  hipStream_t s = ctx->stream;
+  if (kmax_req <= 0) kmax_req = 150;
+  std::vector<int64_t> n(K), xoff(K), woff(K), voff(K), poff(K);
+  std::vector<int> kmc(K), nch(K), blk0(K);
+  std::vector<SbDevice*> sbs(K);
+  int64_t xtot = 0, wtot = 0, vtot = 0, ptot = 0;
+  int btot = 0, kb = 0;
+  for (int c = 0; c < K; ++c) {
+    sbs[c] = jobs[c].sb;
+    n[c] = sb_n(sbs[c]);
+    NSS_REQUIRE(n[c] > 0, NSS_E_STATE, "lanczos_batch: empty superblock");
+    kmc[c] = (int)std::min<int64_t>(kmax_req, n[c]);
+    kmc[c] = std::max(kmc[c], (int)std::min<int64_t>(n[c], 3));
+    kb = std::max(kb, kmc[c]);
+  }
+  const int kb1 = kb + 1, S = 2 * kb1 + 1;
+  for (int c = 0; c < K; ++c) {
+    nch[c] = (int)lz_nch(n[c]);
+    xoff[c] = xtot;
+    xtot += n[c];
+    woff[c] = wtot;
+    wtot += std::max<int64_t>(1, sb_wsize(sbs[c]));
+    voff[c] = vtot;
+    vtot += (int64_t)(kmc[c] + 1) * n[c];
+    poff[c] = ptot;
+    ptot += (int64_t)(kb1 + 1) * nch[c];
+    blk0[c] = btot;
+    btot += nch[c];
+  }
+  DevArr<double> X, Y, Wa, V, P, SL, scal, yd;
+  X.resize(ctx->pool, xtot);
+  Y.resize(ctx->pool, xtot);
+  Wa.resize(ctx->pool, wtot);
+  V.resize(ctx->pool, vtot);
+  P.resize(ctx->pool, ptot);
+  SL.resize(ctx->pool, (size_t)kb * K * S);
+  scal.resize(ctx->pool, K);
+  yd.resize(ctx->pool, kb1);
+  NSS_HIP(hipMemsetAsync(SL.p, 0, (size_t)kb * K * S * sizeof(double), s));
+  GemmPlan Wp, Yp;
+  sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx->pool, s);
+  std::vector<LzChainDev> hc(K);
+  std::vector<int32_t> bmap(btot);
+  for (int c = 0; c < K; ++c) {
+    LzChainDev& C = hc[c];
+    C.V = V.p + voff[c];
+    C.w = Y.p + xoff[c];
+    C.x = X.p + xoff[c];
+    C.slots = SL.p + (size_t)c * S;
+    C.part = P.p + poff[c];
+    C.n = n[c];
+    C.sstride = (int64_t)K * S;
+    C.nch = nch[c];
+    C.blk0 = blk0[c];
+    C.active = 1;
+    C.kmax = kmc[c];
+    for (int b = 0; b < nch[c]; ++b) bmap[blk0[c] + b] = c;
+  }
+  DevArr<LzChainDev> dch;
+  DevArr<int32_t> dmap;
+  dch.resize(ctx->pool, K);
+  dmap.resize(ctx->pool, btot);
+  NSS_HIP(hipMemcpyAsync(dch.p, hc.data(), K * sizeof(LzChainDev), hipMemcpyHostToDevice, s));
+  NSS_HIP(hipMemcpyAsync(dmap.p, bmap.data(), btot * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  // start vectors: the caller's (v0_mode 2, unless vanishing or non-finite),
+  // constant (1) or SplitMix64 (0); normalised into V_c[0] and X_c
+  for (int c = 0; c < K; ++c) {
+    LanczosJob& J = jobs[c];
+    double* v0 = hc[c].V;
+    bool filled = false;
+    if (J.v0_mode == 2) {
+      NSS_HIP(hipMemcpyAsync(v0, J.xdev, n[c] * sizeof(double), hipMemcpyDeviceToDevice, s));
+      launch_norm(v0, n[c], hc[c].part, scal.p + c, s);
+      double nrm = 0.0;
+      NSS_HIP(hipMemcpyAsync(&nrm, scal.p + c, sizeof(double), hipMemcpyDeviceToHost, s));
+      NSS_HIP(hipStreamSynchronize(s));
+      filled = std::isfinite(nrm) && nrm > 1e-8;
+    }
+    if (!filled) {
+      if (J.v0_mode == 1) launch_fill_const(v0, n[c], 1.0 / std::sqrt((double)n[c]), s);
+      else launch_fill_splitmix(v0, n[c], J.seed, s);
+    }
+    launch_norm(v0, n[c], hc[c].part, scal.p + c, s);
+    launch_div(v0, v0, n[c], scal.p + c, s);
+    NSS_HIP(hipMemcpyAsync(hc[c].x, v0, n[c] * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  // host side of every chain: the projected tridiagonal
+  std::vector<std::vector<double>> d(K, std::vector<double>(kb1, 0.0)), e(K, std::vector<double>(kb1, 0.0)),
+      beta(K, std::vector<double>(kb1, 0.0));
+  std::vector<double> hn(K, 0.0);
+  std::vector<int> numop(K, 0);
+  std::vector<char> active(K, 1), fallback(K, 0);
+  int nact = K, j = 0, read_upto = 0;
+  const Bases bases{X.p, Wa.p, Y.p};
+  std::vector<double> hs;
+  while (nact > 0) {
+    hv_plans(ctx, Wp, Yp, bases);
+    launch_lzb_iteration(dch.p, dmap.p, btot, j, kb1, s);
+    ++j;
+    bool at_end = false;
+    for (int c = 0; c < K; ++c)
+      if (active[c]) {
+        ++numop[c];
+        at_end = at_end || j == kmc[c];
+      }
+    if (!(j <= 8 || j % 4 == 0 || at_end)) continue;
+    const int cnt = j - read_upto;
+    hs.resize((size_t)cnt * K * S);
+    NSS_HIP(hipMemcpyAsync(hs.data(), SL.p + (size_t)read_upto * K * S, hs.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipStreamSynchronize(s));
+    bool changed = false;
+    for (int c = 0; c < K; ++c) {
+      if (!active[c]) continue;
+      int m = j;
+      bool breakdown = false;
+      for (int it = read_upto; it < j; ++it) {
+        const double* sl = hs.data() + ((size_t)(it - read_upto) * K + c) * S;
+        d[c][it] = sl[it] + sl[kb1 + it];
+        if (it > 0) e[c][it - 1] = sl[it - 1] + sl[kb1 + it - 1];
+        beta[c][it] = sl[2 * kb1];
+        hn[c] = std::max(hn[c], std::fabs(d[c][it]));
+        if (beta[c][it] <= 1e-14 * std::max(1.0, hn[c]) || !std::isfinite(beta[c][it])) {  // invariant subspace
+          m = it + 1;
+          breakdown = true;
+          break;
+        }
+      }
+      std::vector<double> th(m), zl(m);
+      int row = m - 1;
+      tridiag_eig(m, d[c].data(), e[c].data(), th.data(), 1, &row, zl.data());
+      const double res = std::fabs(beta[c][m - 1] * zl[0]);
+      const bool conv = res <= tol * std::max(1.0, std::fabs(th[0]));
+      if (!(conv || breakdown || m >= kmc[c])) continue;
+      // Ritz vector x = V[0..m) y, y the lowest eigenvector of the tridiagonal
+      std::vector<double> Yv((size_t)m * m), y(m);
+      tridiag_eig_full(m, d[c].data(), e[c].data(), th.data(), Yv.data());
+      for (int a = 0; a < m; ++a) y[a] = Yv[(size_t)a * m];
+      NSS_HIP(hipMemcpyAsync(yd.p, y.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_gemv_n(hc[c].V, n[c], m, yd.p, jobs[c].xdev, n[c], 1, s);
+      launch_norm(jobs[c].xdev, n[c], hc[c].part, scal.p + c, s);
+      launch_div(jobs[c].xdev, jobs[c].xdev, n[c], scal.p + c, s);
+      NSS_HIP(hipStreamSynchronize(s));  // y lives on the host stack
+      jobs[c].out.evals.assign(1, th[0]);
+      jobs[c].out.numop = numop[c];
+      fallback[c] = !(conv || breakdown);
+      active[c] = 0;
+      hc[c].active = 0;
+      --nact;
+      changed = true;
+    }
+    read_upto = j;
+    if (changed && nact > 0) {
+      // drop the finished chains from the merged H*v and the vector launches
+      NSS_HIP(hipMemcpyAsync(dch.p, hc.data(), K * sizeof(LzChainDev), hipMemcpyHostToDevice, s));
+      std::vector<SbDevice*> asb;
+      std::vector<int64_t> ax, aw;
+      for (int c = 0; c < K; ++c)
+        if (active[c]) {
+          asb.push_back(sbs[c]);
+          ax.push_back(xoff[c]);
+          aw.push_back(woff[c]);
+        }
+      sb_merge_hv_plans(asb, ax, aw, Wp, Yp, ctx->pool, s);
+    }
+  }
+  NSS_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c)
+    if (fallback[c]) {
+      LanczosOut o = lanczos_device(ctx, sbs[c], 1, kmax_req, maxiter, tol, jobs[c].seed, 2, NSS_FP64, jobs[c].xdev);
+      jobs[c].out.evals = o.evals;
+      jobs[c].out.numop += o.numop;
+    }
+}
+
 // Dense path of sb_diag (m_sb <= LANC_DIM_THRESHOLD, src/DMRG_SUPERBLOCK.f90:438-467):
 // H_sb is assembled on the device column by column (H e_j) and diagonalised.
 LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev) {

@@ -339,6 +339,56 @@ void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
   launch_f2d(sb->Yf.p, y, sb->n, ctx->stream);
 }
 
+// Merged H*v plans of several superblocks for one W and one Y launch: chain
+// c's vector operands (x, y) sit at xoff[c] of the batch's X / Y arenas and
+// its term scratch at woff[c] of the W arena; operator blocks are absolute.
+int64_t sb_wsize(const SbDevice* sb) { return sb->wsize; }
+
+static void plan_append(GemmPlan& dst, const GemmPlan& src, int64_t xoff, int64_t woff) {
+  auto shift = [&](int base) -> int64_t {
+    return base == BASE_X || base == BASE_Y ? xoff : base == BASE_W ? woff : 0;
+  };
+  const int seg0 = (int)dst.hsegs.size();
+  for (GemmSeg g : src.hsegs) {
+    g.aoff += shift(g.abase);
+    g.boff += shift(g.bbase);
+    dst.hsegs.push_back(g);
+  }
+  for (GemmTile t : src.outs) {
+    t.coff += shift(t.cbase);
+    t.seg0 += seg0;
+    dst.outs.push_back(t);
+  }
+  dst.flops += src.flops;
+}
+
+void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s) {
+  W.clear();
+  Y.clear();
+  for (size_t c = 0; c < sbs.size(); ++c) {
+    plan_append(W, sbs[c]->W64, xoff[c], woff[c]);
+    plan_append(Y, sbs[c]->Y64, xoff[c], woff[c]);
+  }
+  W.upload(pool, s);
+  Y.upload(pool, s);
+}
+
+// H*v of merged plans, timed like sb_hv.
+void hv_plans(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b) {
+  const bool t = ctx->time_hv;
+  if (t) record(ctx);
+  launch_gemm_f64(W, b, ctx->stream, GEMM_HV_W);
+  if (t) record(ctx);
+  launch_gemm_f64(Y, b, ctx->stream, GEMM_HV_Y);
+  if (t) {
+    record(ctx);
+    ctx->ev_kind.push_back(0);
+    ctx->ev_fw.push_back(W.flops);
+    ctx->ev_fy.push_back(Y.flops);
+  }
+}
+
 double nss_hv_timer_flush(nss_ctx* ctx) {
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   double tot = 0.0;

@@ -295,3 +295,70 @@ def test_engine_superblock_spectrum_matches_oracle(ctx):
     # and the engine's GPU plan applies exactly that matrix
     v = O.splitmix64_uniform_fast(sb.n, 2)
     assert np.linalg.norm(ctx.hv(v) - O.hv_direct(sb, v)) < 1e-12 * np.linalg.norm(O.hv_direct(sb, v))
+
+
+def _batch_cases():
+    # chains of one batch: different models, m, U and fillings, equal LDMRG
+    return [
+        dict(kind="hubbard", U=2.0, tq=(0.5, 0.5), M=32, ms=[32, 40]),
+        dict(kind="hubbard", U=6.0, tq=(0.375, 0.375), M=24, ms=[24, 28]),
+        dict(kind="spin", U=0.0, tq=(0.0, 0.0), M=20, ms=[20, 24]),
+        dict(kind="hubbard", U=0.5, tq=(0.5, 0.5), M=40, ms=[40, 48]),
+    ]
+
+
+@pytest.mark.parametrize("predict", [False, True])
+def test_batch_matches_oracle_and_single_runs(ctx, predict):
+    # Independent chains stepped together with one batched Lanczos per step:
+    # each chain matches the oracle (E to 1e-10 relative, kept counts per QN
+    # above W_MIN, entropy) and its own standalone run.
+    cases = _batch_cases()
+    L = 6
+    print(f"case batch predict={predict} chains={len(cases)}", flush=True)
+    kws, oracles = [], []
+    for c in cases:
+        if c["kind"] == "hubbard":
+            site, osite, hij, tq = N.electron_site(), O.electron_site(c["U"], True), HUB_HIJ, c["tq"]
+            otq = tq
+        else:
+            site, osite, hij, tq = N.spin_site(2), O.spin_site(2), SPIN_HIJ, c["tq"]
+            otq = (0.0,)
+        kws.append(dict(site=site, hij=np.array(hij), ldmrg=L, mdmrg=c["M"], msweep=c["ms"], target_qn=tq,
+                        neigen=1, lanc_dim_threshold=32, uloc=c["U"], hfmode=True, predict=predict))
+        p = O.Params(ldmrg=L, mdmrg=c["M"], target_qn=otq, hij=hij, neigen=1, lanc_dim_threshold=32,
+                     msweep=c["ms"])
+        od = O.DMRG(osite, p)
+        od.finite()
+        oracles.append(od)
+    singles = [run(ctx, "f", **kw)[1] for kw in kws]
+    engines = [N.DMRG(ctx, N.DMRGConfig(**kw)) for kw in kws]
+    batch = N.DMRGBatch(engines)
+    res = batch.run("f")
+    for ci, (st, single, od, d) in enumerate(zip(res, singles, oracles, engines)):
+        assert len(st) == len(single) == len(od.records), ci
+        nlz = 0
+        for i, (s, s1, r) in enumerate(zip(st, single, od.records)):
+            e = r.energies[0]
+            assert abs(s["energies"][0] - e) < 1e-10 * max(1.0, abs(e)), (ci, i)
+            assert abs(s["energies"][0] - s1["energies"][0]) < 1e-10 * max(1.0, abs(e)), (ci, i)
+            assert abs(s["entropy_left"] - r.entropy_left) < 1e-8, (ci, i)
+            if s["renormalized"]:
+                # total kept counts include rho null-space states (arbitrary
+                # in the reference too): compared only when every kept
+                # state carries weight above W_MIN
+                if min(s["cut_left"], s["cut_right"], s1["cut_left"], s1["cut_right"]) > W_MIN:
+                    assert s["m_left"] == s1["m_left"] and s["m_right"] == s1["m_right"], (ci, i)
+                assert d.kept_qn(i, "left", W_MIN) == _kept_oracle(r, W_MIN), (ci, i)
+            nlz += s["numop"] > 0
+        assert nlz > 10, ci
+
+
+def test_batch_rejects_mismatched_engines(ctx):
+    a = N.DMRG(ctx, N.DMRGConfig(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=4, mdmrg=8,
+                                 target_qn=(0.0, 0.0)))
+    b = N.DMRG(ctx, N.DMRGConfig(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=5, mdmrg=8,
+                                 target_qn=(0.0, 0.0)))
+    with pytest.raises(N.NssError, match="LDMRG"):
+        N.DMRGBatch([a, b]).run("i")
+    with pytest.raises(N.NssError, match="twice"):
+        N.DMRGBatch([a, a]).run("i")
