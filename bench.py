@@ -20,6 +20,11 @@ value = H_sb.psi GFLOP/s: algorithmic FLOPs of every H.v in the timed sweeps
 with HIP events on the context stream; ms_per_step = finite-sweep wall time.
 The FP32-MFMA path (FP64 Lanczos recurrence, rho and final FP64 restart)
 is timed on the same workload and reported under "fp32_mixed".
+
+--workload hubbard_batch64_L32: 64 independent Hubbard chains of L=32 (m,
+U and filling per chain from the seed) stepped in lockstep, every step's
+ground states from one batched Lanczos whose H.v is one grouped launch per
+phase over all chains; a step is one sweep of all chains.
 """
 from __future__ import annotations
 
@@ -47,7 +52,65 @@ WORKLOADS = {
     "hubbard_L32_m128": dict(model="electron", L=32, m=128, U=4.0, seed=1),
     "spin12_L40_m64": dict(model="spin12", L=40, m=64, U=0.0, seed=0),
     "spin1_L100_m1024": dict(model="spin1", L=100, m=1024, U=0.0, seed=2),
+    # 64 independent Hubbard chains, L=32, per chain from the
+    # seed m in [128, 512], U in [0, 8), N in {16, 14, 18} (N_up = N_dw = N/2)
+    "hubbard_batch64_L32": dict(model="electron", L=32, batch=64, m_lo=128, m_hi=512, U_hi=8.0,
+                                fillings=(16, 14, 18), seed=3),
 }
+
+
+def batch_chains(w):
+    """Per-chain (m, U, N) of a batch workload, drawn from its seed."""
+    rng = np.random.default_rng(w["seed"])
+    out = []
+    for _ in range(w["batch"]):
+        m = int(rng.integers(w["m_lo"], w["m_hi"] + 1))
+        U = float(rng.uniform(0.0, w["U_hi"]))
+        nf = int(rng.choice(w["fillings"]))
+        out.append((m, U, nf))
+    return out
+
+
+def run_batch(N, ctx, w, warmup, steps, verbose, sequential=False):
+    """iDMRG growth and `steps` timed finite sweeps of all chains in lockstep
+    (nss_dmrg_batch_*): each step's ground states from one batched Lanczos."""
+    L = w["L"]
+    ldmrg = (L - 2) // 2
+    engines = []
+    for c, (m, U, nf) in enumerate(batch_chains(w)):
+        q = nf / (2.0 * L)  # QNtype local: int(q * L) electrons of each spin at L sites
+        cfg = N.DMRGConfig(site=N.electron_site(), hij=np.diag([-1.0, -1.0]), ldmrg=ldmrg, mdmrg=m,
+                           msweep=[m] * (warmup + steps), target_qn=(q, q), uloc=U, hfmode=True, neigen=1,
+                           lanc_dim_threshold=128, lanc_tol=1e-12, seed=w["seed"] + c, precision="fp64",
+                           verbose=verbose if c == 0 else 0, predict=True)
+        engines.append(N.DMRG(ctx, cfg))
+    if sequential:  # the same chains one after another, each with its own solver calls
+        class _Seq:
+            def finite_begin(self):
+                for e in engines:
+                    e.finite_begin()
+
+            def sweep(self, im):
+                for e in engines:
+                    e.sweep(im)
+        B = _Seq()
+    else:
+        B = N.DMRGBatch(engines)
+    t0 = time.perf_counter()
+    B.finite_begin()
+    t_idmrg = time.perf_counter() - t0
+    for im in range(warmup):
+        B.sweep(im)
+    n0 = [len(e.steps()) for e in engines]
+    ctx.hv_stats(reset=True)
+    walls = []
+    for im in range(warmup, warmup + steps):
+        t = time.perf_counter()
+        B.sweep(im)
+        walls.append(time.perf_counter() - t)
+    st = ctx.hv_stats(reset=True)
+    timed = [e.steps()[k:] for e, k in zip(engines, n0)]
+    return engines, dict(t_idmrg=t_idmrg, walls=walls, stats=st, steps=timed)
 
 
 def make_engine(N, ctx, w, nsweeps, precision, verbose=0, predict=True):
@@ -137,6 +200,65 @@ def copy_probe(mib, reps):
     return 2.0 * n * 8 / sec.value / 1e9
 
 
+def main_batch(N, ctx, w, a):
+    print(f"[bench] workload {a.workload}: {w['batch']} chains in lockstep, FP64, {a.warmup} warmup + "
+          f"{a.steps} timed sweeps", flush=True)
+    copy_gbs = copy_probe(1024, 10)  # before the chains fill the device memory
+    engines, r = run_batch(N, ctx, w, a.warmup, a.steps, a.verbose, sequential=a.sequential)
+    st = r["stats"]
+    hv_time = st["t_w"] + st["t_y"]
+    value = st["flops"] / hv_time / 1e9 if hv_time > 0 else 0.0
+    ms_per_step = 1e3 * float(np.mean(r["walls"]))
+    calls = max(1, st["calls"])
+    y_avg_s = st["t_y"] / calls
+    y_flops = st["flops_y"] / calls
+    chains = batch_chains(w)
+    steps = [s for ch in r["steps"] for s in ch]
+    e_last = [ch[-1]["e_per_site"][0] for ch in r["steps"]]
+    print(f"[bench] batch done: {ms_per_step:.1f} ms/sweep, H.v {value:.1f} GFLOP/s", flush=True)
+    out = {
+        "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": a.workload, "model": "hubbard", "chains": w["batch"], "L": w["L"],
+                   "ldmrg": (w["L"] - 2) // 2, "m": [c[0] for c in chains], "U": [round(c[1], 6) for c in chains],
+                   "N": [c[2] for c in chains], "bc": "obc", "hfmode": True, "neigen": 1, "lanc_tol": 1e-12,
+                   "seed": w["seed"], "lanczos": ("each chain alone, in turn" if a.sequential else
+                                                  "one batched Lanczos per step over all chains") +
+                   ", predicted starts"},
+        "roofline": {"bound": "mfma", "kernel": "k_hv_y_f64",
+                     "kernel_variants": "k_hv_y_f64 (32x32 split-K) or k_hv_y_f64_t64 (64x64 LDS-staged), "
+                                        "chosen per merged launch by its tile count; the figures cover both",
+                     "achieved": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0,
+                     "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                     "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA if y_avg_s > 0 else 0.0,
+                     "traffic": profiled_traffic(a.workload, "k_hv_y_f64"),
+                     "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
+                     "w_kernel": {"name": "k_hv_w_f64", "avg_launch_us": 1e6 * st["t_w"] / calls,
+                                  "flops_per_launch": st["flops_w"] / calls}},
+        "sweep_wall_s": r["walls"], "idmrg_wall_s": r["t_idmrg"], "hv_calls": st["calls"], "hv_gpu_s": hv_time,
+        "e0_per_site_last": e_last,
+        "numop_per_step": {"max_over_chains": [int(max(ch[i]["numop"] for ch in r["steps"]))
+                                               for i in range(len(r["steps"][0]))],
+                           "mean_over_chains": [float(np.mean([ch[i]["numop"] for ch in r["steps"]]))
+                                                for i in range(len(r["steps"][0]))]},
+        "time_split_s": {k: float(sum(s_[k] for s_ in steps)) for k in
+                         ("t_enlarge", "t_diag", "t_rdm", "t_renorm", "t_step")},
+        "note": "t_* sum over chains; a batch step's Lanczos time is booked on every chain of the step, its "
+                "H.v time on the first",
+    }
+    out["hbm_copy_probe_gbs"] = copy_gbs
+    if os.environ.get("NSS_PROFILE") == "1":
+        out["phase_timers_s"] = ctx.timers()
+    if not a.no_cpu:
+        print("[bench] CPU baseline (1 core, scalar C restatement, chain 0)", flush=True)
+        out["cpu_baseline"] = cpu_baseline(engines[0])
+    for e in engines:
+        e.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +270,8 @@ def main():
     ap.add_argument("--no-random-start", action="store_true",
                     help="skip the FP64 run whose every Lanczos starts from the seeded random vector")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--sequential", action="store_true",
+                    help="batch workloads: run the chains one after another instead of in one batch")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="end with a plain device copy of MIB MiB (counter calibration under rocprofv3 --pmc)")
     a = ap.parse_args()
@@ -158,6 +282,9 @@ def main():
     import nssdmrg_amd as N
     w = WORKLOADS[a.workload]
     ctx = N.Context(0)
+    if "batch" in w:
+        main_batch(N, ctx, w, a)
+        return
     print(f"[bench] workload {a.workload}: FP64 path, {a.warmup} warmup + {a.steps} timed sweeps", flush=True)
     d64, r64 = run_path(N, ctx, w, a.warmup, a.steps, "fp64", a.verbose)
     st = r64["stats"]

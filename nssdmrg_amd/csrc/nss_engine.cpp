@@ -1269,24 +1269,45 @@ struct Engine {
     }
   }
 
+  // rho eigenpairs of a step (sb_get_rdm): both sides, vectors on the device
+  struct RdmOut {
+    std::vector<std::vector<double>> evs[2], unused[2];
+    std::vector<RhoVec> dvs[2];
+  };
+  RdmJob rdm_job(RdmOut& o) {
+    RdmJob J;
+    J.sb = sbd();
+    J.psi = psi.p;
+    J.sides = 3;
+    J.evals = o.evs;
+    J.evecs = o.unused;
+    J.dvecs = o.dvs;
+    return J;
+  }
+
   void end_step(StepState& S) {
+    S.t3 = std::chrono::steady_clock::now();
+    RdmOut o;
+    {
+      PhaseTimer pt(ctx, "step.rdm");
+      sb_rdm_eigh2(ctx, sbd(), psi.p, 3, o.evs, o.unused, o.dvs);
+    }
+    finish_step(S, o);
+  }
+
+  // everything after the rho eigensolve; S.t3 marks the end of the diagonalisation
+  void finish_step(StepState& S, RdmOut& o) {
     using clk = std::chrono::steady_clock;
     const char type = S.type;
     const int label = S.label, neig = S.neig;
     const int64_t msb = S.msb;
     Step& st = S.st;
     nss_step_info& I = st.info;
-    S.t3 = clk::now();
     I.t_diag = secs(S.t2, S.t3);
     for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = S.lo.evals[i];
-    // sb_get_rdm
-    PhaseTimer pt(ctx, "step.rdm");
-    std::vector<std::vector<double>> evs[2], unused[2];
-    std::vector<RhoVec> dvs[2];
-    sb_rdm_eigh2(ctx, sbd(), psi.p, 3, evs, unused, dvs);
-    pt.next("step.spectrum");
-    Spectrum sl = sort_spectrum(std::move(evs[0]), std::move(dvs[0]));
-    Spectrum sr = sort_spectrum(std::move(evs[1]), std::move(dvs[1]));
+    PhaseTimer pt(ctx, "step.spectrum");
+    Spectrum sl = sort_spectrum(std::move(o.evs[0]), std::move(o.dvs[0]));
+    Spectrum sr = sort_spectrum(std::move(o.evs[1]), std::move(o.dvs[1]));
     const auto t4 = clk::now();
     I.t_rdm = secs(S.t3, t4);
     I.entropy_left = entropy(sl.all_desc);
@@ -1454,7 +1475,22 @@ struct Batch {
       St.lo = jobs[k].out;
       St.st.info.numop = jobs[k].out.numop;
     }
-    for (size_t i = 0; i < es.size(); ++i) es[i]->end_step(S[i]);
+    // rho of every chain in one batch; the eigenvector views live in rho
+    // until every chain has renormalized
+    const auto t3 = std::chrono::steady_clock::now();
+    std::vector<Engine::RdmOut> ro(es.size());
+    std::vector<RdmJob> rj;
+    for (size_t i = 0; i < es.size(); ++i) {
+      S[i].t3 = t3;
+      rj.push_back(es[i]->rdm_job(ro[i]));
+    }
+    DevArr<double> rho;
+    GemmPlan R;
+    {
+      PhaseTimer pt(ctx, "step.rdm");
+      sb_rdm_eigh_multi(ctx, rj, rho, R);
+    }
+    for (size_t i = 0; i < es.size(); ++i) es[i]->finish_step(S[i], ro[i]);
   }
 
   void finite_begin() {

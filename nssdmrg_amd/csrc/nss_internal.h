@@ -388,6 +388,19 @@ struct RhoVec {
 void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
                   std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
                   std::vector<RhoVec>* dvecs = nullptr);
+// Several superblocks at once (independent chains): job j's rho blocks of
+// psi_j, its eigenvalues to evals[2], eigenvectors to evecs[2] (host) or
+// dvecs[2] (views into rho, valid until rho is reused).  One GEMM launch,
+// one Jacobi launch and one rocSOLVER batch per size class for all jobs.
+struct RdmJob {
+  SbDevice* sb = nullptr;
+  const double* psi = nullptr;
+  int sides = 3;
+  std::vector<std::vector<double>>* evals = nullptr;  // [2]
+  std::vector<std::vector<double>>* evecs = nullptr;  // [2], or nullptr with dvecs
+  std::vector<RhoVec>* dvecs = nullptr;               // [2], or nullptr
+};
+void sb_rdm_eigh_multi(::nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& rho, GemmPlan& R);
 struct GatherTask {  // dst[r*ncol + c] = src[r*si + cols[c]*sj]
   double* dst;
   const double* src;

@@ -480,8 +480,14 @@ static int rho_class(int d, int dlarge) {
   }();
   if (d <= 256) return (d + small - 1) / small * small;
   if (large > 0) return (d + large - 1) / large * large;
+  if (dlarge <= 0) return (d + 63) / 64 * 64;
   return (dlarge + 63) / 64 * 64;
 }
+
+// Bytes of one rocSOLVER call's padded slab: larger classes are split into
+// several calls (many chains' large blocks in one rho batch would otherwise
+// exhaust the device memory with slab and solver workspace).
+constexpr int64_t RHO_SLAB_BYTES = int64_t(1) << 30;
 
 // Auxiliary stream k with its rocblas handle bound to it (created on first use).
 static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
@@ -497,42 +503,46 @@ static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
   return static_cast<rocblas_handle>(ctx->aux_rocblas[k]);
 }
 
-// rho blocks of both sides in one batch: items (side, sector) share one
-// rho buffer, one GEMM launch, one symmetry check, one Jacobi launch and one
-// strided-batched rocSOLVER call per size class.
-void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
-                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
-                  std::vector<RhoVec>* dvecs) {
-  const int nq = (int)sb->Dl.size();
+// rho blocks of both sides of one or several superblocks in one batch:
+// items (job, side, sector) share one rho buffer, one GEMM launch, one
+// symmetry check, one Jacobi launch and one strided-batched rocSOLVER call
+// per size class.
+void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& rho, GemmPlan& R) {
   struct Item {
-    int side, q, d;
+    int job, side, q, d;
     int64_t roff, eoff;
     int kind;  // 1 jacobi, 2 rocsolver
   };
   std::vector<Item> items;
   int64_t tot = 0;
-  for (int side = 0; side < 2; ++side) {
-    evals[side].assign(nq, {});
-    evecs[side].assign(nq, {});
-    if (!(sides & (1 << side))) continue;
-    for (int q = 0; q < nq; ++q) {
-      if ((int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
-      const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
-      items.push_back({side, q, d, tot, -1, 1});
-      tot += (int64_t)d * d;
+  for (int jb = 0; jb < (int)jobs.size(); ++jb) {
+    RdmJob& J = jobs[jb];
+    const SbDevice* sb = J.sb;
+    const int nq = (int)sb->Dl.size();
+    for (int side = 0; side < 2; ++side) {
+      J.evals[side].assign(nq, {});
+      if (J.evecs) J.evecs[side].assign(nq, {});
+      if (!(J.sides & (1 << side))) continue;
+      for (int q = 0; q < nq; ++q) {
+        if ((int64_t)sb->Dl[q] * sb->Dr[q] == 0) continue;
+        const int d = side == 0 ? sb->Dl[q] : sb->Dr[q];
+        items.push_back({jb, side, q, d, tot, -1, 1});
+        tot += (int64_t)d * d;
+      }
     }
   }
   const int ni = (int)items.size();
-  sb->rho.resize(ctx->pool, std::max<int64_t>(1, tot));
+  rho.resize(ctx->pool, std::max<int64_t>(1, tot));
   PhaseTimer pt(ctx, "rdm.gemm");
-  sb->R.clear();
+  R.clear();
   for (const Item& it : items) {
+    const SbDevice* sb = jobs[it.job].sb;
     const int dl = sb->Dl[it.q], dr = sb->Dr[it.q];
     GemmSeg g{};
     g.alpha = 1.0;
-    g.abase = BASE_X;
-    g.bbase = BASE_X;
-    g.aoff = g.boff = sb->off[it.q];
+    g.abase = BASE_ABS;
+    g.bbase = BASE_ABS;
+    g.a = g.b = jobs[it.job].psi + sb->off[it.q];
     g.lda = g.ldb = dr;
     if (it.side == 0) {  // rho_L = Psi Psi^T : K = Dr
       g.K = dr;
@@ -541,11 +551,11 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
       g.K = dl;
       g.flags = SEG_TA;
     }
-    sb->R.add_output(BASE_Y, nullptr, it.roff, it.d, it.d, it.d, {g});
+    R.add_output(BASE_Y, nullptr, it.roff, it.d, it.d, it.d, {g});
   }
-  sb->R.upload(ctx->pool, ctx->stream);
-  Bases b{psi_dev, nullptr, sb->rho.p};
-  launch_gemm_f64(sb->R, b, ctx->stream, GEMM_GENERIC);
+  R.upload(ctx->pool, ctx->stream);
+  Bases b{nullptr, nullptr, rho.p};
+  launch_gemm_f64(R, b, ctx->stream, GEMM_GENERIC);
   // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the device
   pt.next("rdm.symcheck");
   std::vector<int64_t> hoff;
@@ -566,7 +576,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     dasym.resize(ctx->pool, nchk);
     NSS_HIP(hipMemcpyAsync(doff.p, hoff.data(), nchk * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     NSS_HIP(hipMemcpyAsync(ddim.p, hdim.data(), nchk * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    launch_asym_max(sb->rho.p, doff.p, ddim.p, nchk, dasym.p, ctx->stream);
+    launch_asym_max(rho.p, doff.p, ddim.p, nchk, dasym.p, ctx->stream);
     NSS_HIP(hipMemcpyAsync(asym.data(), dasym.p, nchk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
@@ -611,7 +621,7 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     NSS_HIP(hipMemcpyAsync(djoff.p, joff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     NSS_HIP(hipMemcpyAsync(djeoff.p, jeoff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     NSS_HIP(hipMemcpyAsync(djdim.p, jdim.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    launch_jacobi_eig(sb->rho.p, djoff.p, djdim.p, djeoff.p, sb->rho.p, dD.p, nb, jn, ctx->stream);
+    launch_jacobi_eig(rho.p, djoff.p, djdim.p, djeoff.p, rho.p, dD.p, nb, jn, ctx->stream);
   }
   // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
   // size class (dims rounded up to a multiple of 64, padded slabs).  The
@@ -624,8 +634,23 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   int dlarge = 0;  // largest block above the single-kernel range
   for (const Item& it : items)
     if (it.kind == 2 && it.d > 256) dlarge = std::max(dlarge, it.d);
+  {  // one class for all large blocks only while its padded slab stays modest
+    int64_t nlarge = 0;
+    for (const Item& it : items) nlarge += it.kind == 2 && it.d > 256;
+    const int64_t dl64 = (dlarge + 63) / 64 * 64;
+    if (nlarge * dl64 * dl64 * (int64_t)sizeof(double) > 2 * RHO_SLAB_BYTES) dlarge = 0;
+  }
   for (int i = 0; i < ni; ++i)
     if (items[i].kind == 2) classes[rho_class(items[i].d, dlarge)].push_back(i);
+  // calls: each class cut into groups whose slab fits RHO_SLAB_BYTES
+  std::vector<std::pair<int, std::vector<int>>> calls;
+  for (auto& kv : classes) {
+    const int64_t per = (int64_t)kv.first * kv.first * (int64_t)sizeof(double);
+    const size_t cap = (size_t)std::max<int64_t>(1, RHO_SLAB_BYTES / per);
+    for (size_t i0 = 0; i0 < kv.second.size(); i0 += cap)
+      calls.push_back({kv.first, std::vector<int>(kv.second.begin() + i0,
+                                                  kv.second.begin() + std::min(kv.second.size(), i0 + cap))});
+  }
   struct ClassBatch {
     int ncls = 0, nb = 0;
     hipStream_t st = nullptr;
@@ -635,11 +660,11 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     DevArr<int32_t> dcd, dci;
     DevArr<double> slab, sD, sE;
   };
-  std::vector<ClassBatch> batches(classes.size());
+  std::vector<ClassBatch> batches(calls.size());
   bool used_aux[nss_ctx::NAUX] = {};
   {
     size_t c = 0;
-    for (auto& kv : classes) {
+    for (auto& kv : calls) {
       ClassBatch& B = batches[c];
       const int k = (int)(c % nss_ctx::NAUX);
       rocblas_handle hnd = aux_handle(ctx, k, &B.st);
@@ -668,12 +693,14 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
       NSS_HIP(hipMemcpyAsync(B.dco.p, B.co.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, B.st));
       NSS_HIP(hipMemcpyAsync(B.dceo.p, B.ceo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, B.st));
       NSS_HIP(hipMemcpyAsync(B.dcd.p, B.cd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, B.st));
-      launch_pad_blocks(sb->rho.p, B.dco.p, B.dcd.p, nb, ncls, B.slab.p, B.st);
+      launch_pad_blocks(rho.p, B.dco.p, B.dcd.p, nb, ncls, B.slab.p, B.st);
       const rocblas_status st = rocsolver_dsyevd_strided_batched(
           hnd, rocblas_evect_original, rocblas_fill_upper, ncls, B.slab.p, ncls, (rocblas_stride)ncls * ncls, B.sD.p,
           ncls, B.sE.p, ncls, B.dci.p, nb);
-      if (st != rocblas_status_success) throw Error(NSS_E_HIP, "rocsolver_dsyevd_strided_batched failed");
-      launch_unpad_eig(B.slab.p, B.sD.p, ncls, B.dco.p, B.dcd.p, B.dceo.p, nb, sb->rho.p, dD.p, B.st);
+      if (st != rocblas_status_success)
+        throw Error(NSS_E_HIP, "rocsolver_dsyevd_strided_batched failed (rocblas status " + std::to_string((int)st) +
+                                   ", " + std::to_string(nb) + " blocks of " + std::to_string(ncls) + ")");
+      launch_unpad_eig(B.slab.p, B.sD.p, ncls, B.dco.p, B.dcd.p, B.dceo.p, nb, rho.p, dD.p, B.st);
       NSS_HIP(hipMemcpyAsync(B.ci.data(), B.dci.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, B.st));
     }
   }
@@ -687,27 +714,33 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
   if (etot) NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   // eigenvectors to the host only when asked for (host_vectors); otherwise
   // they stay in rho and are handed out as device views
+  bool host_vecs = false;
+  for (const RdmJob& J : jobs) host_vecs = host_vecs || !J.dvecs;
   std::vector<double> h;
-  if (!dvecs) {
+  if (host_vecs) {
     h.resize(std::max<int64_t>(1, tot));
-    if (tot) NSS_HIP(hipMemcpyAsync(h.data(), sb->rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (tot) NSS_HIP(hipMemcpyAsync(h.data(), rho.p, tot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   }
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   pt.next("rdm.host");
-  if (dvecs) {
-    dvecs[0].assign(nq, {});
-    dvecs[1].assign(nq, {});
-  }
+  for (RdmJob& J : jobs)
+    if (J.dvecs) {
+      const int nq = (int)J.sb->Dl.size();
+      J.dvecs[0].assign(nq, {});
+      J.dvecs[1].assign(nq, {});
+    }
   for (const Item& it : items) {
     const int d = it.d;
-    std::vector<double>& ev = evals[it.side][it.q];
+    RdmJob& J = jobs[it.job];
+    std::vector<RhoVec>* dvecs = J.dvecs;
+    std::vector<double>& ev = J.evals[it.side][it.q];
     const double* a = dvecs ? nullptr : h.data() + it.roff;
     ev.resize(d);
     RhoVec* rv = dvecs ? &dvecs[it.side][it.q] : nullptr;
-    std::vector<double>* vec = dvecs ? nullptr : &evecs[it.side][it.q];
+    std::vector<double>* vec = dvecs ? nullptr : &J.evecs[it.side][it.q];
     if (vec) vec->resize((size_t)d * d);
     if (rv) {
-      rv->base = sb->rho.p + it.roff;
+      rv->base = rho.p + it.roff;
       rv->d = d;
       rv->order.resize(d);
       for (int j = 0; j < d; ++j) rv->order[j] = j;
@@ -742,6 +775,19 @@ void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
     for (double& e : ev)
       if (e < 0.0) e = DBL_MIN;  // where(c%E<0d0)c%E=tiny(1d0), MATRIX_BLOCKS.f90:630
   }
+}
+
+void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
+                  std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
+                  std::vector<RhoVec>* dvecs) {
+  std::vector<RdmJob> jobs(1);
+  jobs[0].sb = sb;
+  jobs[0].psi = psi_dev;
+  jobs[0].sides = sides;
+  jobs[0].evals = evals;
+  jobs[0].evecs = evecs;
+  jobs[0].dvecs = dvecs;
+  sb_rdm_eigh_multi(ctx, jobs, sb->rho, sb->R);
 }
 
 void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
