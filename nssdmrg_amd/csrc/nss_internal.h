@@ -144,8 +144,11 @@ struct GemmPlan {
 constexpr int GEMM_BM = 64, GEMM_BN = 64;
 
 enum : int { GEMM_GENERIC = 0, GEMM_HV_W = 1, GEMM_HV_Y = 2 };
-void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind = GEMM_GENERIC);
-void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind);
+// ev0/ev1 (optional): recorded by the launch itself at its start and end.
+void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind = GEMM_GENERIC, hipEvent_t ev0 = nullptr,
+                     hipEvent_t ev1 = nullptr);
+void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind, hipEvent_t ev0 = nullptr,
+                     hipEvent_t ev1 = nullptr);
 
 // ------------------------------------------------------- vector kernels
 void launch_fill_splitmix(double* v, int64_t n, uint64_t seed, hipStream_t s);
@@ -256,8 +259,8 @@ struct nss_ctx {
   hipStream_t aux[NAUX] = {};
   void* aux_rocblas[NAUX] = {};
   nss::SbDevice* sb = nullptr;
-  // H*v timing with HIP events on the context stream: 3 events per call
-  // (start, after the W launch, after the Y launch).
+  // H*v timing with HIP events recorded by the W and Y launches themselves:
+  // 4 events per call (W start/end, Y start/end).
   bool time_hv = true;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;

@@ -12,6 +12,8 @@
 //  * k_kron: QN-blocked Kronecker scatter used by block enlargement.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "nss_internal.h"
 
 namespace nss {
@@ -406,24 +408,30 @@ NSS_GEMM_KERNEL(k_hv_y_f32, float)
 
 typedef void (*gemm_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases);
 
-static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t k32, gemm_kernel_t k64) {
+// With start/stop events the launch records them itself (the dispatch's own
+// begin and end timestamps, hipExtLaunchKernelGGL): no marker packets
+// between the kernels of an iteration.
+static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t k32, gemm_kernel_t k64,
+                        hipEvent_t ev0, hipEvent_t ev1) {
   const int n = (int)p.htiles.size();
   const unsigned grid = 8u * (unsigned)((n + 7) / 8);
-  hipLaunchKernelGGL(p.tile == 64 ? k64 : k32, dim3(grid), dim3(256), 0, s, p.tiles.p, p.segs.p, n, b);
+  gemm_kernel_t k = p.tile == 64 ? k64 : k32;
+  if (ev0) hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, ev0, ev1, 0, p.tiles.p, p.segs.p, n, b);
+  else hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, p.tiles.p, p.segs.p, n, b);
   NSS_HIP(hipGetLastError());
 }
 
-void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind) {
+void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind, hipEvent_t ev0, hipEvent_t ev1) {
   if (p.htiles.empty()) return;
-  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f64, k_hv_w_f64_t64);
-  else if (kind == GEMM_HV_Y) launch_plan(p, b, s, k_hv_y_f64, k_hv_y_f64_t64);
-  else launch_plan(p, b, s, k_gemm_f64, k_gemm_f64_t64);
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f64, k_hv_w_f64_t64, ev0, ev1);
+  else if (kind == GEMM_HV_Y) launch_plan(p, b, s, k_hv_y_f64, k_hv_y_f64_t64, ev0, ev1);
+  else launch_plan(p, b, s, k_gemm_f64, k_gemm_f64_t64, ev0, ev1);
 }
 
-void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind) {
+void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind, hipEvent_t ev0, hipEvent_t ev1) {
   if (p.htiles.empty()) return;
-  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f32, k_hv_w_f32_t64);
-  else launch_plan(p, b, s, k_hv_y_f32, k_hv_y_f32_t64);
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f32, k_hv_w_f32_t64, ev0, ev1);
+  else launch_plan(p, b, s, k_hv_y_f32, k_hv_y_f32_t64, ev0, ev1);
 }
 
 // ------------------------------------------------------------ vectors

@@ -304,38 +304,46 @@ static hipEvent_t next_event(nss_ctx* ctx) {
   return ctx->ev[ctx->ev_used++];
 }
 
-static void record(nss_ctx* ctx) { NSS_HIP(hipEventRecord(next_event(ctx), ctx->stream)); }
+
+// Timed H*v: each launch records its own start and end (4 events per call),
+// so the statistics are the kernels' durations and no marker packet sits
+// between them.
+static void hv_timed(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b, int kind) {
+  const bool t = ctx->time_hv;
+  hipEvent_t e[4] = {};
+  if (t)
+    for (hipEvent_t& x : e) x = next_event(ctx);
+  if (kind == 0) {
+    launch_gemm_f64(W, b, ctx->stream, GEMM_HV_W, e[0], e[1]);
+    launch_gemm_f64(Y, b, ctx->stream, GEMM_HV_Y, e[2], e[3]);
+  } else {
+    launch_gemm_f32(W, b, ctx->stream, GEMM_HV_W, e[0], e[1]);
+    launch_gemm_f32(Y, b, ctx->stream, GEMM_HV_Y, e[2], e[3]);
+  }
+  if (t) {
+    // an empty plan launches nothing: its events are recorded as markers
+    if (W.htiles.empty()) {
+      NSS_HIP(hipEventRecord(e[0], ctx->stream));
+      NSS_HIP(hipEventRecord(e[1], ctx->stream));
+    }
+    if (Y.htiles.empty()) {
+      NSS_HIP(hipEventRecord(e[2], ctx->stream));
+      NSS_HIP(hipEventRecord(e[3], ctx->stream));
+    }
+    ctx->ev_kind.push_back(kind);
+    ctx->ev_fw.push_back(W.flops);
+    ctx->ev_fy.push_back(Y.flops);
+  }
+}
 
 void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
-  Bases b{x, sb->W.p, y};
-  const bool t = ctx->time_hv;
-  if (t) record(ctx);
-  launch_gemm_f64(sb->W64, b, ctx->stream, GEMM_HV_W);
-  if (t) record(ctx);
-  launch_gemm_f64(sb->Y64, b, ctx->stream, GEMM_HV_Y);
-  if (t) {
-    record(ctx);
-    ctx->ev_kind.push_back(0);
-    ctx->ev_fw.push_back(sb->W64.flops);
-    ctx->ev_fy.push_back(sb->Y64.flops);
-  }
+  hv_timed(ctx, sb->W64, sb->Y64, Bases{x, sb->W.p, y}, 0);
 }
 
 void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
   NSS_REQUIRE(sb->has32, NSS_E_STATE, "FP32 H*v requested but blocks were uploaded for FP64 only");
   launch_d2f(x, sb->Xf.p, sb->n, ctx->stream);
-  Bases b{sb->Xf.p, sb->Wf.p, sb->Yf.p};
-  const bool t = ctx->time_hv;
-  if (t) record(ctx);
-  launch_gemm_f32(sb->W32, b, ctx->stream, GEMM_HV_W);
-  if (t) record(ctx);
-  launch_gemm_f32(sb->Y32, b, ctx->stream, GEMM_HV_Y);
-  if (t) {
-    record(ctx);
-    ctx->ev_kind.push_back(1);
-    ctx->ev_fw.push_back(sb->W32.flops);
-    ctx->ev_fy.push_back(sb->Y32.flops);
-  }
+  hv_timed(ctx, sb->W32, sb->Y32, Bases{sb->Xf.p, sb->Wf.p, sb->Yf.p}, 1);
   launch_f2d(sb->Yf.p, y, sb->n, ctx->stream);
 }
 
@@ -375,27 +383,15 @@ void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int6
 }
 
 // H*v of merged plans, timed like sb_hv.
-void hv_plans(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b) {
-  const bool t = ctx->time_hv;
-  if (t) record(ctx);
-  launch_gemm_f64(W, b, ctx->stream, GEMM_HV_W);
-  if (t) record(ctx);
-  launch_gemm_f64(Y, b, ctx->stream, GEMM_HV_Y);
-  if (t) {
-    record(ctx);
-    ctx->ev_kind.push_back(0);
-    ctx->ev_fw.push_back(W.flops);
-    ctx->ev_fy.push_back(Y.flops);
-  }
-}
+void hv_plans(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b) { hv_timed(ctx, W, Y, b, 0); }
 
 double nss_hv_timer_flush(nss_ctx* ctx) {
   NSS_HIP(hipStreamSynchronize(ctx->stream));
   double tot = 0.0;
   for (size_t c = 0; c < ctx->ev_kind.size(); ++c) {
     float a = 0.f, b = 0.f;
-    NSS_HIP(hipEventElapsedTime(&a, ctx->ev[3 * c], ctx->ev[3 * c + 1]));
-    NSS_HIP(hipEventElapsedTime(&b, ctx->ev[3 * c + 1], ctx->ev[3 * c + 2]));
+    NSS_HIP(hipEventElapsedTime(&a, ctx->ev[4 * c], ctx->ev[4 * c + 1]));
+    NSS_HIP(hipEventElapsedTime(&b, ctx->ev[4 * c + 2], ctx->ev[4 * c + 3]));
     const double w = a * 1e-3, y = b * 1e-3;
     const int k = ctx->ev_kind[c];
     tot += w + y;
@@ -845,6 +841,8 @@ extern "C" int nss_ctx_create(int device, nss_ctx** out) {
   c->sb = sb_create(c);
   const char* pe = std::getenv("NSS_PROFILE");
   c->prof = pe && pe[0] == '1';
+  const char* te = std::getenv("NSS_HV_TIMING");  // 0: no H*v events (the statistics stay empty)
+  c->time_hv = !(te && te[0] == '0');
   *out = c;
   NSS_API_END
 }
