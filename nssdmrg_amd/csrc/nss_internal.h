@@ -166,16 +166,22 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
                    int overwrite, hipStream_t s);
 // One classical Gram-Schmidt pass against V[0..m): h = V^T w (to hout,
 // device), w -= V h; with sq != nullptr, per-block sums of squares of the new
-// w are left in sq[0..orth_blocks(n)).  No host synchronisation.
+// w are left in sq[0..orth_blocks(n)).  With apart != nullptr the pass works
+// on w - alpha va instead of w (alpha = the fixed-order sum of the lz_nch(n)
+// partials in apart, stored to aout), writing that difference minus V h back
+// to w: the three-term step's last subtraction folded into the pass.  No
+// host synchronisation.
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
-                      double* hout, double* sq, hipStream_t s);
+                      double* hout, double* sq, hipStream_t s, const double* apart = nullptr,
+                      const double* va = nullptr, double* aout = nullptr);
 int orth_blocks(int64_t n);
 int64_t orth_chunk(int64_t n);
-// Lanczos three-term step on the device: w -= beta_prev v_prev (when
-// beta_prev != null), alpha = <v_j, w>, w -= alpha v_j; alpha to alpha_out,
-// beta_prev copied to beta_out.
+// First half of the Lanczos three-term step on the device: w -= beta_prev
+// v_prev (when beta_prev != null) and the lz_nch(n) partial sums of
+// alpha = <v_j, w> to apart; beta_prev copied to beta_out.  The subtraction
+// of alpha v_j happens in the following launch_orth_pass.
 void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int64_t n, const double* beta_prev,
-                          double* beta_out, double* alpha_out, double* partial, hipStream_t s);
+                          double* beta_out, double* apart, hipStream_t s);
 // v = w / beta with beta = sqrt(sum sq) written to bout (device).
 void launch_normalize(const double* w, double* v, int64_t n, const double* sq, double* bout, hipStream_t s);
 // Batched Lanczos over independent superblocks ("chains"): one launch per

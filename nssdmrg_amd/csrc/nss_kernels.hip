@@ -594,33 +594,32 @@ __global__ __launch_bounds__(256) void k_lz_alpha(double* __restrict__ w, const 
   }
 }
 
-// alpha = sum_c partial[c]; w -= alpha v_j; block 0 stores alpha.
-__global__ __launch_bounds__(256) void k_lz_sub(double* __restrict__ w, const double* __restrict__ vj, int64_t n,
-                                                const double* __restrict__ partial, int64_t nch,
-                                                double* __restrict__ alpha_out) {
-  __shared__ double sh[4];
-  __shared__ double ash;
-  const double a0 = coop_sum(partial, nch, sh);
-  if (threadIdx.x == 0) {
-    ash = a0;
-    if (blockIdx.x == 0) alpha_out[0] = a0;
-  }
-  __syncthreads();
-  const double a = ash;
-  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) w[i] -= a * vj[i];
-}
-
-// partial[j*nch + c] = <V_j, w> over chunk c
+// partial[j*nch + c] = <V_j, w - alpha v_a> over chunk c, alpha = the sum of
+// apart[0..nch) (every block reduces it in the same fixed order); with
+// apart == null, <V_j, w>
 __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv,
                                                   const double* __restrict__ w, int64_t n,
-                                                  double* __restrict__ partial, int64_t nch) {
+                                                  double* __restrict__ partial, int64_t nch,
+                                                  const double* __restrict__ apart,
+                                                  const double* __restrict__ va) {
   __shared__ double sh[4];
+  __shared__ double ash;
   const int64_t c = blockIdx.x;
   const int j = blockIdx.y;
+  double a = 0.0;
+  if (apart) {
+    const double a0 = coop_sum(apart, nch, sh);
+    if (threadIdx.x == 0) ash = a0;
+    __syncthreads();
+    a = ash;
+  }
   const double* v = V + (size_t)j * ldv;
   const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
   double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * w[i];
+  if (apart)
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i]);
+  else
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * w[i];
   const double r = block_sum256(acc, sh);
   if (threadIdx.x == 0) partial[(size_t)j * nch + c] = r;
 }
@@ -631,9 +630,22 @@ __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, 
 __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ V, int64_t ldv, int m,
                                                      const double* __restrict__ partial, int64_t nch,
                                                      double* __restrict__ h, double* __restrict__ w,
-                                                     int64_t n, double* __restrict__ sq) {
+                                                     int64_t n, double* __restrict__ sq,
+                                                     const double* __restrict__ apart,
+                                                     const double* __restrict__ va, double* __restrict__ aout) {
   extern __shared__ double hs[];
   __shared__ double red[4];
+  __shared__ double ash;
+  double a = 0.0;
+  if (apart) {  // alpha as k_lz_gemvt reduces it; w' = w - alpha v_a on the fly
+    const double a0 = coop_sum(apart, nch, red);
+    if (threadIdx.x == 0) {
+      ash = a0;
+      if (blockIdx.x == 0 && aout) aout[0] = a0;
+    }
+    __syncthreads();
+    a = ash;
+  }
   for (int j = threadIdx.x; j < m; j += 256) {
     const double* p = partial + (size_t)j * nch;
     double a0 = 0.0, a1 = 0.0;
@@ -658,7 +670,8 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
       a3 += V[(size_t)(j + 3) * ldv + i] * hs[j + 3];
     }
     for (; j < m; ++j) a0 += V[(size_t)j * ldv + i] * hs[j];
-    const double x = w[i] - ((a0 + a1) + (a2 + a3));
+    const double wi = apart ? w[i] - a * va[i] : w[i];
+    const double x = wi - ((a0 + a1) + (a2 + a3));
     w[i] = x;
     ss += x * x;
   }
@@ -685,20 +698,21 @@ __global__ __launch_bounds__(256) void k_normalize(const double* __restrict__ w,
 static unsigned wide_grid(int64_t n) { return (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256)); }
 
 void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int64_t n, const double* beta_prev,
-                          double* beta_out, double* alpha_out, double* partial, hipStream_t s) {
+                          double* beta_out, double* apart, hipStream_t s) {
   const int64_t nch = lz_nch(n);
-  hipLaunchKernelGGL(k_lz_alpha, dim3((unsigned)nch), dim3(256), 0, s, w, vprev, vj, n, beta_prev, partial,
+  hipLaunchKernelGGL(k_lz_alpha, dim3((unsigned)nch), dim3(256), 0, s, w, vprev, vj, n, beta_prev, apart,
                      beta_out);
-  hipLaunchKernelGGL(k_lz_sub, dim3(wide_grid(n)), dim3(256), 0, s, w, vj, n, partial, nch, alpha_out);
   NSS_HIP(hipGetLastError());
 }
 
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
-                      double* hout, double* sq, hipStream_t s) {
+                      double* hout, double* sq, hipStream_t s, const double* apart, const double* va,
+                      double* aout) {
   const int64_t nch = lz_nch(n);
-  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch);
+  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch,
+                     apart, va);
   hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
-                     ldv, m, partial, nch, hout, w, n, sq);
+                     ldv, m, partial, nch, hout, w, n, sq, apart, va, aout);
   NSS_HIP(hipGetLastError());
 }
 

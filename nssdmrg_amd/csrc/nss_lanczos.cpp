@@ -57,10 +57,11 @@ struct Lz {
     }
     double* slot = slots.p + (size_t)j * S;
     const bool has_prev = j > restart_at;
+    double* apart = part.p + (size_t)(kmax + 1) * lz_nch(n);  // alpha partials, past the pass's partials
     launch_lz_recurrence(w.p, has_prev ? vec(j - 1) : nullptr, vec(j), n,
                          has_prev ? slots.p + (size_t)(j - 1) * S + 2 * (kmax + 1) : nullptr,
-                         has_prev ? slot + (kmax + 1) + (j - 1) : nullptr, slot + (kmax + 1) + j, part.p, s);
-    launch_orth_pass(V.p, n, j + 1, w.p, n, part.p, slot, sq.p, s);
+                         has_prev ? slot + (kmax + 1) + (j - 1) : nullptr, apart, s);
+    launch_orth_pass(V.p, n, j + 1, w.p, n, part.p, slot, sq.p, s, apart, vec(j), slot + (kmax + 1) + j);
     launch_normalize(w.p, vec(j + 1), n, sq.p, slot + 2 * (kmax + 1), s);
   }
 };
@@ -250,7 +251,7 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   L.slots.resize(ctx->pool, (size_t)kmax * L.S);
   NSS_HIP(hipMemsetAsync(L.slots.p, 0, (size_t)kmax * L.S * sizeof(double), ctx->stream));
   const int64_t nch = (n + orth_chunk(n) - 1) / orth_chunk(n);
-  L.part.resize(ctx->pool, std::max<int64_t>((int64_t)(kmax + 1) * nch, gemv_t_chunks(n)) + 1);
+  L.part.resize(ctx->pool, std::max<int64_t>((int64_t)(kmax + 2) * nch, gemv_t_chunks(n)) + 1);
   L.sq.resize(ctx->pool, orth_blocks(n) + 1);
   L.scal.resize(ctx->pool, 4);
   DevArr<double> v0;
