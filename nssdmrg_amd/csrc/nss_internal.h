@@ -192,7 +192,10 @@ struct LzChainDev {
   double* w;        // H v output, then the orthogonalised residual
   double* x;        // H v input: copy of the newest basis vector
   double* slots;    // iteration j at slots + j * sstride (single-chain slot layout, width 2 kb1 + 1)
-  double* part;     // (kb1 + 1) x nch partial sums
+  double* part;     // (kb1 + 2) x nch partial sums
+  double* om;       // 3 x (kb1 + 1) orthogonality estimates: omega_{j,k} in row j % 3
+  int32_t* st;      // [0] full Gram-Schmidt pass this iteration, [1] second pass pending
+  double* bpre;     // |w| after the three-term step
   int64_t n;
   int64_t sstride;
   int32_t nch, blk0, active, kmax;
@@ -201,7 +204,10 @@ int64_t lz_nch(int64_t n);
 // One Lanczos iteration j after H v (w = H v_j), every active chain: the
 // three-term step, one classical Gram-Schmidt pass against V[0..j], and
 // v_{j+1} = w / |w| (also copied to x).  kb1 = batch kmax + 1.
-void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int j, int kb1, hipStream_t s);
+// full != 0: the Gram-Schmidt pass in every iteration; otherwise only when
+// the chain's orthogonality estimate asks for it (partial reorthogonalisation).
+void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int nchain, int j, int kb1,
+                          int full, hipStream_t s);
 // out[0] = sqrt(<x,x>)
 void launch_norm(const double* x, int64_t n, double* partial, double* out, hipStream_t s);
 // y[i] = x[i] * (1/den[0])

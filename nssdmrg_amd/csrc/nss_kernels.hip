@@ -772,15 +772,96 @@ __global__ __launch_bounds__(256) void k_lzb_sub(const LzChainDev* __restrict__ 
   const double a = ash;
   const double* vj = C.V + (size_t)j * C.n;
   const int64_t i0 = chunk * LZ_CHUNK, i1 = min<int64_t>(C.n, i0 + LZ_CHUNK);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) C.w[i] -= a * vj[i];
+  double ss = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double x = C.w[i] - a * vj[i];
+    C.w[i] = x;
+    ss += x * x;
+  }
+  const double r = block_sum256(ss, sh);
+  if (threadIdx.x == 0) C.part[(size_t)(kb1 + 1) * C.nch + chunk] = r;
+}
+
+// Partial reorthogonalisation (H. D. Simon, Math. Comp. 42 (1984) 115-142):
+// one block per chain propagates the estimates omega_{j+1,k} ~ v_{j+1}^T v_k
+// of the loss of orthogonality with the recurrence
+//   beta_j om_{j+1,k} = beta_k om_{j,k+1} + (alpha_k - alpha_j) om_{j,k}
+//                       + beta_{k-1} om_{j,k-1} - beta_{j-1} om_{j-1,k} +- 2 eps |T|
+// and asks for the full Gram-Schmidt pass when max_k |om_{j+1,k}| exceeds
+// sqrt(eps), then again in the next iteration; after a pass the estimates
+// restart at eps.  beta_j here is |w| after the three-term step.
+__global__ __launch_bounds__(256) void k_lzb_omega(const LzChainDev* __restrict__ ch, int j, int kb1) {
+  __shared__ double sh[4];
+  __shared__ double red[256];
+  __shared__ int now_s;
+  const LzChainDev C = ch[blockIdx.x];
+  if (!C.active) return;
+  const double bp0 = coop_sum(C.part + (size_t)(kb1 + 1) * C.nch, C.nch, sh);
+  __shared__ double bps;
+  if (threadIdx.x == 0) bps = sqrt(bp0);
+  __syncthreads();
+  const double bp = bps;
+  const double* sj = C.slots + (size_t)j * C.sstride;
+  const double aj = sj[kb1 + j];
+  double* cur = C.om + (size_t)(j % 3) * (kb1 + 1);
+  double* prv = C.om + (size_t)((j + 2) % 3) * (kb1 + 1);
+  double* nxt = C.om + (size_t)((j + 1) % 3) * (kb1 + 1);
+  const double bjm1 = j > 0 ? C.slots[(size_t)(j - 1) * C.sstride + 2 * kb1] : 0.0;
+  // |T| estimate: max_k |alpha_k| + beta_k over the iterations so far
+  double tn = fabs(aj) + bp;
+  for (int k = threadIdx.x; k < j; k += 256) {
+    const double* sk = C.slots + (size_t)k * C.sstride;
+    tn = fmax(tn, fabs(sk[kb1 + k]) + sk[2 * kb1]);
+  }
+  red[threadIdx.x] = tn;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  tn = red[0];
+  __syncthreads();
+  const double eps = 2.220446049250313e-16;
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < j; k += 256) {
+    const double* sk = C.slots + (size_t)k * C.sstride;
+    const double ak = sk[kb1 + k], bk = sk[2 * kb1];
+    double t = bk * cur[k + 1] + (ak - aj) * cur[k] - bjm1 * prv[k];
+    if (k > 0) t += C.slots[(size_t)(k - 1) * C.sstride + 2 * kb1] * cur[k - 1];
+    const double v = (t + copysign(2.0 * eps * tn, t)) / bp;
+    nxt[k] = v;
+    mx = fmax(mx, fabs(v));
+  }
+  const double psi = eps * sqrt((double)C.n);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mx = fmax(red[0], psi);
+    const int pend = C.st[1];
+    const int trig = mx > sqrt(eps);
+    const int now = pend || trig || !(bp > 0.0) || !isfinite(bp);
+    C.st[0] = now;
+    C.st[1] = (trig && !pend) ? 1 : 0;
+    C.bpre[0] = bp;
+    now_s = now;
+    nxt[j] = psi;
+    nxt[j + 1] = 1.0;
+  }
+  __syncthreads();
+  if (now_s)
+    for (int k = threadIdx.x; k <= j; k += 256) nxt[k] = eps;
 }
 
 // part[r * nch + chunk] = <V_r, w> over the chunk, r = blockIdx.y <= j
 __global__ __launch_bounds__(256) void k_lzb_gemvt(const LzChainDev* __restrict__ ch,
-                                                   const int32_t* __restrict__ bchain) {
+                                                   const int32_t* __restrict__ bchain, int full) {
   __shared__ double sh[4];
   const LzChainDev C = ch[bchain[blockIdx.x]];
-  if (!C.active) return;
+  if (!C.active || !(full || C.st[0])) return;
   const int64_t chunk = blockIdx.x - C.blk0;
   const int r = blockIdx.y;
   const double* v = C.V + (size_t)r * C.n;
@@ -795,11 +876,11 @@ __global__ __launch_bounds__(256) void k_lzb_gemvt(const LzChainDev* __restrict_
 // fixed order; chunk 0 publishes h); w -= V[0..j] h; the chunk's sum of
 // squares of the new w to part[kb1 * nch + chunk]
 __global__ __launch_bounds__(256) void k_lzb_update(const LzChainDev* __restrict__ ch,
-                                                    const int32_t* __restrict__ bchain, int j, int kb1) {
+                                                    const int32_t* __restrict__ bchain, int j, int kb1, int full) {
   extern __shared__ double hs[];
   __shared__ double red[4];
   const LzChainDev C = ch[bchain[blockIdx.x]];
-  if (!C.active) return;
+  if (!C.active || !(full || C.st[0])) return;
   const int64_t chunk = blockIdx.x - C.blk0;
   const int m = j + 1;
   for (int r = threadIdx.x; r < m; r += 256) {
@@ -838,15 +919,16 @@ __global__ __launch_bounds__(256) void k_lzb_update(const LzChainDev* __restrict
 // beta_j = sqrt(sum of the chain's squares); v_{j+1} = w / beta_j, also to
 // x (the next H*v input); chunk 0 stores beta_j
 __global__ __launch_bounds__(256) void k_lzb_normalize(const LzChainDev* __restrict__ ch,
-                                                       const int32_t* __restrict__ bchain, int j, int kb1) {
+                                                       const int32_t* __restrict__ bchain, int j, int kb1, int full) {
   __shared__ double sh[4];
   __shared__ double bsh;
   const LzChainDev C = ch[bchain[blockIdx.x]];
   if (!C.active) return;
   const int64_t chunk = blockIdx.x - C.blk0;
-  const double ssum = coop_sum(C.part + (size_t)kb1 * C.nch, C.nch, sh);
+  const bool pass = full || C.st[0];
+  const double ssum = pass ? coop_sum(C.part + (size_t)kb1 * C.nch, C.nch, sh) : 0.0;
   if (threadIdx.x == 0) {
-    bsh = sqrt(ssum);
+    bsh = pass ? sqrt(ssum) : C.bpre[0];
     if (chunk == 0) C.slots[(size_t)j * C.sstride + 2 * kb1] = bsh;
   }
   __syncthreads();
@@ -860,13 +942,16 @@ __global__ __launch_bounds__(256) void k_lzb_normalize(const LzChainDev* __restr
   }
 }
 
-void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int j, int kb1, hipStream_t s) {
+void launch_lzb_iteration(const LzChainDev* ch, const int32_t* bchain, int nblk, int nchain, int j, int kb1,
+                          int full, hipStream_t s) {
   if (nblk <= 0) return;
   hipLaunchKernelGGL(k_lzb_alpha, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
   hipLaunchKernelGGL(k_lzb_sub, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
-  hipLaunchKernelGGL(k_lzb_gemvt, dim3(nblk, j + 1), dim3(256), 0, s, ch, bchain);
-  hipLaunchKernelGGL(k_lzb_update, dim3(nblk), dim3(256), sizeof(double) * (size_t)(j + 1), s, ch, bchain, j, kb1);
-  hipLaunchKernelGGL(k_lzb_normalize, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1);
+  if (!full) hipLaunchKernelGGL(k_lzb_omega, dim3(nchain), dim3(256), 0, s, ch, j, kb1);
+  hipLaunchKernelGGL(k_lzb_gemvt, dim3(nblk, j + 1), dim3(256), 0, s, ch, bchain, full);
+  hipLaunchKernelGGL(k_lzb_update, dim3(nblk), dim3(256), sizeof(double) * (size_t)(j + 1), s, ch, bchain, j, kb1,
+                     full);
+  hipLaunchKernelGGL(k_lzb_normalize, dim3(nblk), dim3(256), 0, s, ch, bchain, j, kb1, full);
   NSS_HIP(hipGetLastError());
 }
 

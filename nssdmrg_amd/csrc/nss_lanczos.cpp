@@ -13,6 +13,7 @@
 // matrix in FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "nss_internal.h"
@@ -295,8 +296,20 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   return out;
 }
 
-// Lowest eigenpair of several independent superblocks at once: plain
-// Lanczos with full re-orthogonalisation, every chain in lockstep.  Each
+// Full Gram-Schmidt pass in every batched iteration (NSS_LZB_FULL_REORTH=1)
+// instead of partial reorthogonalisation.
+static int lzb_full_reorth() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_LZB_FULL_REORTH");
+    return e && e[0] == '1' ? 1 : 0;
+  }();
+  return v;
+}
+
+// Lowest eigenpair of several independent superblocks at once: Lanczos
+// with partial reorthogonalisation (the classical Gram-Schmidt pass against
+// the whole basis only when Simon's estimate of the loss of orthogonality
+// exceeds sqrt(eps), decided on the device), every chain in lockstep.  Each
 // iteration is one merged W launch and one merged Y launch over all active
 // chains (their vectors in shared X / Y arenas) and five batched vector
 // launches; the host reads every chain's new slots in one copy at the
@@ -333,11 +346,12 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     voff[c] = vtot;
     vtot += (int64_t)(kmc[c] + 1) * n[c];
     poff[c] = ptot;
-    ptot += (int64_t)(kb1 + 1) * nch[c];
+    ptot += (int64_t)(kb1 + 2) * nch[c];
     blk0[c] = btot;
     btot += nch[c];
   }
-  DevArr<double> X, Y, Wa, V, P, SL, scal, yd;
+  DevArr<double> X, Y, Wa, V, P, SL, scal, yd, OM, BP;
+  DevArr<int32_t> ST;
   X.resize(ctx->pool, xtot);
   Y.resize(ctx->pool, xtot);
   Wa.resize(ctx->pool, wtot);
@@ -347,6 +361,18 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
   scal.resize(ctx->pool, K);
   yd.resize(ctx->pool, kb1);
   NSS_HIP(hipMemsetAsync(SL.p, 0, (size_t)kb * K * S * sizeof(double), s));
+  // orthogonality estimates: omega_{0,0} = 1, everything else 0
+  OM.resize(ctx->pool, (size_t)K * 3 * kb1);
+  BP.resize(ctx->pool, K);
+  ST.resize(ctx->pool, 2 * K);
+  {
+    std::vector<double> om((size_t)K * 3 * kb1, 0.0);
+    for (int c = 0; c < K; ++c) om[(size_t)c * 3 * kb1] = 1.0;
+    NSS_HIP(hipMemcpyAsync(OM.p, om.data(), om.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    NSS_HIP(hipMemsetAsync(ST.p, 0, 2 * K * sizeof(int32_t), s));
+    NSS_HIP(hipStreamSynchronize(s));
+  }
+  const int full = lzb_full_reorth();
   GemmPlan Wp, Yp;
   sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx->pool, s);
   std::vector<LzChainDev> hc(K);
@@ -358,6 +384,9 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     C.x = X.p + xoff[c];
     C.slots = SL.p + (size_t)c * S;
     C.part = P.p + poff[c];
+    C.om = OM.p + (size_t)c * 3 * kb1;
+    C.st = ST.p + 2 * c;
+    C.bpre = BP.p + c;
     C.n = n[c];
     C.sstride = (int64_t)K * S;
     C.nch = nch[c];
@@ -405,7 +434,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
   std::vector<double> hs;
   while (nact > 0) {
     hv_plans(ctx, Wp, Yp, bases);
-    launch_lzb_iteration(dch.p, dmap.p, btot, j, kb1, s);
+    launch_lzb_iteration(dch.p, dmap.p, btot, K, j, kb1, full, s);
     ++j;
     bool at_end = false;
     for (int c = 0; c < K; ++c)
