@@ -109,6 +109,16 @@ struct Site {
   std::map<std::string, SiteOp> ops;
 };
 
+// Both rho sides solved and both blocks renormalized in every finite step
+// (NSS_FAR_RHO=1), also where the sweep replaces the far block next.
+static int full_far_rho() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_FAR_RHO");
+    return e && e[0] == '1' ? 1 : 0;
+  }();
+  return v;
+}
+
 static int fsign(const QN& q) { return (((q.a + q.b) / 2) & 1) ? -1 : 1; }  // (-1)^(Nup+Ndw)
 
 // -------------------------------------------------------------- sites
@@ -576,6 +586,25 @@ struct Engine {
     return s;
   }
 
+  // rho spectrum of the side that was not solved, per superblock sector q:
+  // rho_L,q = Psi_q Psi_q^T and rho_R,q = Psi_q^T Psi_q have the same
+  // nonzero eigenvalues, so the far block of dimension d_f takes the
+  // largest min(d_n, d_f) eigenvalues of the near one, below them
+  // d_f - min(d_n, d_f) null eigenvalues at tiny(1d0), the value the
+  // reference gives a rho eigenvalue below zero (MATRIX_BLOCKS.f90:630).
+  std::vector<std::vector<double>> far_spectrum(const std::vector<std::vector<double>>& near, int farside) const {
+    std::vector<std::vector<double>> f(near.size());
+    for (size_t q = 0; q < near.size(); ++q) {
+      if ((int64_t)lastDl[q] * lastDr[q] == 0) continue;
+      const int df = farside == 0 ? lastDl[q] : lastDr[q];
+      const int dn = (int)near[q].size();
+      const int k = std::min(dn, df);
+      f[q].assign(df, DBL_MIN);
+      std::copy(near[q].end() - k, near[q].end(), f[q].end() - k);
+    }
+    return f;
+  }
+
   static double entropy(const std::vector<double>& ev) {
     double S = 0.0;
     for (double l : ev) {
@@ -585,34 +614,34 @@ struct Engine {
     return S;
   }
 
-  // renormalize_block + operators_renormalization (src/DMRG_RDM.f90:172-404)
-  Block renormalize(const Block& E, const std::vector<int>& sbsec, const Spectrum& sp, int mstates,
-                    double estates, int& ms_out, double& trunc, double& cut,
-                    std::vector<std::pair<QN, std::vector<double>>>& kept) {
+  // The state selection of renormalize_block (src/DMRG_RDM.f90:195-251): the
+  // kept count m_s, the truncation error, the weight at the cut and the new
+  // sectors in order of first appearance over the descending spectrum (their
+  // QNs in N.sec, the enlarged sector and rho block of each).
+  struct Selection {
     int ms = 0;
-    double te = 0.0;
+    double trunc = 0.0, cut = 0.0;
+    std::vector<int> src_sector, blk_of_new;
+  };
+  Selection select_states(const Block& E, const std::vector<int>& sbsec, const Spectrum& sp, int mstates,
+                          double estates, Block& N, std::vector<std::pair<QN, std::vector<double>>>& kept) {
+    Selection R;
     NSS_REQUIRE(!sp.all_desc.empty(), NSS_E_STATE, "renormalize: empty rho spectrum");
     const int rc = nss_kept_count(sp.all_desc.data(), (int)sp.all_desc.size(), mstates, estates, E.dim(),
-                                  par.deg_thr, &ms, &te);
+                                  par.deg_thr, &R.ms, &R.trunc);
     if (rc != NSS_OK) throw Error(rc, nss_last_error());
-    ms = std::min<int>(ms, (int)sp.all_desc.size());
-    // new sectors: first appearance over the descending spectrum
+    const int ms = R.ms = std::min<int>(R.ms, (int)sp.all_desc.size());
     std::vector<int> newsec_of_blk(sp.ev.size(), -1);
-    std::vector<int> cnt(sp.ev.size(), 0);
-    Block N;
     N.length = E.length;
-    std::vector<int> src_sector;  // enlarged sector of each new sector
-    std::vector<int> blk_of_new;
     for (int im = 0; im < ms; ++im) {
       const int b = sp.order[im].first;
       if (newsec_of_blk[b] < 0) {
         newsec_of_blk[b] = (int)N.sec.size();
         N.sec.push_back({E.sec[sbsec[b]].q, 0});
-        src_sector.push_back(sbsec[b]);
-        blk_of_new.push_back(b);
+        R.src_sector.push_back(sbsec[b]);
+        R.blk_of_new.push_back(b);
       }
       N.sec[newsec_of_blk[b]].dim++;
-      cnt[b]++;
     }
     kept.clear();
     for (auto& s : N.sec) kept.push_back({s.q, {}});
@@ -620,7 +649,21 @@ struct Engine {
       const int b = sp.order[im].first;
       kept[newsec_of_blk[b]].second.push_back(sp.all_desc[im]);
     }
-    cut = sp.all_desc[ms - 1];
+    R.cut = sp.all_desc[ms - 1];
+    return R;
+  }
+
+  // renormalize_block + operators_renormalization (src/DMRG_RDM.f90:172-404)
+  Block renormalize(const Block& E, const std::vector<int>& sbsec, const Spectrum& sp, int mstates,
+                    double estates, int& ms_out, double& trunc, double& cut,
+                    std::vector<std::pair<QN, std::vector<double>>>& kept) {
+    Block N;
+    const Selection sel = select_states(E, sbsec, sp, mstates, estates, N, kept);
+    const int ms = sel.ms;
+    const double te = sel.trunc;
+    cut = sel.cut;
+    const std::vector<int>& src_sector = sel.src_sector;  // enlarged sector of each new sector
+    const std::vector<int>& blk_of_new = sel.blk_of_new;
     // U blocks (D x m, row-major): kept eigenvectors, descending eigenvalue
     std::vector<int64_t> uoff(N.sec.size());
     int64_t utot = 0;
@@ -1189,6 +1232,10 @@ struct Engine {
     int label = 0, sweep = 0, mstates = 0, neig = 1, v0 = 0;
     double estates = 0.0;
     bool dense = false;
+    // false in finite steps whose far block (right for label 1, left for
+    // label 2) the sweep replaces before it is used: that side's rho is then
+    // not solved and its block not renormalized
+    bool far = true;
     int64_t msb = 0;
     Step st;
     std::shared_ptr<Basis> fromL0, fromR0;
@@ -1200,11 +1247,12 @@ struct Engine {
     return std::chrono::duration<double>(b - a).count();
   }
 
-  StepState begin_step(char type, int label, int sweep) {
+  StepState begin_step(char type, int label, int sweep, bool far = true) {
     using clk = std::chrono::steady_clock;
     StepState S;
     S.t0 = clk::now();
     S.type = type;
+    S.far = far || type != 'f' || (label != 1 && label != 2) || full_far_rho();
     S.label = label;
     S.sweep = sweep;
     S.mstates = par.mdmrg;
@@ -1274,11 +1322,13 @@ struct Engine {
     std::vector<std::vector<double>> evs[2], unused[2];
     std::vector<RhoVec> dvs[2];
   };
-  RdmJob rdm_job(RdmOut& o) {
+  // sides of rho to solve (bit 0 left, bit 1 right)
+  static int rdm_sides(const StepState& S) { return S.far ? 3 : (S.label == 1 ? 1 : 2); }
+  RdmJob rdm_job(RdmOut& o, const StepState& S) {
     RdmJob J;
     J.sb = sbd();
     J.psi = psi.p;
-    J.sides = 3;
+    J.sides = rdm_sides(S);
     J.evals = o.evs;
     J.evecs = o.unused;
     J.dvecs = o.dvs;
@@ -1290,7 +1340,7 @@ struct Engine {
     RdmOut o;
     {
       PhaseTimer pt(ctx, "step.rdm");
-      sb_rdm_eigh2(ctx, sbd(), psi.p, 3, o.evs, o.unused, o.dvs);
+      sb_rdm_eigh2(ctx, sbd(), psi.p, rdm_sides(S), o.evs, o.unused, o.dvs);
     }
     finish_step(S, o);
   }
@@ -1306,6 +1356,8 @@ struct Engine {
     I.t_diag = secs(S.t2, S.t3);
     for (int i = 0; i < neig && i < 4; ++i) I.energies[i] = S.lo.evals[i];
     PhaseTimer pt(ctx, "step.spectrum");
+    const int farside = S.far ? -1 : (label == 1 ? 1 : 0);
+    if (farside >= 0) o.evs[farside] = far_spectrum(o.evs[1 - farside], farside);
     Spectrum sl = sort_spectrum(std::move(o.evs[0]), std::move(o.dvs[0]));
     Spectrum sr = sort_spectrum(std::move(o.evs[1]), std::move(o.dvs[1]));
     const auto t4 = clk::now();
@@ -1324,12 +1376,30 @@ struct Engine {
     if (ren) {
       int ml = 0, mr = 0;
       double tl = 0.0, tr = 0.0;
-      Block nl = renormalize(left, sbL, sl, S.mstates, S.estates, ml, tl, I.cut_left, st.kept_left);
-      Block nr = renormalize(right, sbR, sr, S.mstates, S.estates, mr, tr, I.cut_right, st.kept_right);
-      nl.parent = left.parent;
-      nr.parent = right.parent;
-      left = std::move(nl);
-      right = std::move(nr);
+      if (farside == 1) {  // right block replaced by the sweep: selection only
+        Block nr;
+        const Selection sel = select_states(right, sbR, sr, S.mstates, S.estates, nr, st.kept_right);
+        mr = sel.ms;
+        tr = sel.trunc;
+        I.cut_right = sel.cut;
+        right.from = identity_basis(right);
+      } else {
+        Block nr = renormalize(right, sbR, sr, S.mstates, S.estates, mr, tr, I.cut_right, st.kept_right);
+        nr.parent = right.parent;
+        right = std::move(nr);
+      }
+      if (farside == 0) {  // left block replaced by the sweep
+        Block nl;
+        const Selection sel = select_states(left, sbL, sl, S.mstates, S.estates, nl, st.kept_left);
+        ml = sel.ms;
+        tl = sel.trunc;
+        I.cut_left = sel.cut;
+        left.from = identity_basis(left);
+      } else {
+        Block nl = renormalize(left, sbL, sl, S.mstates, S.estates, ml, tl, I.cut_left, st.kept_left);
+        nl.parent = left.parent;
+        left = std::move(nl);
+      }
       I.m_left = ml;
       I.m_right = mr;
       I.trunc_left = tl;
@@ -1371,8 +1441,8 @@ struct Engine {
     steps.push_back(std::move(st));
   }
 
-  void step(char type, int label, int sweep) {
-    StepState S = begin_step(type, label, sweep);
+  void step(char type, int label, int sweep, bool far = true) {
+    StepState S = begin_step(type, label, sweep, far);
     diag(S);
     end_step(S);
   }
@@ -1412,19 +1482,21 @@ struct Engine {
     NSS_REQUIRE(finite_ready, NSS_E_STATE, "finite sweep before nss_dmrg_finite_begin");
     NSS_REQUIRE(im >= 0 && im < (int)msweep.size(), NSS_E_ARG, "sweep index outside MSWEEP");
     const int fn = 2 * par.ldmrg;
+    // the far block of a step is needed only by the last step of each loop
+    // (the next loop enlarges it; after the last it is the final state)
     while (left.length < fn) {
       right = bl2.at(fn - left.length);
-      step('f', 1, im);
+      step('f', 1, im, left.length + 1 >= fn);
       bl1[left.length] = left;
     }
     while (right.length < fn) {
       left = bl1.at(fn - right.length);
-      step('f', 2, im);
+      step('f', 2, im, right.length + 1 >= fn);
       bl2[right.length] = right;
     }
     while (left.length <= par.ldmrg) {
       right = bl2.at(fn - left.length);
-      step('f', 1, im);
+      step('f', 1, im, left.length + 1 > par.ldmrg);
       bl1[left.length] = left;
     }
   }
@@ -1445,10 +1517,10 @@ struct Batch {
   nss_ctx* ctx = nullptr;
   std::vector<Engine*> es;
 
-  void step(char type, int label, int sweep) {
+  void step(char type, int label, int sweep, bool far = true) {
     std::vector<Engine::StepState> S;
     S.reserve(es.size());
-    for (Engine* e : es) S.push_back(e->begin_step(type, label, sweep));
+    for (Engine* e : es) S.push_back(e->begin_step(type, label, sweep, far));
     std::vector<LanczosJob> jobs;
     std::vector<size_t> who;
     for (size_t i = 0; i < es.size(); ++i) {
@@ -1482,7 +1554,7 @@ struct Batch {
     std::vector<RdmJob> rj;
     for (size_t i = 0; i < es.size(); ++i) {
       S[i].t3 = t3;
-      rj.push_back(es[i]->rdm_job(ro[i]));
+      rj.push_back(es[i]->rdm_job(ro[i], S[i]));
     }
     DevArr<double> rho;
     GemmPlan R;
@@ -1536,17 +1608,17 @@ struct Batch {
     const int fn = 2 * e0->par.ldmrg;
     while (e0->left.length < fn) {
       for (Engine* e : es) e->right = e->bl2.at(fn - e->left.length);
-      step('f', 1, im);
+      step('f', 1, im, e0->left.length + 1 >= fn);
       for (Engine* e : es) e->bl1[e->left.length] = e->left;
     }
     while (e0->right.length < fn) {
       for (Engine* e : es) e->left = e->bl1.at(fn - e->right.length);
-      step('f', 2, im);
+      step('f', 2, im, e0->right.length + 1 >= fn);
       for (Engine* e : es) e->bl2[e->right.length] = e->right;
     }
     while (e0->left.length <= e0->par.ldmrg) {
       for (Engine* e : es) e->right = e->bl2.at(fn - e->left.length);
-      step('f', 1, im);
+      step('f', 1, im, e0->left.length + 1 > e0->par.ldmrg);
       for (Engine* e : es) e->bl1[e->left.length] = e->left;
     }
   }

@@ -1055,14 +1055,18 @@ __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __rest
   // rotations are skipped, and the sweeps stop, once every off-diagonal
   // element is below 1e-15 ||A||_F: a few ulps of ||A||, above the rounding
   // noise that rotations leave behind, so the absolute accuracy of the
-  // eigenvalues is ~eps*||A|| as LAPACK gives for a PSD matrix
+  // eigenvalues is ~eps*||A|| as LAPACK gives for a PSD matrix.  The
+  // two triangles are updated separately and drift apart by rounding, so
+  // both tests use the symmetric part (A_pq + A_qp)/2: a test on one
+  // triangle and a rotation decided on the other would leave an element
+  // just above the threshold for ever.
   const double thr = 1e-15 * sqrt(red[0]);
   __syncthreads();
   for (int sw = 0; sw < max_sweeps; ++sw) {
     double o = 0.0;
     for (int i = wave; i < np; i += nwave)
       for (int j = lane; j < np; j += 64)
-        if (i != j) o = fmax(o, fabs(A[i * ld + j]));
+        if (i != j) o = fmax(o, fabs(0.5 * (A[i * ld + j] + A[j * ld + i])));
     if (!(block_max(o, red) > thr)) break;
     for (int r = 0; r < np - 1; ++r) {
       if ((int)threadIdx.x < half) {
@@ -1072,7 +1076,7 @@ __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __rest
         int q = ((k1 - 1 + r) % (np - 1)) + 1;
         if (p > q) { const int t = p; p = q; q = t; }
         double c = 1.0, sn = 0.0;
-        const double apq = A[p * ld + q];
+        const double apq = 0.5 * (A[p * ld + q] + A[q * ld + p]);
         if (q < n && fabs(apq) > thr) {
           const double tau = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
           const double t = (tau >= 0.0) ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));

@@ -690,18 +690,24 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
       NSS_HIP(hipMemcpyAsync(B.dceo.p, B.ceo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, B.st));
       NSS_HIP(hipMemcpyAsync(B.dcd.p, B.cd.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, B.st));
       launch_pad_blocks(rho.p, B.dco.p, B.dcd.p, nb, ncls, B.slab.p, B.st);
+      const double te0 = ctx->prof ? now_s() : 0.0;
       const rocblas_status st = rocsolver_dsyevd_strided_batched(
           hnd, rocblas_evect_original, rocblas_fill_upper, ncls, B.slab.p, ncls, (rocblas_stride)ncls * ncls, B.sD.p,
           ncls, B.sE.p, ncls, B.dci.p, nb);
+      if (ctx->prof) ctx->timers["rdm.rocsolver_enqueue"] += now_s() - te0;
       if (st != rocblas_status_success)
         throw Error(NSS_E_HIP, "rocsolver_dsyevd_strided_batched failed (rocblas status " + std::to_string((int)st) +
                                    ", " + std::to_string(nb) + " blocks of " + std::to_string(ncls) + ")");
       launch_unpad_eig(B.slab.p, B.sD.p, ncls, B.dco.p, B.dcd.p, B.dceo.p, nb, rho.p, dD.p, B.st);
-      NSS_HIP(hipMemcpyAsync(B.ci.data(), B.dci.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, B.st));
     }
   }
+  // every class is enqueued before the host waits on any of them
+  const double tw0 = ctx->prof ? now_s() : 0.0;
   for (int k = 0; k < nss_ctx::NAUX; ++k)
     if (used_aux[k]) NSS_HIP(hipStreamSynchronize(ctx->aux[k]));
+  if (ctx->prof) ctx->timers["rdm.rocsolver_wait"] += now_s() - tw0;
+  for (ClassBatch& B : batches)
+    NSS_HIP(hipMemcpy(B.ci.data(), B.dci.p, B.nb * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (const ClassBatch& B : batches)
     for (int c = 0; c < B.nb; ++c)
       if (B.ci[c] != 0) throw Error(NSS_E_NOCONV, "rocsolver_dsyevd did not converge on a rho block");
