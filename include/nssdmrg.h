@@ -187,6 +187,14 @@ typedef struct {
    * Converged energies, entropies and kept counts agree to the Lanczos
    * tolerance either way. */
   int32_t predict;
+  /* PBCdmrg: periodic boundary conditions (src/INPUT_VARS.f90:171-183).  An
+   * even ldmrg is shifted to the next odd value; every infinite iteration is
+   * a link-'n' step followed by a link-'p' step that adds the sites at the
+   * blocks' outer ends (src/DMRG_MAIN.f90:155-158), and the superblock
+   * carries the connecting terms of both links.  The finite algorithm and
+   * nss_dmrg_measure refuse PBC runs, as the reference's finite_DMRG does
+   * (src/DMRG_MAIN.f90:179-180). */
+  int32_t pbc;
 } nss_params;
 
 typedef struct {
@@ -206,6 +214,7 @@ typedef struct {
   int32_t numop_fp32;
   double hv_flops;            /* algorithmic FLOPs of one H*v */
   double t_step, t_diag, t_hv, t_rdm, t_renorm, t_enlarge;  /* seconds */
+  int32_t link;               /* 'n', or 'p' for the PBC_prev step of a PBC iteration */
 } nss_step_info;
 
 int nss_dmrg_create(nss_ctx* ctx, const nss_model* model, const nss_params* params,

@@ -309,6 +309,7 @@ class DMRGConfig:
     precision: str = "fp64"
     verbose: int = 0
     predict: bool = False   # start finite-step Lanczos from the transformed previous ground state
+    pbc: bool = False       # PBCdmrg: periodic ring (infinite algorithm; an even ldmrg becomes odd)
 
 
 class DMRG:
@@ -341,6 +342,7 @@ class DMRG:
         p.precision = F.NSS_FP32_MIXED if cfg.precision in ("fp32", "mixed") else F.NSS_FP64
         p.verbose = cfg.verbose
         p.predict = int(bool(cfg.predict))
+        p.pbc = int(bool(cfg.pbc))
         h = C.c_void_p()
         F.check(F.lib().nss_dmrg_create(ctx.handle, C.byref(m), C.byref(p), C.byref(h)))
         self._h = h
@@ -365,6 +367,7 @@ class DMRG:
             F.check(F.lib().nss_dmrg_step_info(self._h, i, C.byref(s)))
             d = {k: getattr(s, k) for k, _ in F.StepInfo._fields_}
             d["type"] = chr(s.type)
+            d["link"] = chr(s.link) if s.link else "n"
             d["energies"] = np.array(s.energies[:s.neigen])
             d["e_per_site"] = d["energies"] / s.L
             out.append(d)
@@ -478,7 +481,8 @@ def init_dmrg(Hij, ModelDot, cfg: dict | None = None, ctx: Context | None = None
                       uloc=float(uloc), hfmode=bool(c["HFMODE"]), eloc=(float(mh[0]), float(mh[0])),
                       neigen=int(c["LANC_NEIGEN"]), lanc_dim_threshold=int(c["LANC_DIM_THRESHOLD"]),
                       lanc_maxiter=int(c["LANC_NITER"]), lanc_tol=float(c["LANC_TOLERANCE"]),
-                      v0_mode=1 if c["LANC_V0_DBLE"] else 0, deg_thr=float(c["DEG_EVALS_THRESHOLD"]))
+                      v0_mode=1 if c["LANC_V0_DBLE"] else 0, deg_thr=float(c["DEG_EVALS_THRESHOLD"]),
+                      pbc=bool(c["PBCDMRG"]))
     for k, v in over.items():
         setattr(conf, k, v)
     if c["DMRGTYPE"].lower().startswith("i"):

@@ -40,7 +40,8 @@ class Params(C.Structure):
                 ("neigen", C.c_int32), ("lanc_dim_threshold", C.c_int32),
                 ("lanc_kmax", C.c_int32), ("lanc_maxiter", C.c_int32), ("lanc_tol", C.c_double),
                 ("seed", C.c_uint64), ("v0_mode", C.c_int32), ("deg_thr", C.c_double),
-                ("precision", C.c_int32), ("verbose", C.c_int32), ("predict", C.c_int32)]
+                ("precision", C.c_int32), ("verbose", C.c_int32), ("predict", C.c_int32),
+                ("pbc", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -53,7 +54,8 @@ class StepInfo(C.Structure):
                 ("cut_left", C.c_double), ("cut_right", C.c_double), ("m_sb", C.c_int64), ("nsect", C.c_int32), ("nterms", C.c_int32),
                 ("numop", C.c_int32), ("numop_fp32", C.c_int32), ("hv_flops", C.c_double),
                 ("t_step", C.c_double), ("t_diag", C.c_double), ("t_hv", C.c_double),
-                ("t_rdm", C.c_double), ("t_renorm", C.c_double), ("t_enlarge", C.c_double)]
+                ("t_rdm", C.c_double), ("t_renorm", C.c_double), ("t_enlarge", C.c_double),
+                ("link", C.c_int32)]
 
 
 _P = C.c_void_p

@@ -371,7 +371,12 @@ struct Engine {
     NSS_HIP(hipStreamSynchronize(ctx->stream));
   }
 
-  Block enlarge(const Block& b, bool is_left) {
+  // Link 'n' (OBC and PBC_next): the dot joins the block's inner end; link
+  // 'p' (PBC_prev): its outer end.  Operators of the outer end carry the
+  // suffix _p.  Stored bases are always block sector outer, site state inner;
+  // the reference's Kronecker order (block first for left 'n' and right 'p',
+  // dot first otherwise, :251-263) only decides where the signs go.
+  Block enlarge(const Block& b, bool is_left, char link = 'n') {
     Block e;
     e.length = b.length + 1;
     std::vector<std::vector<Piece>> pieces;
@@ -390,41 +395,52 @@ struct Engine {
       return C;
     };
     const QN z{0, 0};
+    const bool dot_last = is_left == (link == 'n');  // the reference's block-first Kronecker order
+    const std::string x = link == 'n' ? "" : "_p";   // operators of the end the dot joins
     std::map<std::string, std::vector<KTerm>> defs;
     std::vector<KTerm>& H = defs["H"];
     H.push_back({b.ops.at("H").get(), 0, 0, 0, I, z, 1.0});
     H.push_back({nullptr, 0, 0, 0, sop("H").m, z, 1.0});
     if (!site.fermion) {
-      const BlockOp* Sz = b.ops.at("Sz").get();
-      const BlockOp* Sp = b.ops.at("Sp").get();
+      const BlockOp* Sz = b.ops.at("Sz" + x).get();
+      const BlockOp* Sp = b.ops.at("Sp" + x).get();
       const double jz = hij[0], jxy = hij[3];
-      const QN up{2, 0}, dn{-2, 0};
-      // connect_spin_blocks :396-454 (symmetric under left <-> right)
+      // connect_spin_blocks :396-454 (symmetric under the exchange of its arguments)
       H.push_back({Sz, 0, 0, 0, sop("Sz").m, z, jz});
       H.push_back({Sp, 0, 0, 0, transpose(sop("Sp").m, d), z, jxy});
       H.push_back({Sp, 1, 0, 0, sop("Sp").m, z, jxy});
-      (void)up; (void)dn;
-      defs["Sz"].push_back({nullptr, 0, 0, 0, sop("Sz").m, z, 1.0});
-      defs["Sp"].push_back({nullptr, 0, 0, 0, sop("Sp").m, sop("Sp").dq, 1.0});
     } else {
       const char* names[2] = {"Cup", "Cdn"};
       for (int sp = 0; sp < 2; ++sp) {
         const double t = hij[sp * 2 + sp];
-        const BlockOp* C = b.ops.at(names[sp]).get();
+        const BlockOp* C = b.ops.at(names[sp] + x).get();
         const std::vector<double>& Cs = sop(names[sp]).m;
-        if (t != 0.0) {
-          if (is_left) {
-            // (C_b^T P_b) x C_s  +  (P_b C_b) x C_s^T      (connect(self,dot))
-            H.push_back({C, 1, 0, 1, Cs, z, t});
-            H.push_back({C, 0, 1, 0, transpose(Cs, d), z, t});
-          } else {
-            // C_b x (C_s^T P_s)  +  C_b^T x (P_s C_s)      (connect(dot,self))
-            H.push_back({C, 0, 0, 0, mul(transpose(Cs, d), Pd), z, t});
-            H.push_back({C, 1, 0, 0, mul(Pd, Cs), z, t});
-          }
+        if (t == 0.0) continue;
+        if (dot_last) {
+          // (C_b^T P_b) x C_s  +  (P_b C_b) x C_s^T      (connect(self,dot))
+          H.push_back({C, 1, 0, 1, Cs, z, t});
+          H.push_back({C, 0, 1, 0, transpose(Cs, d), z, t});
+        } else {
+          // C_b x (C_s^T P_s)  +  C_b^T x (P_s C_s)      (connect(dot,self))
+          H.push_back({C, 0, 0, 0, mul(transpose(Cs, d), Pd), z, t});
+          H.push_back({C, 1, 0, 0, mul(Pd, Cs), z, t});
         }
-        // fermionic edge operator: left P_b x C_s, right C_s x 1_b
-        defs[names[sp]].push_back({nullptr, 0, is_left ? 1 : 0, 0, Cs, sop(names[sp]).dq, 1.0});
+      }
+    }
+    // the other operators (:116-237; P is the sector sign, kept implicit)
+    for (const auto& kv : b.ops) {
+      const std::string& k = kv.first;
+      if (k == "H") continue;
+      const bool kp = k.size() > 2 && k.compare(k.size() - 2, 2, "_p") == 0;
+      const SiteOp& so = sop(k.c_str());
+      if (kp == (link == 'p')) {
+        // the end the dot joins: the dot's operator, behind the block's sign
+        // where the block comes first (fermions)
+        defs[k].push_back({nullptr, 0, (dot_last && site.fermion) ? 1 : 0, 0, so.m, so.dq, 1.0});
+      } else {
+        // the other end keeps the block's operator, behind the dot's sign
+        // where the dot comes first (fermions)
+        defs[k].push_back({kv.second.get(), 0, 0, 0, (!dot_last && site.fermion) ? Pd : I, so.dq, 1.0});
       }
     }
     // tasks
@@ -503,30 +519,36 @@ struct Engine {
       HR[k] = ref(hr, *b, Dr[k], 0, 1.0);
     }
     std::vector<SbTerm> terms;
+    // links of the connecting terms: 'n' (the inner ends), and with PBC 'p'
+    // (the outer ends, suffix _p), in this order within each sector
+    std::vector<std::string> links{""};
+    if (par.pbc) links.push_back("_p");
     if (!site.fermion) {
-      const BlockOp& SzL = *L.ops.at("Sz");
-      const BlockOp& SzR = *R.ops.at("Sz");
-      const BlockOp& SpL = *L.ops.at("Sp");
-      const BlockOp& SpR = *R.ops.at("Sp");
       const double jz = hij[0], jxy = hij[3];
       for (int k = 0; k < nq; ++k) {
         const int lk = sbL[k], rk = sbR[k];
-        if (jz != 0.0) {  // it=1, SETUP:289-299
-          const OpBlock* a = SzL.row(lk);
-          const OpBlock* b = SzR.row(rk);
-          if (a && b) terms.push_back({k, k, ref(SzL, *a, Dl[k], 0, jz), ref(SzR, *b, Dr[k], 1, 1.0)});
+        for (const std::string& x : links) {
+          const BlockOp& SzL = *L.ops.at("Sz" + x);
+          const BlockOp& SzR = *R.ops.at("Sz" + x);
+          const BlockOp& SpL = *L.ops.at("Sp" + x);
+          const BlockOp& SpR = *R.ops.at("Sp" + x);
+          if (jz != 0.0) {  // it=1 (it=4 for 'p'), SETUP:289-299
+            const OpBlock* a = SzL.row(lk);
+            const OpBlock* b = SzR.row(rk);
+            if (a && b) terms.push_back({k, k, ref(SzL, *a, Dl[k], 0, jz), ref(SzR, *b, Dr[k], 1, 1.0)});
+          }
+          if (jxy == 0.0) continue;
+          auto it = index.find(L.sec[lk].q - QN{2, 0});
+          if (it == index.end()) continue;
+          const int j = it->second, lj = sbL[j], rj = sbR[j];
+          const OpBlock* a = SpL.row(lk);   // (lk, lj)
+          const OpBlock* b = SpR.row(rj);   // (rj, rk)
+          if (!a || a->cs != lj || !b || b->cs != rk) continue;
+          // it=2: Y_k += Jxy Sp_L(lk,lj) Psi_j Sp_R(rj,rk)        SETUP:306-312
+          terms.push_back({k, j, ref(SpL, *a, Dl[j], 0, jxy), ref(SpR, *b, Dr[k], 0, 1.0)});
+          // it=3: hconjg partner, Y_j += A2^T Psi_k B2^T^T        SETUP:315-321
+          terms.push_back({j, k, ref(SpL, *a, Dl[j], 1, jxy), ref(SpR, *b, Dr[k], 1, 1.0)});
         }
-        if (jxy == 0.0) continue;
-        auto it = index.find(L.sec[lk].q - QN{2, 0});
-        if (it == index.end()) continue;
-        const int j = it->second, lj = sbL[j], rj = sbR[j];
-        const OpBlock* a = SpL.row(lk);   // (lk, lj)
-        const OpBlock* b = SpR.row(rj);   // (rj, rk)
-        if (!a || a->cs != lj || !b || b->cs != rk) continue;
-        // it=2: Y_k += Jxy Sp_L(lk,lj) Psi_j Sp_R(rj,rk)        SETUP:306-312
-        terms.push_back({k, j, ref(SpL, *a, Dl[j], 0, jxy), ref(SpR, *b, Dr[k], 0, 1.0)});
-        // it=3: hconjg partner, Y_j += A2^T Psi_k B2^T^T        SETUP:315-321
-        terms.push_back({j, k, ref(SpL, *a, Dl[j], 1, jxy), ref(SpR, *b, Dr[k], 1, 1.0)});
       }
     } else {
       const char* names[2] = {"Cup", "Cdn"};
@@ -539,16 +561,18 @@ struct Engine {
           auto it = index.find(L.sec[lk].q - dqs[sp]);
           if (it == index.end()) continue;
           const int j = it->second, lj = sbL[j], rj = sbR[j];
-          const BlockOp& CL = *L.ops.at(names[sp]);
-          const BlockOp& CR = *R.ops.at(names[sp]);
-          const OpBlock* a = CL.row(lj);  // C_L(lj, lk)
-          const OpBlock* b = CR.row(rk);  // C_R(rk, rj)
-          if (!a || a->cs != lk || !b || b->cs != rj) continue;
-          const double alpha = t * fsign(L.sec[lj].q);
-          // it=1: A = t (C_L^T P_L)(lk,lj), B = C_R(rk,rj)       SETUP:601-610
-          terms.push_back({k, j, ref(CL, *a, Dl[k], 1, alpha), ref(CR, *b, Dr[j], 1, 1.0)});
-          // it=2: hconjg partner                                  SETUP:612-619
-          terms.push_back({j, k, ref(CL, *a, Dl[k], 0, alpha), ref(CR, *b, Dr[j], 0, 1.0)});
+          for (const std::string& x : links) {
+            const BlockOp& CL = *L.ops.at(names[sp] + x);
+            const BlockOp& CR = *R.ops.at(names[sp] + x);
+            const OpBlock* a = CL.row(lj);  // C_L(lj, lk)
+            const OpBlock* b = CR.row(rk);  // C_R(rk, rj)
+            if (!a || a->cs != lk || !b || b->cs != rj) continue;
+            const double alpha = t * fsign(L.sec[lj].q);
+            // it=1 (it=3 for 'p'): A = t (C_L^T P_L)(lk,lj), B = C_R(rk,rj)   SETUP:601-610
+            terms.push_back({k, j, ref(CL, *a, Dl[k], 1, alpha), ref(CR, *b, Dr[j], 1, 1.0)});
+            // it=2 (it=4): hconjg partner                                     SETUP:612-619
+            terms.push_back({j, k, ref(CL, *a, Dl[k], 0, alpha), ref(CR, *b, Dr[j], 0, 1.0)});
+          }
         }
       }
     }
@@ -995,10 +1019,17 @@ struct Engine {
       }
     }
     for (long st = 0; st < dim; ++st) NSS_REQUIRE(pos[st].first >= 0, NSS_E_ARG, "read_block: state in no sector");
-    static const std::map<std::string, std::string> keymap = {
+    // link-'p' keys (okey's _p suffix, src/AUX_FUNCS.f90:108-112) are read
+    // only by a PBC run
+    static const std::map<std::string, std::string> keymap_obc = {
         {"H", "H"},         {"S_z", "Sz"},        {"S_z_n", "Sz"},      {"S_p", "Sp"},
         {"S_p_n", "Sp"},    {"C_l1_s1", "Cup"},   {"C_l1_s1_n", "Cup"}, {"C_l1_s2", "Cdn"},
         {"C_l1_s2_n", "Cdn"}};
+    std::map<std::string, std::string> keymap = keymap_obc;
+    if (par.pbc)
+      for (const auto& kv : std::map<std::string, std::string>{
+               {"S_z_p", "Sz_p"}, {"S_p_p", "Sp_p"}, {"C_l1_s1_p", "Cup_p"}, {"C_l1_s2_p", "Cdn_p"}})
+        keymap.insert(kv);
     const long nops = tk.inum();
     for (long o = 0; o < nops; ++o) {
       const std::string key = tk.next();
@@ -1033,9 +1064,8 @@ struct Engine {
       make_fp32(*op);
       b.ops[it->second] = op;
     }
-    for (const char* need : site.fermion ? std::initializer_list<const char*>{"H", "Cup", "Cdn"}
-                                         : std::initializer_list<const char*>{"H", "Sz", "Sp"})
-      NSS_REQUIRE(b.ops.count(need), NSS_E_ARG, (std::string("read_block: operator missing: ") + need).c_str());
+    for (const auto& kv : site.ops)
+      NSS_REQUIRE(b.ops.count(kv.first), NSS_E_ARG, ("read_block: operator missing: " + kv.first).c_str());
     NSS_HIP(hipStreamSynchronize(ctx->stream));
     return b;  // rotation matrices are not needed to continue a run (measurement needs the history)
   }
@@ -1061,6 +1091,13 @@ struct Engine {
         ? std::vector<W>{{"H", "bosonic", "H"}, {"C_l1_s1_n", "fermionic", "Cup"}, {"C_l1_s2_n", "fermionic", "Cdn"},
                          {"P_n", "psign", ""}}
         : std::vector<W>{{"H", "bosonic", "H"}, {"S_z_n", "bosonic", "Sz"}, {"S_p_n", "bosonic", "Sp"}};
+    if (par.pbc) {
+      if (site.fermion)
+        ws.insert(ws.end(), {{"C_l1_s1_p", "fermionic", "Cup_p"}, {"C_l1_s2_p", "fermionic", "Cdn_p"},
+                             {"P_p", "psign", ""}});
+      else
+        ws.insert(ws.end(), {{"S_z_p", "bosonic", "Sz_p"}, {"S_p_p", "bosonic", "Sp_p"}});
+    }
     out << ws.size() << "\n";
     for (const W& w : ws) {
       std::vector<double> dense((size_t)dim * dim, 0.0);
@@ -1247,7 +1284,7 @@ struct Engine {
     return std::chrono::duration<double>(b - a).count();
   }
 
-  StepState begin_step(char type, int label, int sweep, bool far = true) {
+  StepState begin_step(char type, int label, int sweep, bool far = true, char link = 'n') {
     using clk = std::chrono::steady_clock;
     StepState S;
     S.t0 = clk::now();
@@ -1269,8 +1306,9 @@ struct Engine {
     S.fromL0 = left.from;
     S.fromR0 = right.from;
     auto parL = std::make_shared<const Block>(left), parR = std::make_shared<const Block>(right);
-    left = enlarge(left, true);
-    right = enlarge(right, false);
+    left = enlarge(left, true, link);
+    right = enlarge(right, false, link);
+    I.link = link;
     left.parent = parL;
     right.parent = parR;
     pt.next("step.setup");
@@ -1441,12 +1479,14 @@ struct Engine {
     steps.push_back(std::move(st));
   }
 
-  void step(char type, int label, int sweep, bool far = true) {
-    StepState S = begin_step(type, label, sweep, far);
+  void step(char type, int label, int sweep, bool far = true, char link = 'n') {
+    StepState S = begin_step(type, label, sweep, far, link);
     diag(S);
     end_step(S);
   }
 
+  // infinite_DMRG, src/DMRG_MAIN.f90:140-159: with PBC every iteration is a
+  // link-'n' step followed by a link-'p' step.
   void infinite() {
     bl1.clear();
     bl2.clear();
@@ -1454,6 +1494,10 @@ struct Engine {
     bl2[right.length] = right;
     while (left.length < par.ldmrg) {
       step('i', 0, 0);
+      bl1[left.length] = left;
+      bl2[right.length] = right;
+      if (!par.pbc) continue;
+      step('i', 0, 0, true, 'p');
       bl1[left.length] = left;
       bl2[right.length] = right;
     }
@@ -1466,6 +1510,7 @@ struct Engine {
   bool finite_ready = false;
 
   void finite_begin() {
+    NSS_REQUIRE(!par.pbc, NSS_E_ARG, "finite_DMRG ERROR: Finite DMRG with PBC not yet implemented.");
     bl1.clear();
     bl2.clear();
     bl1[1] = left;
@@ -1517,10 +1562,10 @@ struct Batch {
   nss_ctx* ctx = nullptr;
   std::vector<Engine*> es;
 
-  void step(char type, int label, int sweep, bool far = true) {
+  void step(char type, int label, int sweep, bool far = true, char link = 'n') {
     std::vector<Engine::StepState> S;
     S.reserve(es.size());
-    for (Engine* e : es) S.push_back(e->begin_step(type, label, sweep, far));
+    for (Engine* e : es) S.push_back(e->begin_step(type, label, sweep, far, link));
     std::vector<LanczosJob> jobs;
     std::vector<size_t> who;
     for (size_t i = 0; i < es.size(); ++i) {
@@ -1566,6 +1611,7 @@ struct Batch {
   }
 
   void finite_begin() {
+    NSS_REQUIRE(!es[0]->par.pbc, NSS_E_ARG, "finite_DMRG ERROR: Finite DMRG with PBC not yet implemented.");
     for (Engine* e : es) {
       e->bl1.clear();
       e->bl2.clear();
@@ -1590,10 +1636,12 @@ struct Batch {
       e->bl2[e->right.length] = e->right;
     }
     while (es[0]->left.length < es[0]->par.ldmrg) {
-      step('i', 0, 0);
-      for (Engine* e : es) {
-        e->bl1[e->left.length] = e->left;
-        e->bl2[e->right.length] = e->right;
+      for (char link : es[0]->par.pbc ? std::string("np") : std::string("n")) {
+        step('i', 0, 0, true, link);
+        for (Engine* e : es) {
+          e->bl1[e->left.length] = e->left;
+          e->bl2[e->right.length] = e->right;
+        }
       }
     }
   }
@@ -1665,8 +1713,16 @@ extern "C" int nss_dmrg_create(nss_ctx* ctx, const nss_model* model, const nss_p
   }
   E->hij.assign(model->hij, model->hij + 4);
   E->want32 = params->precision == NSS_FP32_MIXED;
+  if (E->par.pbc && E->par.ldmrg % 2 == 0) ++E->par.ldmrg;  // Ldmrg = 2k+1 under PBC, src/INPUT_VARS.f90:178-183
   try {
     E->site = make_site(*model);
+    if (E->par.pbc) {
+      // the operators of the outer ends (PBC_prev), src/BLOCKS/SITES.f90:339-342, :413-424
+      std::vector<std::string> keys;
+      for (const auto& kv : E->site.ops)
+        if (kv.first != "H") keys.push_back(kv.first);
+      for (const std::string& k : keys) E->site.ops[k + "_p"] = E->site.ops[k];
+    }
     E->left = E->block_from_site();
     E->right = E->block_from_site();
   } catch (...) {
@@ -1727,6 +1783,7 @@ static Batch make_batch(int n, nss_dmrg* const* ds, bool finite) {
     if (!B.ctx) B.ctx = e->ctx;
     NSS_REQUIRE(e->ctx == B.ctx, NSS_E_ARG, "nss_dmrg_batch: engines on different contexts");
     NSS_REQUIRE(e->par.ldmrg == ds[0]->e->par.ldmrg, NSS_E_ARG, "nss_dmrg_batch: engines with different LDMRG");
+    NSS_REQUIRE(!e->par.pbc == !ds[0]->e->par.pbc, NSS_E_ARG, "nss_dmrg_batch: engines with different PBCdmrg");
     if (finite)
       NSS_REQUIRE(e->msweep.size() == ds[0]->e->msweep.size(), NSS_E_ARG, "nss_dmrg_batch: different NSWEEP");
     for (Engine* o : B.es) NSS_REQUIRE(o != e, NSS_E_ARG, "nss_dmrg_batch: an engine appears twice");
@@ -1896,6 +1953,9 @@ extern "C" int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int np
   NSS_API_BEGIN
   NSS_REQUIRE(d && d->e && ops && pos && out && nops > 0 && npos > 0, NSS_E_ARG, "nss_dmrg_measure: bad arguments");
   nss::Engine* E = d->e;
+  // Measure_DMRG under PBC walks the ring's own site order
+  // (src/DMRG_MEASURE.f90:320-360), which the engine does not keep
+  NSS_REQUIRE(!E->par.pbc, NSS_E_ARG, "nss_dmrg_measure: not available for PBC runs");
   NSS_HIP(hipSetDevice(E->ctx->device));
   const int dd = E->site.dim;
   for (int o = 0; o < nops; ++o) {

@@ -42,6 +42,7 @@ module nss_dmrg
      integer(c_int32_t) :: v0_mode = 0
      real(c_double) :: deg_thr = 0.1d0
      integer(c_int32_t) :: precision = 0, verbose = 0, predict = 0
+     integer(c_int32_t) :: pbc = 0
   end type nss_params
 
   type, bind(C), public :: nss_step_info
@@ -54,6 +55,7 @@ module nss_dmrg
      integer(c_int32_t) :: nsect, nterms, numop, numop_fp32
      real(c_double) :: hv_flops
      real(c_double) :: t_step, t_diag, t_hv, t_rdm, t_renorm, t_enlarge
+     integer(c_int32_t) :: link
   end type nss_step_info
 
   !> dense operator block of a superblock sector (column-major, Fortran)

@@ -23,6 +23,12 @@ Pinning (tests/test_oracle_*.py): the reference's golden ``.check`` values
 (spin-1/2 Heisenberg L40 M20 and Hubbard U=0 L40 M40 trajectories), closed
 forms (4-site Heisenberg, free fermions), brute-force exact diagonalisation of
 full chains, and invariants (symmetry of H_sb, trace of rho, lambda_L=lambda_R).
+The PBC path (link-'p' steps) is pinned by periodic-ring closed forms
+(Heisenberg rings of 4 and 6 sites, free-fermion rings), a Fock-space ED of
+the Hubbard ring, a brute-force spin-1 ring, the fermion algebra of both
+block ends and the symmetry of every PBC H_sb.  Under PBC the spin terms are
+the Hermitian ones of connect_spin_blocks(link='p'); the reference's direct
+spin setup differs there (see setup_direct).
 """
 from __future__ import annotations
 
@@ -174,13 +180,33 @@ def sectors(qn: np.ndarray):
     return order, {q: np.array(v, dtype=np.int64) for q, v in maps.items()}
 
 
-def _connect(left_ops, right_ops, kind, hij):
-    """connect_{spin,fermion}_blocks, src/DMRG_CONNECT.f90:296-454 (link 'n')."""
+def pbc_site(s: Site) -> Site:
+    """The site of a PBC run: every operator but H also under its link-'p'
+    key (suffix ``_p``), src/BLOCKS/SITES.f90:339-342 (spin) and :413-424
+    (electron: C and P)."""
+    ops, optype = dict(s.ops), dict(s.optype)
+    for k in s.ops:
+        if k != "H":
+            ops[k + "_p"] = s.ops[k].copy()
+            optype[k + "_p"] = s.optype[k]
+    return Site(s.kind, s.dim, s.qn, ops, optype)
+
+
+def _link(key: str) -> str:
+    """Link of an operator key: ``_p`` keys are the 'p' end, all others 'n'."""
+    return "p" if key.endswith("_p") else "n"
+
+
+def _connect(left_ops, right_ops, kind, hij, link: str = "n"):
+    """connect_{spin,fermion}_blocks, src/DMRG_CONNECT.f90:296-454: the
+    operators of the given link on both sides, the sign operator P of that
+    link on the left one."""
+    x = "" if link == "n" else "_p"
     if kind == "spin":
-        return (hij[0][0] * np.kron(left_ops["Sz"], right_ops["Sz"])
-                + hij[1][1] * np.kron(left_ops["Sp"], right_ops["Sp"].T)
-                + hij[1][1] * np.kron(left_ops["Sp"].T, right_ops["Sp"]))
-    pl = left_ops["P"]
+        return (hij[0][0] * np.kron(left_ops["Sz" + x], right_ops["Sz" + x])
+                + hij[1][1] * np.kron(left_ops["Sp" + x], right_ops["Sp" + x].T)
+                + hij[1][1] * np.kron(left_ops["Sp" + x].T, right_ops["Sp" + x]))
+    pl = left_ops["P" + x]
     out = 0.0
     names = ["Cup", "Cdn"]
     for io in range(2):
@@ -188,39 +214,65 @@ def _connect(left_ops, right_ops, kind, hij):
             t = hij[io][jo]
             if t == 0.0:
                 continue
-            cl, cr = left_ops[names[io]], right_ops[names[jo]]
+            cl, cr = left_ops[names[io] + x], right_ops[names[jo] + x]
             out = out + t * np.kron(cl.T @ pl, cr) + t * np.kron(pl @ cl, cr.T)
     return out
 
 
-def enlarge_block(b: Block, dot: Site, side: str, hij) -> Block:
-    """enlarge_block, src/DMRG_CONNECT.f90:18-291 (OBC, link 'n').
+def enlarge_block(b: Block, dot: Site, side: str, hij, link: str = "n") -> Block:
+    """enlarge_block, src/DMRG_CONNECT.f90:18-291.
 
-    left : H = H_b x 1_d + 1_b x H_d + H_{b,d};  O -> (1_b | P_b) x O_d
-    right: H = 1_d x H_b + H_d x 1_b + H_{d,b};  O -> O_d x 1_b,  P -> P_d x P_b
-    basis: left = self .o. dot, right = dot .o. self (QN_TUPLE_BASIS.f90:371-398).
+    Link 'n' (OBC and PBC_next, the dot joins the inner end):
+      left : H = H_b x 1_d + 1_b x H_d + H_{b,d}, basis self .o. dot;
+      right: H = 1_d x H_b + H_d x 1_b + H_{d,b}, basis dot .o. self.
+    Link 'p' (PBC_prev, the dot joins the outer end):
+      left : H = 1_d x H_b + H_d x 1_b + H_{d,b}, basis dot .o. self;
+      right: H = H_b x 1_d + 1_b x H_d + H_{b,d}, basis self .o. dot.
+    H_{x,y} is connect(x, y) on the step's link.  The other operators
+    (:116-237) follow the table below, with O of link olink (``_p`` keys
+    'p'), P_b / P_d the sign operators of that link, and 1 identities:
+
+      link side olink  bosonic      fermionic    psign
+      n    l    n      1_b x O_d    P_b x O_d    P_b x O_d
+      n    l    p      O_b x 1_d    O_b x 1_d    O_b x P_d
+      n    r    n      O_d x 1_b    O_d x 1_b    O_d x P_b
+      n    r    p      1_d x O_b    P_d x O_b    P_d x O_b
+      p    l    n      1_d x O_b    P_d x O_b    P_d x O_b
+      p    l    p      O_d x 1_b    O_d x 1_b    O_d x P_b
+      p    r    n      O_b x 1_d    O_b x 1_d    O_b x P_d
+      p    r    p      1_b x O_d    P_b x O_d    P_b x O_d
     """
     db, dd = b.dim, dot.dim
     ib, idd = np.eye(db), np.eye(dd)
     ops, optype = {}, dict(b.optype)
-    if side == "left":
+    # dot joins after the block (block factor first) or before it
+    dot_last = (side == "left") == (link == "n")
+    if dot_last:
         ops["H"] = (np.kron(b.ops["H"], idd) + np.kron(ib, dot.ops["H"])
-                    + _connect(b.ops, dot.ops, b.kind, hij))
-        for k, t in b.optype.items():
-            if k == "H":
-                continue
-            eo = ib if t == "bosonic" else b.ops["P"]
-            ops[k] = np.kron(eo, dot.ops[k])
+                    + _connect(b.ops, dot.ops, b.kind, hij, link))
         qn = (b.qn[:, None, :] + dot.qn[None, :, :]).reshape(db * dd, -1)
     else:
         ops["H"] = (np.kron(idd, b.ops["H"]) + np.kron(dot.ops["H"], ib)
-                    + _connect(dot.ops, b.ops, b.kind, hij))
-        for k, t in b.optype.items():
-            if k == "H":
-                continue
-            eo = b.ops["P"] if t == "psign" else ib
-            ops[k] = np.kron(dot.ops[k], eo)
+                    + _connect(dot.ops, b.ops, b.kind, hij, link))
         qn = (dot.qn[:, None, :] + b.qn[None, :, :]).reshape(db * dd, -1)
+    for k, t in b.optype.items():
+        if k == "H":
+            continue
+        x = "" if _link(k) == "n" else "_p"
+        if _link(k) == link:
+            # the end the dot joins: the dot's operator, with the block's
+            # sign in front of it where the block precedes the dot
+            if dot_last:
+                ops[k] = np.kron(ib if t == "bosonic" else b.ops["P" + x], dot.ops[k])
+            else:
+                ops[k] = np.kron(dot.ops[k], b.ops["P" + x] if t == "psign" else ib)
+        else:
+            # the other end keeps the block's operator, with the dot's sign
+            # in front of it where the dot precedes the block
+            if dot_last:
+                ops[k] = np.kron(b.ops[k], dot.ops["P" + x] if t == "psign" else idd)
+            else:
+                ops[k] = np.kron(idd if t == "bosonic" else dot.ops["P" + x], b.ops[k])
     return Block(b.kind, b.length + 1, qn, ops, optype)
 
 
@@ -271,33 +323,42 @@ def sb_get_states(left: Block, right: Block, target: tuple) -> Superblock:
     return Superblock(target, qns, dl, dr, off, lm, rm)
 
 
-def setup_direct(sb: Superblock, left: Block, right: Block, hij) -> Superblock:
+def setup_direct(sb: Superblock, left: Block, right: Block, hij, pbc: bool = False) -> Superblock:
     """Setup_SuperBlock_{Spin,Fermion}_Direct, src/DMRG_SUPERBLOCK_SETUP.f90:152-693.
 
-    Terms are (dst, src, A, B, is_hconjg): Y_dst += A . Psi_src . B^T.
+    Terms are (dst, src, A, B, is_hconjg): Y_dst += A . Psi_src . B^T.  With
+    PBC the terms of link 'p' (the operators of both outer ends) follow those
+    of link 'n' in every sector (spin :323-358) or every sector and spin
+    (fermion :619-638).  The spin terms under PBC are those of
+    connect_spin_blocks(link='p') (src/DMRG_CONNECT.f90:396-454, the dense
+    path's H, SETUP:74): every sector gets its Sz.Sz term and the S+S- term
+    its own conjugate partner.  The reference's direct spin setup instead
+    copies the partner of the link-'n' term (SETUP:352-354) and skips the
+    sectors without a qn-1 neighbour (the cycle at :305).
     """
     index = {q: i for i, q in enumerate(sb.qns)}
     sb.HL = [left.ops["H"][np.ix_(m, m)] for m in sb.lmap]
     sb.HR = [right.ops["H"][np.ix_(m, m)] for m in sb.rmap]
     sb.terms = []
+    links = [""] + (["_p"] if pbc else [])
     for isb, q in enumerate(sb.qns):
         ai, bi = sb.lmap[isb], sb.rmap[isb]
-        if left.kind == "spin":                       # :289-321
-            a = hij[0][0] * left.ops["Sz"][np.ix_(ai, ai)]
-            b = right.ops["Sz"][np.ix_(bi, bi)]
-            sb.terms.append((isb, isb, a, b, 0))
-            qm = (q[0] - 1.0,)
-            if qm not in index:
-                continue
-            jsb = index[qm]
-            aj, bj = sb.lmap[jsb], sb.rmap[jsb]
-            a2 = hij[1][1] * left.ops["Sp"][np.ix_(ai, aj)]
-            b2 = right.ops["Sp"].T[np.ix_(bi, bj)]
-            sb.terms.append((isb, jsb, a2, b2, 0))
-            sb.terms.append((jsb, isb, a2.T.copy(), b2.T.copy(), 1))
-        else:                                          # :574-641
+        if left.kind == "spin":                           # :289-358
+            for x in links:
+                a = hij[0][0] * left.ops["Sz" + x][np.ix_(ai, ai)]
+                b = right.ops["Sz" + x][np.ix_(bi, bi)]
+                sb.terms.append((isb, isb, a, b, 0))
+                qm = (q[0] - 1.0,)
+                if qm not in index:
+                    continue
+                jsb = index[qm]
+                aj, bj = sb.lmap[jsb], sb.rmap[jsb]
+                a2 = hij[1][1] * left.ops["Sp" + x][np.ix_(ai, aj)]
+                b2 = right.ops["Sp" + x].T[np.ix_(bi, bj)]
+                sb.terms.append((isb, jsb, a2, b2, 0))
+                sb.terms.append((jsb, isb, a2.T.copy(), b2.T.copy(), 1))
+        else:                                              # :574-641
             names = ["Cup", "Cdn"]
-            pl = left.ops["P"]
             for ispin, dq in enumerate([(1.0, 0.0), (0.0, 1.0)]):
                 qm = (q[0] - dq[0], q[1] - dq[1])
                 if qm not in index:
@@ -307,10 +368,11 @@ def setup_direct(sb: Superblock, left: Block, right: Block, hij) -> Superblock:
                 t = hij[ispin][ispin]
                 if t == 0.0:
                     continue
-                a1 = t * (left.ops[names[ispin]].T @ pl)[np.ix_(ai, aj)]
-                b1 = right.ops[names[ispin]][np.ix_(bi, bj)]
-                sb.terms.append((isb, jsb, a1, b1, 0))
-                sb.terms.append((jsb, isb, a1.T.copy(), b1.T.copy(), 1))
+                for x in links:
+                    a1 = t * (left.ops[names[ispin] + x].T @ left.ops["P" + x])[np.ix_(ai, aj)]
+                    b1 = right.ops[names[ispin] + x][np.ix_(bi, bj)]
+                    sb.terms.append((isb, jsb, a1, b1, 0))
+                    sb.terms.append((jsb, isb, a1.T.copy(), b1.T.copy(), 1))
     return sb
 
 
@@ -537,6 +599,7 @@ class Params:
     seed: int = 0
     norb: int = 1
     ncv_max: int = 600
+    pbc: bool = False               # PBCdmrg (infinite algorithm only)
 
 
 @dataclass
@@ -566,9 +629,19 @@ class StepRecord:
 
 
 class DMRG:
-    """init_dmrg / run_DMRG / step_dmrg, src/DMRG_MAIN.f90:22-433 (OBC)."""
+    """init_dmrg / run_DMRG / step_dmrg, src/DMRG_MAIN.f90:22-433.
+
+    With ``p.pbc`` the site carries the link-'p' operators, an even LDMRG is
+    shifted to the next odd one (src/INPUT_VARS.f90:178-183), every infinite
+    iteration is a link-'n' step followed by a link-'p' step, and the finite
+    algorithm is refused, as in the reference (src/DMRG_MAIN.f90:179-180).
+    """
 
     def __init__(self, dot: Site, p: Params):
+        if p.pbc:
+            dot = pbc_site(dot)
+            if p.ldmrg % 2 == 0:
+                p.ldmrg += 1
         self.dot, self.p = dot, p
         self.left = block_from_site(dot)
         self.right = block_from_site(dot)
@@ -576,15 +649,15 @@ class DMRG:
         self.last_sb = None
         self.last_psi = None
 
-    def step(self, kind: str, label: int = 0, mstates: int = None) -> StepRecord:
+    def step(self, kind: str, label: int = 0, mstates: int = None, link: str = "n") -> StepRecord:
         p = self.p
         ms = p.mdmrg if mstates is None else mstates
-        self.left = enlarge_block(self.left, self.dot, "left", p.hij)
-        self.right = enlarge_block(self.right, self.dot, "right", p.hij)
+        self.left = enlarge_block(self.left, self.dot, "left", p.hij, link)
+        self.right = enlarge_block(self.right, self.dot, "right", p.hij, link)
         L = self.left.length + self.right.length
         target = current_target(L, p.target_qn, p.norb)
         sb = sb_get_states(self.left, self.right, target)
-        setup_direct(sb, self.left, self.right, p.hij)
+        setup_direct(sb, self.left, self.right, p.hij, p.pbc)
         msb = sb.n
         neig = min(msb, p.neigen)
         nop = 0
@@ -622,13 +695,17 @@ class DMRG:
         return rec
 
     def infinite(self):
-        """infinite_DMRG, src/DMRG_MAIN.f90:140-159 (OBC)."""
+        """infinite_DMRG, src/DMRG_MAIN.f90:140-159."""
         while self.left.length < self.p.ldmrg:
-            self.step("i")
+            self.step("i", link="n")
+            if self.p.pbc:
+                self.step("i", link="p")
 
     def finite(self):
         """finite_DMRG, src/DMRG_MAIN.f90:168-241 (OBC only; PBC refused :179-180)."""
         p = self.p
+        if p.pbc:
+            raise RuntimeError("finite_DMRG ERROR: Finite DMRG with PBC not yet implemented.")
         fn = 2 * p.ldmrg
         bl = {1: {1: self.left.copy()}, 2: {1: self.right.copy()}}
         while self.left.length < p.ldmrg:

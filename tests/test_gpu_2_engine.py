@@ -362,3 +362,83 @@ def test_batch_rejects_mismatched_engines(ctx):
         N.DMRGBatch([a, b]).run("i")
     with pytest.raises(N.NssError, match="twice"):
         N.DMRGBatch([a, a]).run("i")
+
+
+# --------------------------------------------------------------------------- PBC
+# Periodic rings (PBCdmrg): the infinite algorithm alternates link-'n' and
+# link-'p' steps; the oracle's PBC path is pinned by ring closed forms and a
+# Fock-space ED (tests/test_oracle.py).
+def _pbc_case(model):
+    if model == "hubbard":
+        return O.electron_site(3.0, True), N.electron_site(), HUB_HIJ, (0.5, 0.5), 3.0, 24
+    if model == "spin1":
+        return O.spin_site(3), N.spin_site(3), SPIN_HIJ, (0.0, 0.0), 0.0, 20
+    return O.spin_site(2), N.spin_site(2), SPIN_HIJ, (0.0, 0.0), 0.0, 16
+
+
+@pytest.mark.parametrize("model", ["spin12", "spin1", "hubbard"])
+def test_pbc_infinite_matches_oracle(ctx, model):
+    osite, site, hij, tq, U, M = _pbc_case(model)
+    print(f"case pbc infinite {model}", flush=True)
+    L = 7
+    p = O.Params(ldmrg=L, mdmrg=M, target_qn=tq[:2] if model == "hubbard" else (0.0,), hij=hij, neigen=1,
+                 lanc_dim_threshold=64, pbc=True)
+    od = O.DMRG(osite, p)
+    od.infinite()
+    d, st = run(ctx, "i", site=site, hij=np.array(hij), ldmrg=L, mdmrg=M, target_qn=tq, neigen=1,
+                lanc_dim_threshold=64, uloc=U, hfmode=True, pbc=True)
+    assert len(st) == len(od.records)
+    assert [s["link"] for s in st] == ["n", "p"] * (len(st) // 2)
+    assert any(s["numop"] > 0 for s in st)           # the Lanczos path ran
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert s["L"] == r.L and s["m_sb"] == r.m_sb and s["nsect"] == r.nsect, i
+        assert abs(s["energies"][0] - r.energies[0]) < 1e-10 * max(1.0, abs(r.energies[0])), i
+        assert abs(s["entropy_left"] - r.entropy_left) < 1e-9, i
+        if s["renormalized"]:
+            assert d.kept_qn(i, "left", W_MIN) == _kept_oracle(r, W_MIN), i
+            if min(s["cut_left"], r.cut_left) > W_MIN:
+                assert s["m_left"] == r.m_left, i
+
+
+def test_pbc_ring_closed_forms(ctx):
+    # untruncated rings: spin-1/2 Heisenberg E0(4) = -2, E0(6) = -2.8027756377319946;
+    # free fermions (U=0) on periodic rings of 4 and 6 sites at half filling: -4, -8
+    print("case pbc closed forms", flush=True)
+    _, st = run(ctx, "i", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=3, mdmrg=200, target_qn=(0.0, 0.0),
+                neigen=1, lanc_dim_threshold=8, pbc=True)
+    assert [s["L"] for s in st] == [4, 6]
+    assert abs(st[0]["energies"][0] + 2.0) < 1e-11 and abs(st[1]["energies"][0] + 2.8027756377319946) < 1e-11
+    _, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=3, mdmrg=400,
+                target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=16, uloc=0.0, hfmode=True, pbc=True)
+    assert abs(st[0]["energies"][0] + 4.0) < 1e-11 and abs(st[1]["energies"][0] + 8.0) < 1e-11
+
+
+def test_pbc_refusals_and_odd_ldmrg(ctx):
+    # an even LDMRG becomes odd (INPUT_VARS.f90:178-183); finite_DMRG and
+    # Measure_DMRG refuse PBC runs
+    print("case pbc refusals", flush=True)
+    kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=4, mdmrg=8, msweep=[8], target_qn=(0.0, 0.0),
+              neigen=1, lanc_dim_threshold=64, pbc=True)
+    d, st = run(ctx, "i", **kw)
+    assert st[-1]["left_len"] == 5 and len(st) == 4
+    with pytest.raises(N.NssError, match="PBC"):
+        d.measure(np.diag([0.5, -0.5]), [1])
+    d2 = N.DMRG(ctx, N.DMRGConfig(**kw))
+    with pytest.raises(N.NssError, match="PBC"):
+        d2.run("f")
+
+
+def test_pbc_batch_matches_single_runs(ctx):
+    print("case pbc batch", flush=True)
+    cases = [dict(site=N.electron_site(), hij=np.array(HUB_HIJ), uloc=2.0, target_qn=(0.5, 0.5), mdmrg=24),
+             dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), uloc=0.0, target_qn=(0.0, 0.0), mdmrg=16)]
+    common = dict(ldmrg=7, neigen=1, lanc_dim_threshold=64, hfmode=True, pbc=True)
+    single = [run(ctx, "i", **c, **common)[1] for c in cases]
+    engines = [N.DMRG(ctx, N.DMRGConfig(**c, **common)) for c in cases]
+    batched = N.DMRGBatch(engines).run("i")
+    for sb, ss in zip(batched, single):
+        assert len(sb) == len(ss)
+        for a, b in zip(sb, ss):
+            assert a["link"] == b["link"] and a["m_sb"] == b["m_sb"]
+            assert abs(a["energies"][0] - b["energies"][0]) < 1e-10 * max(1.0, abs(b["energies"][0]))
+            assert abs(a["entropy_left"] - b["entropy_left"]) < 1e-8

@@ -273,3 +273,141 @@ def test_c_restatement_matches_numpy_oracle(model):
     v = O.splitmix64_uniform_fast(sb.n, 4)
     assert np.allclose(c.hv(v), O.hv_direct(sb, v), atol=1e-12)
     assert np.allclose(c.hv(v), O.hv_direct_loops(sb, v), atol=1e-12)
+
+
+# --------------------------------------------------------------------------- PBC
+# Periodic rings (PBCdmrg, infinite algorithm).  Independent pins: ring
+# closed forms and a Fock-space ED whose fermion signs come from the mode
+# order alone, so a wrong Jordan-Wigner string at either block end fails.
+def _ring_ed_fermion(N, hij_t, U, hf, nup, ndw):
+    """Lowest energy of the Hubbard ring (N sites, bonds (i, i+1 mod N)) in
+    the (nup, ndw) sector: modes 2i (up) and 2i+1 (dw), c_m |n> carries
+    (-1)^(number of occupied modes below m)."""
+    states = [s for s in range(1 << (2 * N))
+              if bin(s & int("01" * N, 2)).count("1") == nup and bin(s & int("10" * N, 2)).count("1") == ndw]
+    idx = {s: i for i, s in enumerate(states)}
+    H = np.zeros((len(states), len(states)))
+
+    def hop(s, a, b):  # c_a^+ c_b |s>: (sign, new state) or None
+        if not (s >> b) & 1 or (a != b and (s >> a) & 1):
+            return None
+        sg = (-1) ** bin(s & ((1 << b) - 1)).count("1")
+        s1 = s & ~(1 << b)
+        sg *= (-1) ** bin(s1 & ((1 << a) - 1)).count("1")
+        return sg, s1 | (1 << a)
+
+    bonds = [(i, (i + 1) % N) for i in range(N)] if N > 2 else [(0, 1)]
+    for s in states:
+        i0 = idx[s]
+        for i in range(N):
+            nu, nd = (s >> (2 * i)) & 1, (s >> (2 * i + 1)) & 1
+            H[i0, i0] += U * nu * nd - (0.5 * U * (nu + nd) if hf else 0.0)
+        for i, j in bonds:
+            for sp in range(2):
+                for a, b in ((2 * i + sp, 2 * j + sp), (2 * j + sp, 2 * i + sp)):
+                    r = hop(s, a, b)
+                    if r:
+                        H[idx[r[1]], i0] += hij_t * r[0]
+    return float(np.linalg.eigvalsh(H)[0])
+
+
+def _ring_ed_spin(sun, N, hij):
+    """Lowest energy of the Heisenberg ring in the Sz=0 sector (brute force)."""
+    s = O.spin_site(sun)
+    d = s.dim
+    Sz, Sp = s.ops["Sz"], s.ops["Sp"]
+
+    def at(op, i):
+        m = np.eye(1)
+        for k in range(N):
+            m = np.kron(m, op if k == i else np.eye(d))
+        return m
+
+    H = 0.0
+    for i in range(N):
+        j = (i + 1) % N
+        H = H + hij[0][0] * at(Sz, i) @ at(Sz, j) + hij[1][1] * (at(Sp, i) @ at(Sp.T, j) + at(Sp.T, i) @ at(Sp, j))
+    sz = sum(np.diag(at(Sz, i)) for i in range(N))
+    sel = np.isclose(sz, 0.0)
+    return float(np.linalg.eigvalsh(H[np.ix_(sel, sel)])[0])
+
+
+def _pbc_run(site, ldmrg, m, tq, hij, thr=100000):
+    p = O.Params(ldmrg=ldmrg, mdmrg=m, target_qn=tq, hij=hij, neigen=1, lanc_dim_threshold=thr, pbc=True)
+    d = O.DMRG(site, p)
+    d.infinite()
+    return d
+
+
+def test_pbc_heisenberg_ring_closed_forms():
+    # spin-1/2 Heisenberg rings: E0(4) = -2, E0(6) = -2.8027756377319946
+    # (exact diagonalisation values quoted throughout the literature)
+    d = _pbc_run(O.spin_site(2), 3, 200, (0.0,), SPIN_HIJ)
+    assert [r.L for r in d.records] == [4, 6]
+    assert abs(d.records[0].energies[0] + 2.0) < 1e-12
+    assert abs(d.records[1].energies[0] + 2.8027756377319946) < 1e-12
+
+
+def test_pbc_even_ldmrg_shifted_and_finite_refused():
+    p = O.Params(ldmrg=4, mdmrg=8, target_qn=(0.0,), hij=SPIN_HIJ, pbc=True)
+    d = O.DMRG(O.spin_site(2), p)
+    assert d.p.ldmrg == 5                                  # INPUT_VARS.f90:178-183
+    with pytest.raises(RuntimeError, match="PBC"):
+        d.finite()
+
+
+@pytest.mark.parametrize("N", [4, 6])
+def test_pbc_free_fermion_ring(N):
+    # U=0: sum of the lowest ring levels -2 cos(2 pi k / N) per spin (periodic
+    # fermions; the antiperiodic ring would give -5.657 and -6.928)
+    d = _pbc_run(O.electron_site(0.0, True), 3, 400, (0.5, 0.5), HUB_HIJ)
+    rec = {r.L: r for r in d.records}[N]
+    eps = np.sort(-2.0 * np.cos(2.0 * np.pi * np.arange(N) / N))
+    assert abs(rec.energies[0] - 2.0 * eps[:N // 2].sum()) < 1e-11
+
+
+def test_pbc_hubbard_ring_equals_fock_ed():
+    d = _pbc_run(O.electron_site(4.0, True), 3, 400, (0.5, 0.5), HUB_HIJ)
+    for r in d.records:
+        e = _ring_ed_fermion(r.L, -1.0, 4.0, True, r.L // 2, r.L // 2)
+        assert abs(r.energies[0] - e) < 1e-10 * abs(e), r.L
+
+
+def test_pbc_spin1_ring_equals_ed():
+    d = _pbc_run(O.spin_site(3), 3, 400, (0.0,), SPIN_HIJ)
+    for r in d.records:
+        assert abs(r.energies[0] - _ring_ed_spin(3, r.L, SPIN_HIJ)) < 1e-10, r.L
+
+
+def test_pbc_blocks_keep_fermion_algebra_at_both_ends():
+    # operators of the inner (n) and outer (p) end of an enlarged block are
+    # fermions: {C_a, C_b^+} = delta_ab, {C_a, C_b} = 0, P C P = -C
+    s = O.pbc_site(O.electron_site(2.0, True))
+    for side in ("left", "right"):
+        b = O.block_from_site(s)
+        for link in ("n", "p", "p", "n"):
+            b = O.enlarge_block(b, s, side, HUB_HIJ, link)
+            cs = [b.ops[k] for k in ("Cup", "Cdn", "Cup_p", "Cdn_p")]
+            P = b.ops["P"]
+            assert np.allclose(P, b.ops["P_p"])
+            one = np.eye(b.dim)
+            for i, a in enumerate(cs):
+                assert np.allclose(P @ a @ P, -a)
+                for j, c in enumerate(cs):
+                    assert np.allclose(a @ c + c @ a, 0.0), (side, link, i, j)
+                    anti = a @ c.T + c.T @ a
+                    # the two ends are different sites from the first enlargement on
+                    assert np.allclose(anti, one if i == j else 0.0), (side, link, i, j)
+
+
+def test_pbc_superblock_hamiltonian_is_symmetric():
+    # Hermitian H_sb in every step: the link-'p' spin-flip term carries its
+    # own conjugate partner
+    for site, tq, hij in ((O.spin_site(2), (0.0,), SPIN_HIJ), (O.electron_site(3.0, True), (0.5, 0.5), HUB_HIJ)):
+        p = O.Params(ldmrg=5, mdmrg=12, target_qn=tq, hij=hij, neigen=1, lanc_dim_threshold=2000, pbc=True)
+        d = O.DMRG(site, p)
+        while d.left.length < p.ldmrg:
+            for link in ("n", "p"):
+                d.step("i", link=link)
+                h = O.sb_dense(d.last_sb)
+                assert np.allclose(h, h.T, atol=1e-12)
