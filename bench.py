@@ -25,6 +25,11 @@ is timed on the same workload and reported under "fp32_mixed".
 U and filling per chain from the seed) stepped in lockstep, every step's
 ground states from one batched Lanczos whose H.v is one grouped launch per
 phase over all chains; a step is one sweep of all chains.
+
+--workload hubbard_pbc_L48_m2048: the periodic Hubbard ring (U=8, half
+filling, m=2048).  The reference refuses finite sweeps under PBC, so the
+iDMRG growth through the 48-site ring is timed (a step is one iDMRG step),
+in FP64 and with the FP32-MFMA H.v, whose |dE0/L| against FP64 is reported.
 """
 from __future__ import annotations
 
@@ -46,16 +51,21 @@ PEAK_FP32_MFMA = 157.3e12
 METRIC = "finite-sweep wall time + H·v GFLOP/s, FP64 vs FP32-MFMA at |ΔE0/L|≤1e-9"
 
 WORKLOADS = {
-    # configs[1] of the benchmark definition
+    # the headline workload: Hubbard U=4, OBC, L=64, m=256, finite sweeps
     "hubbard_L64_m256": dict(model="electron", L=64, m=256, U=4.0, seed=1),
     # smaller variants for quick checks
     "hubbard_L32_m128": dict(model="electron", L=32, m=128, U=4.0, seed=1),
     "spin12_L40_m64": dict(model="spin12", L=40, m=64, U=0.0, seed=0),
     "spin1_L100_m1024": dict(model="spin1", L=100, m=1024, U=0.0, seed=2),
-    # 64 independent Hubbard chains, L=32, per chain from the
-    # seed m in [128, 512], U in [0, 8), N in {16, 14, 18} (N_up = N_dw = N/2)
+    # 64 independent Hubbard chains, L=32, per chain from the seed m in
+    # [128, 512], U in [0, 8), N in {16, 14, 18} (N_up = N_dw = N/2)
     "hubbard_batch64_L32": dict(model="electron", L=32, batch=64, m_lo=128, m_hi=512, U_hi=8.0,
                                 fillings=(16, 14, 18), seed=3),
+    # periodic Hubbard ring, U=8, half filling, m=2048: the iDMRG growth with
+    # link-'n' and link-'p' steps through the 48-site ring (LDMRG 24, shifted
+    # to 25 under PBC as the reference does; the growth ends at 50 sites).
+    # The reference refuses finite sweeps under PBC, so the growth is timed.
+    "hubbard_pbc_L48_m2048": dict(model="electron", pbc=True, L=48, m=2048, U=8.0, seed=4),
 }
 
 
@@ -259,6 +269,59 @@ def main_batch(N, ctx, w, a):
     print(json.dumps(out), flush=True)
 
 
+def run_pbc(N, ctx, w, precision, verbose):
+    """iDMRG growth of the periodic ring (link 'n' and 'p' steps) in one precision."""
+    cfg = N.DMRGConfig(site=N.electron_site(), hij=np.diag([-1.0, -1.0]), ldmrg=w["L"] // 2, mdmrg=w["m"],
+                       target_qn=(0.5, 0.5), uloc=w["U"], hfmode=True, neigen=1, lanc_dim_threshold=128,
+                       lanc_tol=1e-12, seed=w["seed"], precision=precision, verbose=verbose, pbc=True)
+    d = N.DMRG(ctx, cfg)
+    ctx.hv_stats(reset=True)
+    t0 = time.perf_counter()
+    steps = d.run("i")
+    wall = time.perf_counter() - t0
+    return d, steps, wall, ctx.hv_stats(reset=True)
+
+
+def main_pbc(N, ctx, w, a):
+    print(f"[bench] workload {a.workload}: PBC iDMRG growth, FP64 then FP32-MFMA", flush=True)
+    d64, st64, wall64, s = run_pbc(N, ctx, w, "fp64", a.verbose)
+    hv_time = s["t_w"] + s["t_y"]
+    calls = max(1, s["calls"])
+    y_avg_s, y_flops = s["t_y"] / calls, s["flops_y"] / calls
+    at_L = [x for x in st64 if x["L"] == w["L"]]
+    out = {
+        "metric": "iDMRG growth wall time + H·v GFLOP/s under PBC, FP64 vs FP32-MFMA (|ΔE0/L|)",
+        "value": s["flops"] / hv_time / 1e9 if hv_time > 0 else 0.0, "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": len(st64), "warmup": 0, "ms_per_step": 1e3 * wall64 / len(st64), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": a.workload, "model": "hubbard", "bc": "pbc", "L": w["L"], "ldmrg": w["L"] // 2,
+                   "ldmrg_used": st64[-1]["left_len"], "final_ring": st64[-1]["L"], "m": w["m"], "U": w["U"],
+                   "hfmode": True, "neigen": 1, "lanc_tol": 1e-12, "seed": w["seed"],
+                   "step": "one iDMRG step (finite sweeps are refused under PBC, src/DMRG_MAIN.f90:179-180)"},
+        "roofline": {"bound": "mfma", "kernel": "k_hv_y_f64", "achieved": y_flops / y_avg_s / 1e12,
+                     "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s", "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA,
+                     "traffic": None, "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops},
+        "growth_wall_s": wall64, "hv_calls": s["calls"], "hv_gpu_s": hv_time,
+        "e0_per_site_L": at_L[0]["e_per_site"][0] if at_L else None,
+        "max_m_sb": int(max(x["m_sb"] for x in st64)), "max_nsect": int(max(x["nsect"] for x in st64)),
+        "time_split_s": {k: float(sum(x[k] for x in st64)) for k in
+                         ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
+    }
+    print(f"[bench] fp64 done: {wall64:.1f} s growth, E0/L at L={w['L']} {out['e0_per_site_L']}", flush=True)
+    d64.close()
+    if not a.no_mixed:
+        d32, st32, wall32, s32 = run_pbc(N, ctx, w, "mixed", a.verbose)
+        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(st64, st32))
+        t32 = s32["t_w32"] + s32["t_y32"]
+        out["fp32_mixed"] = {"growth_wall_s": wall32, "hv_fp32_gflops": s32["flops32"] / t32 / 1e9 if t32 > 0 else 0.0,
+                             "hv_fp32_calls": s32["calls32"], "hv_fp64_refine_calls": s32["calls"],
+                             "max_abs_dE0_per_site_vs_fp64": de}
+        print(f"[bench] mixed done: {wall32:.1f} s growth, max|dE0/L| vs FP64 {de:.2e}", flush=True)
+        d32.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +347,9 @@ def main():
     ctx = N.Context(0)
     if "batch" in w:
         main_batch(N, ctx, w, a)
+        return
+    if w.get("pbc"):
+        main_pbc(N, ctx, w, a)
         return
     print(f"[bench] workload {a.workload}: FP64 path, {a.warmup} warmup + {a.steps} timed sweeps", flush=True)
     d64, r64 = run_path(N, ctx, w, a.warmup, a.steps, "fp64", a.verbose)

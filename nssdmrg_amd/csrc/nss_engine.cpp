@@ -1347,6 +1347,19 @@ struct Engine {
     if (S.dense) {
       S.lo = dense_diag(ctx, sbd(), S.neig, psi.p);
       I.numop = 0;
+    } else if (S.neig == 1 && par.precision == NSS_FP64 && S.msb >= lz_pro_min_n()) {
+      // large superblocks: Lanczos with partial reorthogonalisation (a batch
+      // of one chain; the thick-restart solver past kmax), where the full
+      // Gram-Schmidt pass would cost more HBM traffic than H*v itself
+      std::vector<LanczosJob> jobs(1);
+      jobs[0].sb = sbd();
+      jobs[0].xdev = psi.p;
+      jobs[0].v0_mode = S.v0;
+      jobs[0].seed = par.seed;
+      lanczos_batch(ctx, jobs, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol);
+      S.lo = jobs[0].out;
+      I.numop = S.lo.numop;
+      I.numop_fp32 = 0;
     } else {
       S.lo = lanczos_device(ctx, sbd(), S.neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed, S.v0,
                             par.precision, psi.p);

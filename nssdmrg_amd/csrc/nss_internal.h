@@ -344,6 +344,11 @@ struct LanczosJob {
 // Lowest eigenpair of every job, the H*v of all chains in one grouped launch
 // per phase (nss_lanczos.cpp).
 void lanczos_batch(::nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax, int maxiter, double tol);
+// Superblock dimension from which a single engine's FP64 lowest-eigenpair
+// solve uses lanczos_batch (partial reorthogonalisation) instead of the
+// thick-restart solver with a Gram-Schmidt pass every iteration
+// (NSS_LZ_PRO_MIN_N; 0 for every size).
+int64_t lz_pro_min_n();
 // Dense path: H_sb assembled on the device from unit vectors, diagonalised on the host.
 LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev);
 

@@ -296,6 +296,14 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   return out;
 }
 
+int64_t lz_pro_min_n() {
+  static const int64_t v = [] {
+    const char* e = std::getenv("NSS_LZ_PRO_MIN_N");
+    return e ? (int64_t)std::max(0LL, std::atoll(e)) : (int64_t)1 << 17;
+  }();
+  return v;
+}
+
 // Full Gram-Schmidt pass in every batched iteration (NSS_LZB_FULL_REORTH=1)
 // instead of partial reorthogonalisation.
 static int lzb_full_reorth() {

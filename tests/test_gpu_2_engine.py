@@ -442,3 +442,19 @@ def test_pbc_batch_matches_single_runs(ctx):
             assert a["link"] == b["link"] and a["m_sb"] == b["m_sb"]
             assert abs(a["energies"][0] - b["energies"][0]) < 1e-10 * max(1.0, abs(b["energies"][0]))
             assert abs(a["entropy_left"] - b["entropy_left"]) < 1e-8
+
+
+def test_large_superblocks_partial_reorth_free_fermions(ctx):
+    # single-engine FP64 steps with >= 2^17 superblock states use the Lanczos
+    # with partial reorthogonalisation; U=0 finite sweeps must reach the
+    # free-fermion ground energy of the open chain
+    print("case large superblocks, partial reorthogonalisation", flush=True)
+    L, M = 16, 256
+    d, st = run(ctx, "f", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=(L - 2) // 2, mdmrg=M,
+                msweep=[M, M], target_qn=(0.5, 0.5), neigen=1, lanc_dim_threshold=128, uloc=0.0, hfmode=True,
+                predict=True)
+    assert max(s["m_sb"] for s in st) >= 1 << 17
+    e = O.free_fermion_obc(L, L // 2, L // 2)
+    fin = [s for s in st if s["type"] == "f" and s["L"] == L]
+    assert abs(fin[-1]["energies"][0] - e) < 1e-8 * abs(e)
+    assert all(abs(s["energies"][0] - e) < 1e-6 * abs(e) for s in fin[len(fin) // 2:])
