@@ -4,7 +4,8 @@
 !> energies (tests/golden/spin12_L40_M20_energy.check, copied data).
 !> Part 2 binds spMatVec_gpu_main to a procedure pointer of the reference's
 !> abstract interface sparse_HxV and checks it against the direct-product
-!> loops of spMatVec_direct_main written out here on the host.
+!> loops of spMatVec_direct_main written out here on the host.  Part 3 runs
+!> PBCdmrg=T on the spin-1/2 Heisenberg rings of 4 and 6 sites.
 program spin_chain_golden
   use iso_c_binding
   use nss_dmrg
@@ -81,10 +82,41 @@ program spin_chain_golden
 
   ! ---------------------------------------------------------------- part 2
   call check_drop_in()
+
+  ! ---------------------------------------------------------------- part 3
+  call check_pbc_ring()
   call nss_gpu_finalize()
   write(*, "(A)") "PASS"
 
 contains
+
+  !> PBCdmrg=T, LDMRG=3, untruncated: a link-'n' step (ring of 4 sites)
+  !> then a link-'p' step (ring of 6) against the exact ring energies -2 and
+  !> -2.8027756377319946.
+  subroutine check_pbc_ring()
+    type(nss_model) :: m
+    type(nss_params) :: p
+    type(nss_step_info) :: s1, s2
+    m%model = NSS_MODEL_SPIN12
+    p%ldmrg = 3
+    p%mdmrg = 200
+    p%neigen = 1
+    p%lanc_dim_threshold = 8
+    p%lanc_tol = 1d-12
+    p%pbc = 1
+    call init_dmrg_gpu(Hij, m, p)
+    call run_dmrg_gpu('i')
+    s1 = dmrg_gpu_step(1)
+    s2 = dmrg_gpu_step(2)
+    write(*, "(A,2F20.15)") "PBC rings E(4), E(6):", s1%energies(1), s2%energies(1)
+    if (dmrg_gpu_nsteps() /= 2 .or. s1%L /= 4 .or. s2%L /= 6 .or. s1%link /= ichar('n') .or. &
+        s2%link /= ichar('p') .or. abs(s1%energies(1) + 2d0) > 1d-11 .or. &
+        abs(s2%energies(1) + 2.8027756377319946d0) > 1d-11) then
+       write(*, "(A)") "FAIL PBC rings"
+       stop 1
+    endif
+    call finalize_dmrg_gpu()
+  end subroutine check_pbc_ring
 
   integer function dmrg_steps_checked()
     dmrg_steps_checked = 39

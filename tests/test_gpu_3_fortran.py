@@ -3,7 +3,9 @@ the driver tests/fortran/spin_chain_golden.f90 runs the reference's spin-1/2
 L40 M20 iDMRG through init_dmrg_gpu/run_dmrg_gpu and checks the golden
 energies (1e-10) and the golden S_z profile through dmrg_gpu_measure (1e-6),
 then binds spMatVec_gpu_main to a sparse_HxV procedure
-pointer and checks it against the direct-product loops."""
+pointer and checks it against the direct-product loops, and finally runs
+PBCdmrg=T on the spin-1/2 Heisenberg rings of 4 and 6 sites against their
+exact energies."""
 import os
 import subprocess
 
