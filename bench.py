@@ -300,7 +300,9 @@ def main_pbc(N, ctx, w, a):
                    "step": "one iDMRG step (finite sweeps are refused under PBC, src/DMRG_MAIN.f90:179-180)"},
         "roofline": {"bound": "mfma", "kernel": "k_hv_y_f64", "achieved": y_flops / y_avg_s / 1e12,
                      "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s", "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA,
-                     "traffic": None, "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops},
+                     "traffic": None, "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
+                     "w_kernel": {"name": "k_hv_w_f64", "avg_launch_us": 1e6 * s["t_w"] / calls,
+                                  "flops_per_launch": s["flops_w"] / calls}},
         "growth_wall_s": wall64, "hv_calls": s["calls"], "hv_gpu_s": hv_time,
         "e0_per_site_L": at_L[0]["e_per_site"][0] if at_L else None,
         "max_m_sb": int(max(x["m_sb"] for x in st64)), "max_nsect": int(max(x["nsect"] for x in st64)),
