@@ -262,7 +262,11 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
   return t < ntiles ? t : -1;
 }
 
-template <typename T, int NBUF>
+// Fragment buffers of the 32x32 kernel (three measured slower: the chains
+// of these launches are a few chunks per wave).
+#define NSS_NBUF 2
+
+template <typename T, int NBUF, bool SKIPLOAD>
 __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
                                             int ntiles, const Bases& bs) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -322,14 +326,18 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
     return true;
   };
   // NBUF fragment buffers in rotation, each refilled right after its chunk
-  // has gone to the MFMA pipe, so loads run NBUF-1 chunks ahead (two
-  // buffers measured fastest: the chains of small sectors are only a few
-  // chunks long per wave, and every buffer costs a full round of loads).  Every
-  // refill issues its loads (an exhausted buffer re-reads valid addresses
-  // with alpha = 0), which keeps the loop straight-line with a single exit:
-  // the accumulators stay in place and each buffer waits only on its own
-  // loads.  Buffers take the wave's chunks in order, so once buffer 0 finds
-  // the chain exhausted every later buffer does too.
+  // has gone to the MFMA pipe, so loads run NBUF-1 chunks ahead.  The MFMAs
+  // of an exhausted buffer are skipped (a wave-uniform branch).  Buffers take
+  // the wave's chunks in order, so once buffer 0 has found the chain
+  // exhausted, the chunks still held by the others are computed in the same
+  // round.  Refills of exhausted buffers:
+  //  - SKIPLOAD = false: issued anyway (they re-read valid clamped
+  //    addresses), which keeps the load order the same on every path, so the
+  //    wait before each buffer's MFMAs leaves the later buffers' loads in
+  //    flight.  For long K chains (Y).
+  //  - SKIPLOAD = true: skipped, which spares short chains (W: one or two
+  //    chunks per wave) a last round of loads; the compiler's waits then
+  //    drain every load in flight before each buffer's MFMAs.
   int m[NBUF][2];
   T al[NBUF];
 #pragma unroll
@@ -340,22 +348,22 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   }
   if (advance(m[0], al[0])) {
     Frag32<T> f[NBUF];
+    bool live[NBUF];
+    live[0] = true;
     load_frag32<T>(v, m[0][0], fk, f[0]);
 #pragma unroll
     for (int b = 1; b < NBUF; ++b) {
-      advance(m[b], al[b]);
-      load_frag32<T>(v, m[b][0], fk, f[b]);
+      live[b] = advance(m[b], al[b]);
+      if (!SKIPLOAD || live[b]) load_frag32<T>(v, m[b][0], fk, f[b]);
     }
-    bool h0;
     do {
 #pragma unroll
       for (int b = 0; b < NBUF; ++b) {
-        mma_chunk32<T>(f[b], m[b][0], m[b][1], al[b], fk, acc);
-        const bool hb = advance(m[b], al[b]);
-        if (b == 0) h0 = hb;
-        load_frag32<T>(v, m[b][0], fk, f[b]);
+        if (live[b]) mma_chunk32<T>(f[b], m[b][0], m[b][1], al[b], fk, acc);
+        live[b] = advance(m[b], al[b]);
+        if (!SKIPLOAD || live[b]) load_frag32<T>(v, m[b][0], fk, f[b]);
       }
-    } while (h0);
+    } while (live[0]);
   }
 
 #pragma unroll
@@ -388,10 +396,10 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
 // phases (W: Psi.B^T per term, Y: per-sector accumulation) and the generic
 // grouped GEMM (rho, rotations); the _t64 symbols are the 64x64 LDS-staged
 // variant of each.
-#define NSS_GEMM_KERNEL(name, T)                                                                        \
+#define NSS_GEMM_KERNEL(name, T, SKIPLOAD)                                                              \
   __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,                       \
                                               const GemmSeg* __restrict__ segs, int ntiles, Bases bs) { \
-    gemm_tile32<T, 2>(tiles, segs, ntiles, bs);                                                         \
+    gemm_tile32<T, NSS_NBUF, SKIPLOAD>(tiles, segs, ntiles, bs);                                        \
   }                                                                                                     \
   __global__ __launch_bounds__(256) void name##_t64(const GemmTile* __restrict__ tiles,                 \
                                                     const GemmSeg* __restrict__ segs, int ntiles,       \
@@ -399,12 +407,14 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
     const int t = xcd_tile(ntiles);                                                                     \
     if (t >= 0) gemm_tile<T>(tiles + t, segs, bs);                                                      \
   }
-NSS_GEMM_KERNEL(k_hv_w_f64, double)
-NSS_GEMM_KERNEL(k_hv_y_f64, double)
+// W and the generic GEMM (rho, rotations) have short K chains; Y's chain
+// runs over every term of its sector.
+NSS_GEMM_KERNEL(k_hv_w_f64, double, true)
+NSS_GEMM_KERNEL(k_hv_y_f64, double, false)
 
-NSS_GEMM_KERNEL(k_gemm_f64, double)
-NSS_GEMM_KERNEL(k_hv_w_f32, float)
-NSS_GEMM_KERNEL(k_hv_y_f32, float)
+NSS_GEMM_KERNEL(k_gemm_f64, double, true)
+NSS_GEMM_KERNEL(k_hv_w_f32, float, true)
+NSS_GEMM_KERNEL(k_hv_y_f32, float, false)
 
 typedef void (*gemm_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases);
 
