@@ -589,6 +589,7 @@ __global__ __launch_bounds__(256) void k_lz_alpha(double* __restrict__ w, const 
   const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
   const double b = beta_prev ? beta_prev[0] : 0.0;
   double acc = 0.0;
+#pragma unroll 8
   for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
     double x = w[i];
     if (beta_prev) {
@@ -626,10 +627,13 @@ __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, 
   const double* v = V + (size_t)j * ldv;
   const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
   double acc = 0.0;
-  if (apart)
+  if (apart) {
+#pragma unroll 8
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i]);
-  else
+  } else {
+#pragma unroll 8
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * w[i];
+  }
   const double r = block_sum256(acc, sh);
   if (threadIdx.x == 0) partial[(size_t)j * nch + c] = r;
 }
@@ -671,13 +675,17 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
   __syncthreads();
   double ss = 0.0;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    // eight basis vectors per step, their loads issued together
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int j = 0;
-    for (; j + 4 <= m; j += 4) {
-      a0 += V[(size_t)j * ldv + i] * hs[j];
-      a1 += V[(size_t)(j + 1) * ldv + i] * hs[j + 1];
-      a2 += V[(size_t)(j + 2) * ldv + i] * hs[j + 2];
-      a3 += V[(size_t)(j + 3) * ldv + i] * hs[j + 3];
+    for (; j + 8 <= m; j += 8) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = V[(size_t)(j + u) * ldv + i];
+      a0 += x[0] * hs[j] + x[4] * hs[j + 4];
+      a1 += x[1] * hs[j + 1] + x[5] * hs[j + 5];
+      a2 += x[2] * hs[j + 2] + x[6] * hs[j + 6];
+      a3 += x[3] * hs[j + 3] + x[7] * hs[j + 7];
     }
     for (; j < m; ++j) a0 += V[(size_t)j * ldv + i] * hs[j];
     const double wi = apart ? w[i] - a * va[i] : w[i];
