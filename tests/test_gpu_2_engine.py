@@ -458,3 +458,39 @@ def test_large_superblocks_partial_reorth_free_fermions(ctx):
     fin = [s for s in st if s["type"] == "f" and s["L"] == L]
     assert abs(fin[-1]["energies"][0] - e) < 1e-8 * abs(e)
     assert all(abs(s["energies"][0] - e) < 1e-6 * abs(e) for s in fin[len(fin) // 2:])
+
+
+@pytest.mark.parametrize("model", ["spin12", "hubbard"])
+def test_pbc_restart_files_carry_both_ends(ctx, tmp_path, model):
+    # write_block under PBC writes the operators of both block ends (okey's
+    # _n and _p keys); a PBC run reads them back and continues; a block file
+    # of an OBC run lacks the outer-end operators and a PBC run refuses it
+    print(f"case pbc restart {model}", flush=True)
+    if model == "spin12":
+        kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=5, mdmrg=16, target_qn=(0.0, 0.0), neigen=1,
+                  lanc_dim_threshold=64)
+        keys = ["S_z_n", "S_p_n", "S_z_p", "S_p_p"]
+    else:
+        kw = dict(site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=5, mdmrg=24, target_qn=(0.5, 0.5),
+                  neigen=1, lanc_dim_threshold=64, uloc=3.0, hfmode=True)
+        keys = ["C_l1_s1_n", "C_l1_s2_n", "P_n", "C_l1_s1_p", "C_l1_s2_p", "P_p"]
+    d1, st1 = run(ctx, "i", pbc=True, **kw)
+    lp, rp = str(tmp_path / "pbc_left.restart"), str(tmp_path / "pbc_right.restart")
+    d1.save_blocks(lp, rp)
+    for path in (lp, rp):
+        toks = open(path).read().split()
+        for k in keys:
+            assert k in toks, (path, k)
+    d2 = N.DMRG(ctx, N.DMRGConfig(pbc=True, **kw))
+    d2.load_blocks(lp, rp)
+    st2 = d2.run("i")
+    # the saved blocks entered the last (link-'p') step; the resumed run
+    # starts a new iteration from them, as the reference's loop does
+    assert [s["link"] for s in st2] == ["n", "p"] and st2[0]["left_len"] == st1[-1]["left_len"]
+    assert all(np.isfinite(s["energies"][0]) for s in st2)
+    d3, _ = run(ctx, "i", **kw)
+    op, orp = str(tmp_path / "obc_left.restart"), str(tmp_path / "obc_right.restart")
+    d3.save_blocks(op, orp)
+    d4 = N.DMRG(ctx, N.DMRGConfig(pbc=True, **kw))
+    with pytest.raises(N.NssError, match="operator missing"):
+        d4.load_blocks(op, orp)
