@@ -114,6 +114,9 @@ struct Bases {
   const void* x;
   void* w;
   void* y;
+  // H*v Y plans only (may be null): dot[4t + q] = sum over wave q's share
+  // of output tile t of y . x at the same offsets (partial sums of x^T H x)
+  double* dot = nullptr;
 };
 
 // Forced tile edge (NSS_GEMM_TILE = 32 or 64), 0 for the size-based choice.
@@ -171,9 +174,13 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
 // partials in apart, stored to aout), writing that difference minus V h back
 // to w: the three-term step's last subtraction folded into the pass.  No
 // host synchronisation.
+// With apart, the pass works on w - alpha v_a (- beta v_b with vb and bprev:
+// beta = bprev[0], written to bout by block 0), alpha the sum of the na
+// partials in apart (na < 0: lz_nch(n) of them), written to aout.
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
                       double* hout, double* sq, hipStream_t s, const double* apart = nullptr,
-                      const double* va = nullptr, double* aout = nullptr);
+                      const double* va = nullptr, double* aout = nullptr, int64_t na = -1,
+                      const double* vb = nullptr, const double* bprev = nullptr, double* bout = nullptr);
 int orth_blocks(int64_t n);
 int64_t orth_chunk(int64_t n);
 // First half of the Lanczos three-term step on the device: w -= beta_prev
@@ -349,6 +356,9 @@ void lanczos_batch(::nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax, int 
 // thick-restart solver with a Gram-Schmidt pass every iteration
 // (NSS_LZ_PRO_MIN_N; 0 for every size).
 int64_t lz_pro_min_n();
+// FP64 thick-restart iterations take alpha_j from the Y launch's partial
+// sums of v_j . H v_j (NSS_LZ_ALPHA_FROM_Y; 0 for a separate dot pass).
+bool lz_alpha_from_y();
 // Dense path: H_sb assembled on the device from unit vectors, diagonalised on the host.
 LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev);
 
@@ -374,6 +384,12 @@ void sb_set_layout(SbDevice* sb, const std::vector<int>& Dl, const std::vector<i
 void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL,
                   const std::vector<OpRef>& HR, const std::vector<SbTerm>& terms, bool want_fp32);
 void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);          // device, async
+// sb_hv that also leaves, for wave q of every output tile t of the Y launch,
+// the partial sum dot[4t + q] of y . x over that wave's share of the tile
+// (4 x sb_hv_ntiles of them; summed in a fixed order they give x^T H x
+// deterministically).
+void sb_hv_dot(nss_ctx* ctx, SbDevice* sb, const double* x, double* y, double* dot);
+int64_t sb_hv_ntiles(const SbDevice* sb);
 void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);     // device, async
 // Batch H*v: merged W and Y plans of several superblocks whose vectors sit at
 // xoff[c] of shared X / Y arenas and whose term scratch sits at woff[c] of a

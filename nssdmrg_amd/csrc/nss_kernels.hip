@@ -45,6 +45,8 @@ __device__ __forceinline__ const T* resolve_c(int base, const void* abs, int64_t
   }
 }
 
+__device__ __forceinline__ double wave_sum(double v);
+
 template <typename T>
 struct SliceRegs {  // one K-slice of A and B held in registers while the previous slice computes
   T a[8], b[8];
@@ -82,7 +84,7 @@ __device__ __forceinline__ void load_slice(const GemmSeg& g, const GemmTile& tl,
 
 template <typename T>
 __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
-                                          const Bases& bs) {
+                                          const Bases& bs, int tix) {
   constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = 32;
   constexpr int PAD = 1;
   __shared__ T As[BK][BM + PAD];
@@ -154,6 +156,8 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
     k0 = nk0;
   }
   T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
+  const T* X = static_cast<const T*>(bs.x) + tl.coff;  // read only with bs.dot (Y and x share offsets)
+  double dsum = 0.0;
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -166,9 +170,15 @@ __device__ __forceinline__ void gemm_tile(const GemmTile* __restrict__ tiles, co
         const int col = tl.col0 + wn + ni * 16 + fr;
         if (row < tl.M && col < tl.N) {
           T* p = C + (size_t)row * tl.ldc + col;
-          *p = tl.beta ? *p + acc[mi][ni][q] : acc[mi][ni][q];
+          const T out = tl.beta ? *p + acc[mi][ni][q] : acc[mi][ni][q];
+          *p = out;
+          if (bs.dot) dsum += (double)out * (double)X[(size_t)row * tl.ldc + col];
         }
       }
+  if (bs.dot) {  // each wave's share of x^T y (no block barrier)
+    const double r = wave_sum(dsum);
+    if ((threadIdx.x & 63) == 0) bs.dot[4 * tix + (threadIdx.x >> 6)] = r;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -379,6 +389,8 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
       }
   __syncthreads();
   T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
+  const T* X = static_cast<const T*>(bs.x) + tl.coff;  // read only with bs.dot (Y and x share offsets)
+  double dsum = 0.0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int e = threadIdx.x + 256 * u;
@@ -387,8 +399,14 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
     if (gi < tl.M && gj < tl.N) {
       const T v = (red[0][row][col] + red[1][row][col]) + (red[2][row][col] + red[3][row][col]);
       T* p = C + (size_t)gi * tl.ldc + gj;
-      *p = tl.beta ? *p + v : v;
+      const T out = tl.beta ? *p + v : v;
+      *p = out;
+      if (bs.dot) dsum += (double)out * (double)X[(size_t)gi * tl.ldc + gj];
     }
+  }
+  if (bs.dot) {  // each wave's share of x^T y (no block barrier)
+    const double r = wave_sum(dsum);
+    if ((threadIdx.x & 63) == 0) bs.dot[4 * t + (threadIdx.x >> 6)] = r;
   }
 }
 
@@ -405,7 +423,7 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
                                                     const GemmSeg* __restrict__ segs, int ntiles,       \
                                                     Bases bs) {                                         \
     const int t = xcd_tile(ntiles);                                                                     \
-    if (t >= 0) gemm_tile<T>(tiles + t, segs, bs);                                                      \
+    if (t >= 0) gemm_tile<T>(tiles + t, segs, bs, t);                                                   \
   }
 // W and the generic GEMM (rho, rotations) have short K chains; Y's chain
 // runs over every term of its sector.
@@ -605,21 +623,22 @@ __global__ __launch_bounds__(256) void k_lz_alpha(double* __restrict__ w, const 
   }
 }
 
-// partial[j*nch + c] = <V_j, w - alpha v_a> over chunk c, alpha = the sum of
-// apart[0..nch) (every block reduces it in the same fixed order); with
-// apart == null, <V_j, w>
+// partial[j*nch + c] = <V_j, w - alpha v_a - beta v_b> over chunk c, alpha =
+// the sum of apart[0..na) (every block reduces it in the same fixed order),
+// beta = bprev[0] (v_b and bprev may be null); with apart == null, <V_j, w>
 __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv,
                                                   const double* __restrict__ w, int64_t n,
                                                   double* __restrict__ partial, int64_t nch,
-                                                  const double* __restrict__ apart,
-                                                  const double* __restrict__ va) {
+                                                  const double* __restrict__ apart, int64_t na,
+                                                  const double* __restrict__ va, const double* __restrict__ vb,
+                                                  const double* __restrict__ bprev) {
   __shared__ double sh[4];
   __shared__ double ash;
   const int64_t c = blockIdx.x;
   const int j = blockIdx.y;
   double a = 0.0;
   if (apart) {
-    const double a0 = coop_sum(apart, nch, sh);
+    const double a0 = coop_sum(apart, na, sh);
     if (threadIdx.x == 0) ash = a0;
     __syncthreads();
     a = ash;
@@ -627,7 +646,11 @@ __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, 
   const double* v = V + (size_t)j * ldv;
   const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
   double acc = 0.0;
-  if (apart) {
+  if (apart && vb) {
+    const double b = bprev[0];
+#pragma unroll 8
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i] - b * vb[i]);
+  } else if (apart) {
 #pragma unroll 8
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i]);
   } else {
@@ -645,21 +668,25 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
                                                      const double* __restrict__ partial, int64_t nch,
                                                      double* __restrict__ h, double* __restrict__ w,
                                                      int64_t n, double* __restrict__ sq,
-                                                     const double* __restrict__ apart,
-                                                     const double* __restrict__ va, double* __restrict__ aout) {
+                                                     const double* __restrict__ apart, int64_t na,
+                                                     const double* __restrict__ va, double* __restrict__ aout,
+                                                     const double* __restrict__ vb,
+                                                     const double* __restrict__ bprev, double* __restrict__ bout) {
   extern __shared__ double hs[];
   __shared__ double red[4];
   __shared__ double ash;
   double a = 0.0;
-  if (apart) {  // alpha as k_lz_gemvt reduces it; w' = w - alpha v_a on the fly
-    const double a0 = coop_sum(apart, nch, red);
+  if (apart) {  // alpha as k_lz_gemvt reduces it; w' = w - alpha v_a - beta v_b on the fly
+    const double a0 = coop_sum(apart, na, red);
     if (threadIdx.x == 0) {
       ash = a0;
       if (blockIdx.x == 0 && aout) aout[0] = a0;
+      if (blockIdx.x == 0 && bout) bout[0] = vb ? bprev[0] : 0.0;
     }
     __syncthreads();
     a = ash;
   }
+  const double b = vb ? bprev[0] : 0.0;
   for (int j = threadIdx.x; j < m; j += 256) {
     const double* p = partial + (size_t)j * nch;
     double a0 = 0.0, a1 = 0.0;
@@ -688,7 +715,7 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
       a3 += x[3] * hs[j + 3] + x[7] * hs[j + 7];
     }
     for (; j < m; ++j) a0 += V[(size_t)j * ldv + i] * hs[j];
-    const double wi = apart ? w[i] - a * va[i] : w[i];
+    const double wi = apart ? (vb ? w[i] - a * va[i] - b * vb[i] : w[i] - a * va[i]) : w[i];
     const double x = wi - ((a0 + a1) + (a2 + a3));
     w[i] = x;
     ss += x * x;
@@ -725,12 +752,13 @@ void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int6
 
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
                       double* hout, double* sq, hipStream_t s, const double* apart, const double* va,
-                      double* aout) {
+                      double* aout, int64_t na, const double* vb, const double* bprev, double* bout) {
   const int64_t nch = lz_nch(n);
+  if (na < 0) na = nch;
   hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch,
-                     apart, va);
+                     apart, na, va, vb, bprev);
   hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
-                     ldv, m, partial, nch, hout, w, n, sq, apart, va, aout);
+                     ldv, m, partial, nch, hout, w, n, sq, apart, na, va, aout, vb, bprev, bout);
   NSS_HIP(hipGetLastError());
 }
 

@@ -5,7 +5,9 @@
 // with full re-orthogonalisation (a classical Gram-Schmidt pass against the
 // whole basis every iteration).  The
 // Krylov basis lives in HBM.  An iteration is a fixed sequence of launches
-// on the context stream (W and Y of H*v, one fused Gram-Schmidt pass, one
+// on the context stream (W and Y of H*v, Y also leaving the partial sums of
+// alpha_j = v_j^T H v_j per wave of each output tile; one Gram-Schmidt pass that
+// subtracts alpha_j v_j and beta_{j-1} v_{j-1} on the fly; one
 // normalisation) that writes its projection coefficients and beta into a
 // per-iteration slot in device memory; the host does not wait for it.  The
 // host reads the slots back only at convergence checks, whose spacing
@@ -34,6 +36,8 @@ struct Lz {
   DevArr<double> part;   // Gram-Schmidt partials
   DevArr<double> sq;     // per-block sums of squares
   DevArr<double> scal;   // scratch scalars
+  DevArr<double> adot;   // FP64 H*v: the Y launch's partials of alpha_j, one per wave of each tile
+  int64_t nyt = 0;       // their count (4 x the Y plan's tiles)
   std::vector<double> T;  // (kmax+1)^2 host projected matrix
   int numop = 0, numop32 = 0;
 
@@ -49,20 +53,29 @@ struct Lz {
   // matrix entries are h[i] + slot[kmax+1+i].
   void iterate(int j, bool f32) {
     hipStream_t s = ctx->stream;
-    if (f32) {
-      sb_hv_fp32(ctx, sb, vec(j), w.p);
-      ++numop32;
-    } else {
-      sb_hv(ctx, sb, vec(j), w.p);
-      ++numop;
-    }
     double* slot = slots.p + (size_t)j * S;
     const bool has_prev = j > restart_at;
-    double* apart = part.p + (size_t)(kmax + 1) * lz_nch(n);  // alpha partials, past the pass's partials
-    launch_lz_recurrence(w.p, has_prev ? vec(j - 1) : nullptr, vec(j), n,
-                         has_prev ? slots.p + (size_t)(j - 1) * S + 2 * (kmax + 1) : nullptr,
-                         has_prev ? slot + (kmax + 1) + (j - 1) : nullptr, apart, s);
-    launch_orth_pass(V.p, n, j + 1, w.p, n, part.p, slot, sq.p, s, apart, vec(j), slot + (kmax + 1) + j);
+    const double* bprev = has_prev ? slots.p + (size_t)(j - 1) * S + 2 * (kmax + 1) : nullptr;
+    double* bout = has_prev ? slot + (kmax + 1) + (j - 1) : nullptr;
+    if (!f32 && nyt > 0) {
+      // alpha_j = v_j^T H v_j from the Y launch's per-tile partials; the
+      // Gram-Schmidt pass subtracts alpha_j v_j and beta_{j-1} v_{j-1} on the fly
+      sb_hv_dot(ctx, sb, vec(j), w.p, adot.p);
+      ++numop;
+      launch_orth_pass(V.p, n, j + 1, w.p, n, part.p, slot, sq.p, s, adot.p, vec(j), slot + (kmax + 1) + j, nyt,
+                       has_prev ? vec(j - 1) : nullptr, bprev, bout);
+    } else {
+      if (f32) {
+        sb_hv_fp32(ctx, sb, vec(j), w.p);
+        ++numop32;
+      } else {
+        sb_hv(ctx, sb, vec(j), w.p);
+        ++numop;
+      }
+      double* apart = part.p + (size_t)(kmax + 1) * lz_nch(n);  // alpha partials, past the pass's partials
+      launch_lz_recurrence(w.p, has_prev ? vec(j - 1) : nullptr, vec(j), n, bprev, bout, apart, s);
+      launch_orth_pass(V.p, n, j + 1, w.p, n, part.p, slot, sq.p, s, apart, vec(j), slot + (kmax + 1) + j);
+    }
     launch_normalize(w.p, vec(j + 1), n, sq.p, slot + 2 * (kmax + 1), s);
   }
 };
@@ -255,6 +268,8 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   L.part.resize(ctx->pool, std::max<int64_t>((int64_t)(kmax + 2) * nch, gemv_t_chunks(n)) + 1);
   L.sq.resize(ctx->pool, orth_blocks(n) + 1);
   L.scal.resize(ctx->pool, 4);
+  L.nyt = lz_alpha_from_y() ? 4 * sb_hv_ntiles(sb) : 0;
+  L.adot.resize(ctx->pool, std::max<int64_t>(1, L.nyt));
   DevArr<double> v0;
   v0.resize(ctx->pool, n);
   bool filled = false;
@@ -294,6 +309,16 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   out.numop = L.numop;
   out.numop_fp32 = L.numop32;
   return out;
+}
+
+// alpha_j from the Y launch's partials (NSS_LZ_ALPHA_FROM_Y=0: a separate
+// dot-product pass over H v_j, as the FP32 iterations do)
+bool lz_alpha_from_y() {
+  static const bool v = [] {
+    const char* e = std::getenv("NSS_LZ_ALPHA_FROM_Y");
+    return !e || std::atoi(e) != 0;
+  }();
+  return v;
 }
 
 int64_t lz_pro_min_n() {

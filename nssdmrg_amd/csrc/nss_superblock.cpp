@@ -308,14 +308,17 @@ static hipEvent_t next_event(nss_ctx* ctx) {
 // Timed H*v: each launch records its own start and end (4 events per call),
 // so the statistics are the kernels' durations and no marker packet sits
 // between them.
-static void hv_timed(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b, int kind) {
+static void hv_timed(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b, int kind,
+                     double* ydot = nullptr) {
   const bool t = ctx->time_hv;
   hipEvent_t e[4] = {};
   if (t)
     for (hipEvent_t& x : e) x = next_event(ctx);
   if (kind == 0) {
     launch_gemm_f64(W, b, ctx->stream, GEMM_HV_W, e[0], e[1]);
-    launch_gemm_f64(Y, b, ctx->stream, GEMM_HV_Y, e[2], e[3]);
+    Bases by = b;
+    by.dot = ydot;  // Y's per-tile x . y partials
+    launch_gemm_f64(Y, by, ctx->stream, GEMM_HV_Y, e[2], e[3]);
   } else {
     launch_gemm_f32(W, b, ctx->stream, GEMM_HV_W, e[0], e[1]);
     launch_gemm_f32(Y, b, ctx->stream, GEMM_HV_Y, e[2], e[3]);
@@ -339,6 +342,12 @@ static void hv_timed(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b
 void sb_hv(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
   hv_timed(ctx, sb->W64, sb->Y64, Bases{x, sb->W.p, y}, 0);
 }
+
+void sb_hv_dot(nss_ctx* ctx, SbDevice* sb, const double* x, double* y, double* dot) {
+  hv_timed(ctx, sb->W64, sb->Y64, Bases{x, sb->W.p, y}, 0, dot);
+}
+
+int64_t sb_hv_ntiles(const SbDevice* sb) { return (int64_t)sb->Y64.htiles.size(); }
 
 void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y) {
   NSS_REQUIRE(sb->has32, NSS_E_STATE, "FP32 H*v requested but blocks were uploaded for FP64 only");
