@@ -76,11 +76,14 @@ int nss_sb_upload_blocks(nss_ctx* ctx, const double* const* HL, const double* co
 int nss_sb_hv(nss_ctx* ctx, const double* v, double* hv, int64_t n);
 /* Same on device pointers, enqueued on the context stream (no host sync). */
 int nss_sb_hv_device(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n);
-/* H*v timing recorded with HIP events on the context stream (every H*v of
- * nss_sb_hv*, nss_lanczos and the engine).  out[12]: for FP64 (out[0..5])
- * then FP32 (out[6..11]): {calls, seconds in the W launch (Psi.B^T per term),
- * seconds in the Y launch (per-sector sums), algorithmic FLOPs of the W
- * launches, FLOPs of the Y launches, 0}; reset clears the counters. */
+/* H*v timing recorded with HIP events on the context stream, on every k-th
+ * H*v of nss_sb_hv*, nss_lanczos and the engine (k from the environment
+ * variable NSS_HV_TIMING when the context is created: default 4, 1 times
+ * every call, 0 none).  out[12]: for FP64 (out[0..5]) then FP32
+ * (out[6..11]): {timed calls, seconds in their W launches (Psi.B^T per
+ * term), seconds in their Y launches (per-sector sums), algorithmic FLOPs of
+ * their W launches, FLOPs of their Y launches, all calls}; reset clears the
+ * counters. */
 int nss_ctx_hv_stats(nss_ctx* ctx, int reset, double* out);
 /* Phase timers collected when the environment sets NSS_PROFILE=1 (each
  * phase ends with a stream synchronisation): a JSON object of seconds per
@@ -133,6 +136,10 @@ int nss_rotate_blocks(nss_ctx* ctx, int nblk, const int32_t* rows, const int32_t
  * tridiagonal (d[n], e[n-1]) -> w ascending, z (n x n row-major) columns. */
 int nss_host_sym_eig(int n, double* a, double* w);
 int nss_host_tridiag_eig(int n, const double* d, const double* e, double* w, double* z);
+/* Lowest eigenvalue of the tridiagonal (d[n], e[n-1]) and the last
+ * component of its unit eigenvector (sign arbitrary): the residual estimate
+ * |beta * last component| of the Lanczos convergence test of nss_lanczos. */
+int nss_host_tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast);
 
 /* Lane-layout check of the MFMA builtins (prec 0: f64 16x16x4, 1: f32
  * 16x16x4): one wave, lane l supplies a[l] (A operand) and b[l] (B operand);

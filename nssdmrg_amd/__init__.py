@@ -101,14 +101,15 @@ class Context:
         return f.value
 
     def hv_stats(self, reset: bool = False) -> dict:
-        """H*v event timings: per precision calls, W/Y launch seconds and FLOPs."""
+        """H*v event timings per precision: timed calls with their W/Y launch
+        seconds and FLOPs, and all calls (``calls_all``)."""
         out = np.zeros(12)
         F.check(F.lib().nss_ctx_hv_stats(self._h, int(reset), F.dptr(out)))
         r = {}
         for k, tag in ((0, ""), (1, "32")):
             o = out[6 * k:6 * k + 6]
             r.update({f"calls{tag}": int(o[0]), f"t_w{tag}": o[1], f"t_y{tag}": o[2],
-                      f"flops_w{tag}": o[3], f"flops_y{tag}": o[4]})
+                      f"flops_w{tag}": o[3], f"flops_y{tag}": o[4], f"calls_all{tag}": int(o[5])})
         r["flops"] = r["flops_w"] + r["flops_y"]
         r["flops32"] = r["flops_w32"] + r["flops_y32"]
         return r
@@ -187,6 +188,16 @@ def host_tridiag_eig(d, e):
     z = np.zeros((n, n))
     F.check(F.lib().nss_host_tridiag_eig(n, F.dptr(d), F.dptr(e), F.dptr(w), F.dptr(z)))
     return w, z
+
+
+def host_tridiag_lowest(d, e):
+    """Lowest eigenvalue of the tridiagonal (d, e) and the last component of
+    its unit eigenvector (sign arbitrary)."""
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    th, zl = C.c_double(), C.c_double()
+    F.check(F.lib().nss_host_tridiag_lowest(d.size, F.dptr(d), F.dptr(e), C.byref(th), C.byref(zl)))
+    return th.value, zl.value
 
 
 # ---------------------------------------------------------------- sites

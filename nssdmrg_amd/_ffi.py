@@ -90,6 +90,7 @@ SIGNATURES = {
     "nss_dmrg_load_blocks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "nss_dmrg_save_blocks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "nss_host_tridiag_eig": (C.c_int, [C.c_int, _D, _D, _D, _D]),
+    "nss_host_tridiag_lowest": (C.c_int, [C.c_int, _D, _D, _D, _D]),
     "nss_dmrg_create": (C.c_int, [_P, C.POINTER(Model), C.POINTER(Params), C.POINTER(_P)]),
     "nss_dmrg_run": (C.c_int, [_P, C.c_int]),
     "nss_dmrg_finite_begin": (C.c_int, [_P]),

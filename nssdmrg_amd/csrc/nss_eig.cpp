@@ -7,6 +7,7 @@
 // Computations", sec. 8.3; Wilkinson & Reinsch, Handbook vol. II, II/2-II/3).
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -99,6 +100,98 @@ void tridiag_eig(int n, const double* d, const double* e, double* w, int nr, con
   }
   ql_implicit(n, w, ee.data(), nr, zrows);
   sort_pairs(n, w, nr, zrows);
+}
+
+// Lowest eigenvalue by bisection on Sturm counts (Wilkinson, The Algebraic
+// Eigenvalue Problem, 5.37-5.39: the negative pivots of the LDL^T of T - x I
+// count the eigenvalues below x), from the Gershgorin interval down to a few
+// ulps of its width.  The last component of the unit eigenvector by inverse
+// iteration: two solves with T - theta I, factored by Gaussian elimination
+// with partial pivoting (the tridiagonal LU of LAPACK's dgttrf, U with two
+// superdiagonals), zero pivots replaced by eps ||T||.
+void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast) {
+  if (n <= 0) return;
+  if (n == 1) {
+    *theta = d[0];
+    *zlast = 1.0;
+    return;
+  }
+  const double eps = std::numeric_limits<double>::epsilon();
+  double lo = HUGE_VAL, hi = -HUGE_VAL, emax2 = 0.0, tnorm = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+    tnorm = std::max(tnorm, std::fabs(d[i]) + r);
+    if (i + 1 < n) emax2 = std::max(emax2, e[i] * e[i]);
+  }
+  const double pivmin = std::numeric_limits<double>::min() * std::max(1.0, emax2);
+  auto below = [&](double x) {  // number of eigenvalues < x (at least one?)
+    double q = d[0] - x;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) return true;
+    for (int i = 1; i < n; ++i) {
+      q = d[i] - x - e[i - 1] * e[i - 1] / q;
+      if (std::fabs(q) < pivmin) q = -pivmin;
+      if (q < 0.0) return true;
+    }
+    return false;
+  };
+  const double tol = 2.0 * eps * std::max(tnorm, std::numeric_limits<double>::min());
+  for (int it = 0; it < 256 && hi - lo > tol; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid)) hi = mid;
+    else lo = mid;
+  }
+  const double th = 0.5 * (lo + hi);
+  *theta = th;
+  // LU of T - th I with row interchanges: dl (multipliers), dd, du, du2, piv
+  std::vector<double> dl(n - 1), dd(n), du(n - 1), du2(n > 2 ? n - 2 : 1, 0.0), x(n);
+  std::vector<char> piv(n - 1, 0);
+  for (int i = 0; i < n; ++i) dd[i] = d[i] - th;
+  for (int i = 0; i + 1 < n; ++i) dl[i] = du[i] = e[i];
+  const double tiny = eps * std::max(tnorm, std::numeric_limits<double>::min());
+  for (int i = 0; i + 1 < n; ++i) {
+    if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+      if (dd[i] == 0.0) dd[i] = tiny;
+      const double f = dl[i] / dd[i];
+      dl[i] = f;
+      dd[i + 1] -= f * du[i];
+      if (i + 2 < n) du2[i] = 0.0;
+    } else {
+      const double f = dd[i] / dl[i];
+      dd[i] = dl[i];
+      dl[i] = f;
+      const double t = du[i];
+      du[i] = dd[i + 1];
+      dd[i + 1] = t - f * dd[i + 1];
+      if (i + 2 < n) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du[i + 1];
+      }
+      piv[i] = 1;
+    }
+  }
+  if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
+  // start: a fixed sign pattern of unit entries (not orthogonal to any
+  // eigenvector of a generic T)
+  for (int i = 0; i < n; ++i) x[i] = (i % 3 == 1) ? -1.0 : 1.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i + 1 < n; ++i) {  // L: apply the interchanges and multipliers
+      if (piv[i]) std::swap(x[i], x[i + 1]);
+      x[i + 1] -= dl[i] * x[i];
+    }
+    x[n - 1] /= dd[n - 1];  // U
+    x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+    double nrm = 0.0, mx = 0.0;
+    for (int i = 0; i < n; ++i) mx = std::max(mx, std::fabs(x[i]));
+    for (int i = 0; i < n; ++i) nrm += (x[i] / mx) * (x[i] / mx);
+    nrm = mx * std::sqrt(nrm);
+    for (int i = 0; i < n; ++i) x[i] /= nrm;
+  }
+  *zlast = x[n - 1];
 }
 
 void tridiag_eig_full(int n, const double* d, const double* e, double* w, double* z) {
@@ -199,6 +292,16 @@ extern "C" int nss_host_sym_eig(int n, double* a, double* w) {
   } catch (const nss::Error& e) {
     nss::set_last_error(e.what());
     return e.code;
+  }
+  return NSS_OK;
+}
+
+extern "C" int nss_host_tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast) {
+  try {
+    nss::tridiag_lowest(n, d, e, theta, zlast);
+  } catch (const nss::Error& er) {
+    nss::set_last_error(er.what());
+    return er.code;
   }
   return NSS_OK;
 }

@@ -279,16 +279,28 @@ struct nss_ctx {
   void* aux_rocblas[NAUX] = {};
   nss::SbDevice* sb = nullptr;
   // H*v timing with HIP events recorded by the W and Y launches themselves:
-  // 4 events per call (W start/end, Y start/end).
-  bool time_hv = true;
+  // 4 events per timed call (W start/end, Y start/end).  An event-carrying
+  // launch costs the host several microseconds more than a plain one, and
+  // the host's enqueue rate bounds the Lanczos iterations of small
+  // superblocks, so only every hv_every-th call is timed (NSS_HV_TIMING:
+  // the period, 1 every call, 0 none; default 4).  st_all counts every call.
+  bool time_hv = true;  // false: calls are neither timed nor counted (dense assembly)
+  int hv_every = 4;
+  int64_t hv_seq = 0;
+  int64_t fl_all = 0, fl_timed = 0;  // calls since the last flush: all, timed
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
   // per recorded call: flops of the W and Y launches
   std::vector<double> ev_fw, ev_fy;
   // accumulated seconds / flops, [0] FP64, [1] FP32
   double st_tw[2] = {0, 0}, st_ty[2] = {0, 0}, st_fw[2] = {0, 0}, st_fy[2] = {0, 0};
-  int64_t st_calls[2] = {0, 0};
+  int64_t st_calls[2] = {0, 0}, st_all[2] = {0, 0};
   std::vector<int> ev_kind;  // 0 fp64, 1 fp32 per recorded call
+  // Lanczos convergence checks: pinned host copy of the iteration slots and
+  // the event that marks it (created on first use, freed with the context)
+  double* lz_pin = nullptr;
+  size_t lz_pin_n = 0;
+  hipEvent_t lz_ev = nullptr;
   // engines created on this context: destroyed (their device memory
   // returned) before the context's pool goes away
   std::vector<struct nss_dmrg*> engines;
@@ -316,7 +328,8 @@ void nss_release_rocblas(::nss_ctx* ctx);
 // Frees the state of every engine still alive on ctx (nss_engine.cpp).
 void engines_release(::nss_ctx* ctx);
 // Synchronises the context stream, folds the recorded H*v events into the
-// context statistics and returns the seconds spent in H*v since the last flush.
+// context statistics and returns the seconds spent in H*v since the last
+// flush (the timed calls' seconds scaled by all calls / timed calls).
 double nss_hv_timer_flush(::nss_ctx* ctx);
 
 // Host symmetric eigen-solvers (nss_eig.cpp).
@@ -326,6 +339,10 @@ void sym_eig(int n, double* a, double* w);
 // Symmetric tridiagonal (d[0..n), e[0..n-1)): eigenvalues ascending to w and
 // the requested rows of the eigenvector matrix Z (T = Z diag(w) Z^T):
 // zrows (nr x n, row-major) must enter holding the same rows of the identity.
+// Lowest eigenvalue theta of the tridiagonal (d, e) and the last component of
+// its unit eigenvector (sign arbitrary): the Lanczos residual estimate
+// |beta_m z_last| in O(n) per bisection step instead of a full QL.
+void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast);
 void tridiag_eig(int n, const double* d, const double* e, double* w, int nr, const int* rows,
                  double* zrows);
 // Full eigenvectors of the tridiagonal (z n x n row-major, columns = vectors).
@@ -359,6 +376,9 @@ int64_t lz_pro_min_n();
 // FP64 thick-restart iterations take alpha_j from the Y launch's partial
 // sums of v_j . H v_j (NSS_LZ_ALPHA_FROM_Y; 0 for a separate dot pass).
 bool lz_alpha_from_y();
+// Thick-restart iterations: the iteration after a convergence check is
+// enqueued before the host waits on the check (NSS_LZ_CHECK_AHEAD; 0 waits first).
+bool lz_check_ahead();
 // Dense path: H_sb assembled on the device from unit vectors, diagonalised on the host.
 LanczosOut dense_diag(nss_ctx* ctx, SbDevice* sb, int neigen, double* xdev);
 

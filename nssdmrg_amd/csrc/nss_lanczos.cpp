@@ -14,6 +14,7 @@
 // follows the observed residual decay, and solves the small projected
 // matrix in FP64.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -80,6 +81,27 @@ struct Lz {
   }
 };
 
+// Host-side split of a solve (NSS_PROFILE=1), without stream synchronisations
+// of its own: enqueueing the iterations, waiting at the convergence checks,
+// the host eigen-solves of the checks, the Ritz vectors at the end.
+struct LzClock {
+  nss_ctx* ctx;
+  double t0 = 0.0;
+  static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  }
+  explicit LzClock(nss_ctx* c) : ctx(c), t0(c->prof ? now() : 0.0) {}
+  void lap(const char* name) {
+    if (!ctx->prof) return;
+    const double t = now();
+    ctx->timers[name] += t - t0;
+    t0 = t;
+  }
+  void count(const char* name) {
+    if (ctx->prof) ctx->timers[name] += 1;
+  }
+};
+
 }  // namespace
 
 // One thick-restart Lanczos run from the (unnormalised) device vector v0.
@@ -102,7 +124,16 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
   int cycles = 0;
   std::vector<double> th, Y;
   const int nkeep = std::min(std::max(2 * neigen, neigen + 8), std::max(neigen, kmax / 2));
-  std::vector<double> hs;
+  nss_ctx* ctx = L.ctx;
+  if (ctx->lz_pin_n < (size_t)kmax * S) {
+    if (ctx->lz_pin) NSS_HIP(hipHostFree(ctx->lz_pin));
+    ctx->lz_pin = nullptr;
+    ctx->lz_pin_n = 0;
+    NSS_HIP(hipHostMalloc((void**)&ctx->lz_pin, (size_t)kmax * S * sizeof(double), hipHostMallocDefault));
+    ctx->lz_pin_n = (size_t)kmax * S;
+  }
+  if (!ctx->lz_ev) NSS_HIP(hipEventCreateWithFlags(&ctx->lz_ev, hipEventDisableTiming));
+  LzClock clk(L.ctx);
   for (;;) {
     bool converged = false, breakdown = false;
     int m = 0;
@@ -111,20 +142,35 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
     int next_check = j + 1;
     double last_res = -1.0;
     int last_res_m = 0;
+    bool ahead = false;  // iteration j already enqueued behind the last check's copy
     while (j < kmax) {
-      L.iterate(j, f32);
+      if (!ahead) L.iterate(j, f32);
+      ahead = false;
       ++j;
       const bool at_end = (j == kmax) || (j == n);
       if (j < next_check && !at_end) continue;
-      // read back the slots of iterations read_upto .. j-1
+      // copy the slots of iterations read_upto .. j-1 to pinned memory, then
+      // enqueue iteration j before waiting on the copy, so that the device
+      // keeps working while the host runs the check (if the check ends the
+      // run, that iteration's output is left unused: it writes only slot j,
+      // w and basis vector j+1)
       const int cnt = j - read_upto;
-      hs.resize((size_t)cnt * S);
-      NSS_HIP(hipMemcpyAsync(hs.data(), L.slots.p + (size_t)read_upto * S, hs.size() * sizeof(double),
+      clk.lap("lz.enqueue");
+      double* hs = ctx->lz_pin;
+      NSS_HIP(hipMemcpyAsync(hs, L.slots.p + (size_t)read_upto * S, (size_t)cnt * S * sizeof(double),
                              hipMemcpyDeviceToHost, s));
-      NSS_HIP(hipStreamSynchronize(s));
+      NSS_HIP(hipEventRecord(ctx->lz_ev, s));
+      if (!at_end && lz_check_ahead()) {
+        L.iterate(j, f32);
+        ahead = true;
+      }
+      clk.lap("lz.enqueue");
+      NSS_HIP(hipEventSynchronize(ctx->lz_ev));
+      clk.lap("lz.wait");
+      clk.count("lz.checks");
       double hn = 0.0;
       for (int it = read_upto; it < j; ++it) {
-        const double* sl = hs.data() + (size_t)(it - read_upto) * S;
+        const double* sl = hs + (size_t)(it - read_upto) * S;
         for (int i = 0; i <= it; ++i) {
           const bool keep = (i == it) || (i == it - 1) || (it == k && i < k);
           const double v = keep ? sl[i] + sl[(kmax + 1) + i] : 0.0;
@@ -142,10 +188,13 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
         }
       }
       read_upto = j;
-      if (breakdown) j = m;  // later iterations used a null vector: discard them
+      if (breakdown) {  // later iterations used a null vector: discard them
+        j = m;
+        ahead = false;
+      }
       if (m < neigen) {
         if (breakdown) throw Error(NSS_E_NOCONV, "lanczos: invariant Krylov space smaller than neigen");
-        next_check = neigen;
+        next_check = std::max(neigen, j + 1);
         continue;
       }
       th.assign(m, 0.0);
@@ -154,10 +203,14 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
         std::vector<double> d(m), e(m, 0.0);
         for (int i = 0; i < m; ++i) d[i] = L.t(i, i);
         for (int i = 0; i + 1 < m; ++i) e[i] = L.t(i + 1, i);
-        std::vector<double> zl(m);
-        int row = m - 1;
-        tridiag_eig(m, d.data(), e.data(), th.data(), 1, &row, zl.data());
-        for (int i = 0; i < m; ++i) Y[(size_t)(m - 1) * m + i] = zl[i];
+        if (neigen == 1) {  // the test needs only the lowest pair's last component
+          tridiag_lowest(m, d.data(), e.data(), &th[0], &Y[(size_t)(m - 1) * m]);
+        } else {
+          std::vector<double> zl(m);
+          int row = m - 1;
+          tridiag_eig(m, d.data(), e.data(), th.data(), 1, &row, zl.data());
+          for (int i = 0; i < m; ++i) Y[(size_t)(m - 1) * m + i] = zl[i];
+        }
       } else {
         for (int a = 0; a < m; ++a)
           for (int b = 0; b < m; ++b) Y[(size_t)a * m + b] = L.t(a, b);
@@ -197,7 +250,9 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
       last_res = worst;
       last_res_m = m;
       next_check = std::min(kmax, j + step);
+      clk.lap("lz.host");
     }
+    clk.lap("lz.host");
     if (m == 0) throw Error(NSS_E_STATE, "lanczos: empty Krylov space");
     const bool give_up = soft && cycles >= maxiter;
     if (converged || breakdown || m == n || give_up) {
@@ -214,6 +269,7 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
         launch_div(x + (size_t)i * n, x + (size_t)i * n, n, L.scal.p, s);
       }
       NSS_HIP(hipStreamSynchronize(s));
+      clk.lap("lz.ritz");
       return std::vector<double>(th.begin(), th.begin() + neigen);
     }
     // thick restart: keep the nkeep lowest Ritz vectors + the residual direction
@@ -242,11 +298,13 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
     L.restart_at = kk;
     // slots of the new iterations must not carry stale recurrence terms
     NSS_HIP(hipMemsetAsync(L.slots.p + (size_t)kk * S, 0, (size_t)(kmax - kk) * S * sizeof(double), s));
+    clk.lap("lz.restart");
   }
 }
 
 LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int maxiter, double tol,
                           uint64_t seed, int v0_mode, int precision, double* xdev) {
+  LzClock clk(ctx);
   const int64_t n = sb_n(sb);
   NSS_REQUIRE(n > 0, NSS_E_STATE, "lanczos: empty superblock");
   NSS_REQUIRE(neigen >= 1 && neigen <= 4 && neigen <= n, NSS_E_ARG, "lanczos: 1 <= neigen <= min(4,n)");
@@ -287,6 +345,7 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
     if (v0_mode == 1) launch_fill_const(v0.p, n, 1.0 / std::sqrt((double)n), ctx->stream);
     else launch_fill_splitmix(v0.p, n, seed, ctx->stream);
   }
+  clk.lap("lz.setup");
   LanczosOut out;
   if (precision == NSS_FP32_MIXED) {
     // FP32-MFMA H*v to an FP32-attainable residual, then the FP64 restart
@@ -316,6 +375,16 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
 bool lz_alpha_from_y() {
   static const bool v = [] {
     const char* e = std::getenv("NSS_LZ_ALPHA_FROM_Y");
+    return !e || std::atoi(e) != 0;
+  }();
+  return v;
+}
+
+// Enqueue the next iteration before waiting on a convergence check
+// (NSS_LZ_CHECK_AHEAD=0: wait first).
+bool lz_check_ahead() {
+  static const bool v = [] {
+    const char* e = std::getenv("NSS_LZ_CHECK_AHEAD");
     return !e || std::atoi(e) != 0;
   }();
   return v;
@@ -498,9 +567,8 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
           break;
         }
       }
-      std::vector<double> th(m), zl(m);
-      int row = m - 1;
-      tridiag_eig(m, d[c].data(), e[c].data(), th.data(), 1, &row, zl.data());
+      std::vector<double> th(m), zl(1);
+      tridiag_lowest(m, d[c].data(), e[c].data(), &th[0], &zl[0]);
       const double res = std::fabs(beta[c][m - 1] * zl[0]);
       const bool conv = res <= tol * std::max(1.0, std::fabs(th[0]));
       if (!(conv || breakdown || m >= kmc[c])) continue;

@@ -310,7 +310,13 @@ static hipEvent_t next_event(nss_ctx* ctx) {
 // between them.
 static void hv_timed(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b, int kind,
                      double* ydot = nullptr) {
-  const bool t = ctx->time_hv;
+  bool t = false;
+  if (ctx->time_hv) {
+    ++ctx->st_all[kind];
+    ++ctx->fl_all;
+    t = ctx->hv_every > 0 && ctx->hv_seq++ % ctx->hv_every == 0;
+    ctx->fl_timed += t;
+  }
   hipEvent_t e[4] = {};
   if (t)
     for (hipEvent_t& x : e) x = next_event(ctx);
@@ -414,7 +420,9 @@ double nss_hv_timer_flush(nss_ctx* ctx) {
   ctx->ev_kind.clear();
   ctx->ev_fw.clear();
   ctx->ev_fy.clear();
-  return tot;
+  const double scale = ctx->fl_timed > 0 ? (double)ctx->fl_all / (double)ctx->fl_timed : 0.0;
+  ctx->fl_all = ctx->fl_timed = 0;
+  return tot * scale;
 }
 
 static double now_s() {
@@ -442,6 +450,11 @@ PhaseTimer::~PhaseTimer() {
 void nss_release_rocblas(nss_ctx* ctx) {
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   ctx->ev.clear();
+  if (ctx->lz_ev) (void)hipEventDestroy(ctx->lz_ev);
+  ctx->lz_ev = nullptr;
+  if (ctx->lz_pin) (void)hipHostFree(ctx->lz_pin);
+  ctx->lz_pin = nullptr;
+  ctx->lz_pin_n = 0;
   if (ctx->rocblas) {
     (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
     ctx->rocblas = nullptr;
@@ -856,8 +869,8 @@ extern "C" int nss_ctx_create(int device, nss_ctx** out) {
   c->sb = sb_create(c);
   const char* pe = std::getenv("NSS_PROFILE");
   c->prof = pe && pe[0] == '1';
-  const char* te = std::getenv("NSS_HV_TIMING");  // 0: no H*v events (the statistics stay empty)
-  c->time_hv = !(te && te[0] == '0');
+  const char* te = std::getenv("NSS_HV_TIMING");  // period of the timed H*v calls (0: none)
+  c->hv_every = te ? std::max(0, std::atoi(te)) : 4;
   *out = c;
   NSS_API_END
 }
@@ -1020,9 +1033,9 @@ extern "C" int nss_ctx_hv_stats(nss_ctx* ctx, int reset, double* out) {
     out[6 * k + 2] = ctx->st_ty[k];
     out[6 * k + 3] = ctx->st_fw[k];
     out[6 * k + 4] = ctx->st_fy[k];
-    out[6 * k + 5] = 0.0;
+    out[6 * k + 5] = (double)ctx->st_all[k];
     if (reset) {
-      ctx->st_calls[k] = 0;
+      ctx->st_calls[k] = ctx->st_all[k] = 0;
       ctx->st_tw[k] = ctx->st_ty[k] = ctx->st_fw[k] = ctx->st_fy[k] = 0.0;
     }
   }

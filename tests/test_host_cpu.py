@@ -87,6 +87,50 @@ def test_host_tridiag_eig():
     assert np.allclose(t @ z, z * w, atol=1e-11)
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 40, 120])
+def test_host_tridiag_lowest_matches_lapack(n):
+    rng = np.random.default_rng(100 + n)
+    d = rng.standard_normal(n) * 3.0
+    e = rng.standard_normal(max(0, n - 1))
+    t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    w, z = np.linalg.eigh(t)
+    th, zl = N.host_tridiag_lowest(d, e)
+    scale = max(1.0, np.abs(w).max())
+    assert abs(th - w[0]) <= 1e-13 * scale
+    if n == 1 or w[1] - w[0] > 1e-6 * scale:
+        assert abs(abs(zl) - abs(z[-1, 0])) <= 1e-10
+
+
+def test_host_tridiag_lowest_closed_form_and_edge_cases():
+    # T = tridiag(-1, 2, -1): theta = 2 - 2 cos(pi/(n+1)), z_k = sqrt(2/(n+1)) sin(k pi/(n+1))
+    for n in (5, 64, 150):
+        th, zl = N.host_tridiag_lowest(np.full(n, 2.0), np.full(n - 1, -1.0))
+        assert abs(th - (2.0 - 2.0 * np.cos(np.pi / (n + 1)))) <= 1e-14
+        assert abs(abs(zl) - np.sqrt(2.0 / (n + 1)) * np.sin(n * np.pi / (n + 1))) <= 1e-12
+    # a converged Lanczos tridiagonal: tiny last coupling, so z_last is tiny and
+    # |beta z_last| must come out at the size numpy gives it
+    rng = np.random.default_rng(5)
+    n = 60
+    d = rng.uniform(-3, 3, n)
+    e = rng.uniform(0.2, 1.0, n - 1)
+    e[-1] = 1e-9
+    d[-1] = 10.0
+    t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    w, z = np.linalg.eigh(t)
+    th, zl = N.host_tridiag_lowest(d, e)
+    assert abs(th - w[0]) <= 1e-13 * np.abs(w).max()
+    assert abs(abs(zl) - abs(z[-1, 0])) <= 1e-14 + 1e-6 * abs(z[-1, 0])
+    # decoupled: the lowest pair lives in the upper block, so z_last = 0
+    d = np.array([-5.0, 1.0, 2.0, 3.0, 4.0])
+    e = np.array([0.5, 0.0, 0.3, 0.2])
+    th, zl = N.host_tridiag_lowest(d, e)
+    assert abs(th - np.linalg.eigvalsh(np.diag(d) + np.diag(e, 1) + np.diag(e, -1))[0]) <= 1e-14
+    assert abs(zl) <= 1e-12
+    # diagonal matrix: the smallest entry
+    th, zl = N.host_tridiag_lowest(np.array([3.0, -1.0, 2.0]), np.zeros(2))
+    assert th == pytest.approx(-1.0, abs=1e-15) and abs(zl) <= 1e-12
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_kept_count_matches_oracle(seed):
     rng = np.random.default_rng(seed)
