@@ -225,6 +225,15 @@ void launch_div(const double* x, double* y, int64_t n, const double* den, hipStr
 // -> eigenvectors as columns at vout+off[b], eigenvalues (unsorted) at
 // wout+eoff[b].
 int jacobi_max_dim(int device);
+// Tridiagonal-path eigen-solver for symmetric blocks of dimension 3..nmax
+// (nmax <= rho_tri_max_dim): one workgroup per block, same layout and
+// output as launch_jacobi_eig (eigenvalues unsorted across the blocks of
+// its tridiagonal form); ws holds
+// rho_tri_workspace(nblk) doubles.
+int rho_tri_max_dim(int device);
+int64_t rho_tri_workspace(int nblk);
+void launch_rho_eig_tri(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
+                        double* wout, double* ws, int nblk, int nmax, hipStream_t s);
 void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
                        double* wout, int nblk, int nmax, hipStream_t s);
 

@@ -11,6 +11,7 @@
 //    then LDS, two fixed-order stages (deterministic dot/norm/GEMV).
 //  * k_kron: QN-blocked Kronecker scatter used by block enlargement.
 #include <algorithm>
+#include <cfloat>
 
 #include <hip/hip_ext.h>
 
@@ -1051,18 +1052,39 @@ void launch_div(const double* x, double* y, int64_t n, const double* den, hipStr
 // angles: Golub & Van Loan, Algorithm 8.4.1 (symSchur2).  Output:
 // eigenvalues (unsorted) and eigenvectors as columns, row-major.
 constexpr int JAC_THREADS = 512;
+constexpr int JAC_WAVES = JAC_THREADS / 64;
 
-__device__ __forceinline__ double block_max(double v, double* red) {
-  red[threadIdx.x] = v;
+// max over the workgroup, valid on every thread
+__device__ __forceinline__ double jac_block_max(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int st = JAC_THREADS / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < JAC_WAVES; ++w) r = fmax(r, red[w]);
   __syncthreads();
   return r;
 }
+
+// sum over the workgroup in a fixed order, valid on every thread
+__device__ __forceinline__ double jac_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < JAC_WAVES; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+// The updates of a round are flattened over the workgroup: item e of the A
+// update is the 2x2 sub-block (pair e / half, pair e % half), item e of the
+// V update is column e % np of the row pair e / np.  Each thread loads the
+// operands of JAC_U items before it computes and stores any of them (the
+// items of a round touch disjoint elements), so their LDS latencies overlap.
+constexpr int JAC_U = 4;
 
 __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __restrict__ a,
                                                             const int64_t* __restrict__ off,
@@ -1073,30 +1095,23 @@ __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __rest
   extern __shared__ double sm[];
   __shared__ double cs_c[64], cs_s[64];
   __shared__ int pr_p[64], pr_q[64];
-  __shared__ double red[JAC_THREADS];
+  __shared__ double red[JAC_WAVES];
   const int b = blockIdx.x;
   const int n = dim[b];
   const int np = n + (n & 1);  // even number of slots (one dummy when n is odd)
   const int ld = np + 1;
   const int half = np / 2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = JAC_THREADS / 64;
+  const int tid = threadIdx.x;
   double* A = sm;
   double* Vt = sm + (size_t)np * ld;
   const double* src = a + off[b];
-  for (int i = wave; i < np; i += nwave)
-    for (int j = lane; j < np; j += 64) {
-      A[i * ld + j] = (i < n && j < n) ? src[(size_t)i * n + j] : 0.0;
-      Vt[i * ld + j] = (i == j) ? 1.0 : 0.0;
-    }
-  __syncthreads();
   double f = 0.0;
-  for (int i = wave; i < np; i += nwave)
-    for (int j = lane; j < np; j += 64) f += A[i * ld + j] * A[i * ld + j];
-  red[threadIdx.x] = f;
-  __syncthreads();
-  for (int st = JAC_THREADS / 2; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
+  for (int e = tid; e < np * np; e += JAC_THREADS) {
+    const int i = e / np, j = e - i * np;
+    const double x = (i < n && j < n) ? src[(size_t)i * n + j] : 0.0;
+    A[i * ld + j] = x;
+    Vt[i * ld + j] = (i == j) ? 1.0 : 0.0;
+    f += x * x;
   }
   // rotations are skipped, and the sweeps stop, once every off-diagonal
   // element is below 1e-15 ||A||_F: a few ulps of ||A||, above the rounding
@@ -1105,18 +1120,20 @@ __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __rest
   // two triangles are updated separately and drift apart by rounding, so
   // both tests use the symmetric part (A_pq + A_qp)/2: a test on one
   // triangle and a rotation decided on the other would leave an element
-  // just above the threshold for ever.
-  const double thr = 1e-15 * sqrt(red[0]);
-  __syncthreads();
+  // just above the threshold for ever.  (The sum also orders the loads
+  // above before the sweeps read A.)
+  const double thr = 1e-15 * sqrt(jac_block_sum(f, red));
+  const int na = half * half, nv = half * np;
   for (int sw = 0; sw < max_sweeps; ++sw) {
     double o = 0.0;
-    for (int i = wave; i < np; i += nwave)
-      for (int j = lane; j < np; j += 64)
-        if (i != j) o = fmax(o, fabs(0.5 * (A[i * ld + j] + A[j * ld + i])));
-    if (!(block_max(o, red) > thr)) break;
+    for (int e = tid; e < np * np; e += JAC_THREADS) {
+      const int i = e / np, j = e - i * np;
+      if (i != j) o = fmax(o, fabs(0.5 * (A[i * ld + j] + A[j * ld + i])));
+    }
+    if (!(jac_block_max(o, red) > thr)) break;
     for (int r = 0; r < np - 1; ++r) {
-      if ((int)threadIdx.x < half) {
-        const int i = threadIdx.x;
+      if (tid < half) {
+        const int i = tid;
         const int k1 = np - 1 - i;
         int p = (i == 0) ? 0 : ((i - 1 + r) % (np - 1)) + 1;
         int q = ((k1 - 1 + r) % (np - 1)) + 1;
@@ -1136,37 +1153,74 @@ __global__ __launch_bounds__(JAC_THREADS) void k_jacobi_eig(const double* __rest
       }
       __syncthreads();
       // A <- J^T A J, one 2x2 sub-block per item
-      for (int i = wave; i < half; i += nwave) {
-        const int p = pr_p[i], q = pr_q[i];
-        const double ci = cs_c[i], si = cs_s[i];
-        for (int j = lane; j < half; j += 64) {
-          const int rr = pr_p[j], ss = pr_q[j];
-          const double cj = cs_c[j], sj = cs_s[j];
-          const double b00 = A[p * ld + rr], b01 = A[p * ld + ss], b10 = A[q * ld + rr], b11 = A[q * ld + ss];
-          const double t00 = ci * b00 - si * b10, t01 = ci * b01 - si * b11;
-          const double t10 = si * b00 + ci * b10, t11 = si * b01 + ci * b11;
-          double n01 = sj * t00 + cj * t01, n10 = cj * t10 - sj * t11;
-          if (i == j && si != 0.0) n01 = n10 = 0.0;  // the rotated pair is annihilated
-          A[p * ld + rr] = cj * t00 - sj * t01;
-          A[p * ld + ss] = n01;
-          A[q * ld + rr] = n10;
-          A[q * ld + ss] = sj * t10 + cj * t11;
+      for (int e0 = tid; e0 < na; e0 += JAC_THREADS * JAC_U) {
+        double b00[JAC_U], b01[JAC_U], b10[JAC_U], b11[JAC_U], ci[JAC_U], si[JAC_U], cj[JAC_U], sj[JAC_U];
+        int p[JAC_U], q[JAC_U], rr[JAC_U], ss[JAC_U];
+        bool diag[JAC_U];
+#pragma unroll
+        for (int u = 0; u < JAC_U; ++u) {
+          const int e = min(e0 + u * JAC_THREADS, na - 1);  // past the end: repeat the last item, not stored
+          const int i = e / half, j = e - i * half;
+          p[u] = pr_p[i];
+          q[u] = pr_q[i];
+          rr[u] = pr_p[j];
+          ss[u] = pr_q[j];
+          ci[u] = cs_c[i];
+          si[u] = cs_s[i];
+          cj[u] = cs_c[j];
+          sj[u] = cs_s[j];
+          diag[u] = i == j;
+          b00[u] = A[p[u] * ld + rr[u]];
+          b01[u] = A[p[u] * ld + ss[u]];
+          b10[u] = A[q[u] * ld + rr[u]];
+          b11[u] = A[q[u] * ld + ss[u]];
         }
-        // V <- V J  (rows p, q of V^T)
-        if (si != 0.0)
-          for (int k = lane; k < np; k += 64) {
-            const double vp = Vt[p * ld + k], vq = Vt[q * ld + k];
-            Vt[p * ld + k] = ci * vp - si * vq;
-            Vt[q * ld + k] = si * vp + ci * vq;
+#pragma unroll
+        for (int u = 0; u < JAC_U; ++u) {
+          if (e0 + u * JAC_THREADS >= na) break;
+          const double t00 = ci[u] * b00[u] - si[u] * b10[u], t01 = ci[u] * b01[u] - si[u] * b11[u];
+          const double t10 = si[u] * b00[u] + ci[u] * b10[u], t11 = si[u] * b01[u] + ci[u] * b11[u];
+          double n01 = sj[u] * t00 + cj[u] * t01, n10 = cj[u] * t10 - sj[u] * t11;
+          if (diag[u] && si[u] != 0.0) n01 = n10 = 0.0;  // the rotated pair is annihilated
+          A[p[u] * ld + rr[u]] = cj[u] * t00 - sj[u] * t01;
+          A[p[u] * ld + ss[u]] = n01;
+          A[q[u] * ld + rr[u]] = n10;
+          A[q[u] * ld + ss[u]] = sj[u] * t10 + cj[u] * t11;
+        }
+      }
+      // V <- V J  (rows p, q of V^T)
+      for (int e0 = tid; e0 < nv; e0 += JAC_THREADS * JAC_U) {
+        double vp[JAC_U], vq[JAC_U], c[JAC_U], sn[JAC_U];
+        int ip[JAC_U], iq[JAC_U];
+#pragma unroll
+        for (int u = 0; u < JAC_U; ++u) {
+          const int e = min(e0 + u * JAC_THREADS, nv - 1);
+          const int i = e / np, k = e - i * np;
+          c[u] = cs_c[i];
+          sn[u] = cs_s[i];
+          ip[u] = pr_p[i] * ld + k;
+          iq[u] = pr_q[i] * ld + k;
+          vp[u] = Vt[ip[u]];
+          vq[u] = Vt[iq[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < JAC_U; ++u) {
+          if (e0 + u * JAC_THREADS >= nv) break;
+          if (sn[u] != 0.0) {
+            Vt[ip[u]] = c[u] * vp[u] - sn[u] * vq[u];
+            Vt[iq[u]] = sn[u] * vp[u] + c[u] * vq[u];
           }
+        }
       }
       __syncthreads();
     }
   }
   double* vo = vout + off[b];
-  for (int i = wave; i < n; i += nwave)
-    for (int j = lane; j < n; j += 64) vo[(size_t)i * n + j] = Vt[j * ld + i];
-  for (int i = threadIdx.x; i < n; i += JAC_THREADS) wout[eoff[b] + i] = A[i * ld + i];
+  for (int e = tid; e < n * n; e += JAC_THREADS) {
+    const int i = e / n, j = e - i * n;
+    vo[(size_t)i * n + j] = Vt[j * ld + i];
+  }
+  for (int i = tid; i < n; i += JAC_THREADS) wout[eoff[b] + i] = A[i * ld + i];
 }
 
 int jacobi_max_dim(int device) {
@@ -1191,6 +1245,389 @@ void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, 
   const int np = nmax + (nmax & 1);
   const size_t lds = (size_t)2 * np * (np + 1) * sizeof(double);
   hipLaunchKernelGGL(k_jacobi_eig, dim3(nblk), dim3(JAC_THREADS), lds, s, a, off, dim, eoff, vout, wout, 40);
+  NSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ batched tridiagonal-path eigensolver
+// One 1024-thread workgroup per symmetric block of dimension 3..TRI_MAX
+// (LAPACK's dsyevx path: dsytrd + dstebz + dstein + dormtr):
+//  1. Householder tridiagonalisation in LDS (dsytd2, lower: Q =
+//     H_0 ... H_{n-3}, H_k = I - tau_k v_k v_k^T, the v_k kept in a global
+//     workspace), the p = tau A v products eight lanes per row;
+//  2. every eigenvalue by multisection on Sturm counts (Wilkinson,
+//     The Algebraic Eigenvalue Problem, 5.37-5.39), eight lanes per
+//     eigenvalue, each evaluating one of eight interior points per step;
+//  3. its eigenvector by inverse iteration (dstein): two solves with
+//     T - lambda I, factored by Gaussian elimination with partial pivoting
+//     (dgttrf), from a per-vector pseudo-random start, the factors in the
+//     global workspace and the iterate in column t of Z (LDS);
+//  4. Gram-Schmidt within clusters of eigenvalues closer than 1e-3 ||T||,
+//     as dstein does (right-looking modified Gram-Schmidt, two sweeps);
+//  5. Z <- Q Z, one reflector at a time.
+// T is split into unreduced blocks at |e_k| <= eps ||T|| first (dstebz) and
+// every eigenpair is computed within its block.  Output as the Jacobi
+// kernel's: eigenvectors as columns (row-major), eigenvalues at wout +
+// eoff[b], ascending within each block of T (the caller sorts).
+constexpr int TRI_THREADS = 1024;
+constexpr int TRI_WAVES = TRI_THREADS / 64;
+constexpr int TRI_MAX = 128;
+constexpr int64_t TRI_WS = (int64_t)6 * TRI_MAX * TRI_MAX;  // workspace doubles per block
+
+int64_t rho_tri_workspace(int nblk) { return (int64_t)nblk * TRI_WS; }
+
+__device__ __forceinline__ double tri_block_sum(double v, double* red) {  // all threads get the sum
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < TRI_WAVES; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+// sum over the aligned group of 8 lanes
+__device__ __forceinline__ double group8_sum(double v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  return v;
+}
+
+// 1/q to full precision: the hardware reciprocal and two Newton steps
+__device__ __forceinline__ double tri_rcp(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return fma(fma(-q, r, 1.0), r, r);
+}
+
+__global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __restrict__ a,
+                                                             const int64_t* __restrict__ off,
+                                                             const int32_t* __restrict__ dim,
+                                                             const int64_t* __restrict__ eoff,
+                                                             double* __restrict__ vout, double* __restrict__ wout,
+                                                             double* __restrict__ ws) {
+  extern __shared__ double sm[];  // n x (n+1): A, then Z
+  __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
+  __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX];
+  __shared__ double red[TRI_WAVES];
+  const int b = blockIdx.x;
+  const int n = dim[b];
+  const int ld = n + 1;
+  const int tid = threadIdx.x;
+  const int g8 = tid >> 3, l8 = tid & 7;    // groups of 8 lanes
+  const int r32 = tid >> 5, c32 = tid & 31;  // 32 x 32 thread grid
+  double* A = sm;
+  const double* src = a + off[b];
+  double* hv = ws + (size_t)b * TRI_WS;           // reflector k at hv[k*TRI_MAX + r], r < n-k-1
+  double* lu = hv + (size_t)TRI_MAX * TRI_MAX;    // factor f, row i, vector t at lu[(f*TRI_MAX + i)*TRI_MAX + t]
+  for (int i = r32; i < n; i += 32)
+    for (int j = c32; j < n; j += 32) A[i * ld + j] = src[(size_t)i * n + j];
+  __syncthreads();
+  // 1. tridiagonalisation (every branch below is uniform: its inputs are
+  //    LDS values or block sums that every thread reads alike)
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;  // x = A[k+1..n-1][k]
+    double s2 = 0.0;
+    if (tid + k + 2 < n) {
+      const double x = A[(tid + k + 2) * ld + k];
+      s2 = x * x;
+    }
+    const double xn2 = tri_block_sum(s2, red);
+    const double alpha = A[(k + 1) * ld + k];
+    double t = 0.0, beta = alpha, scale = 0.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      t = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    if (tid < m) {
+      const double v = (tid == 0) ? 1.0 : A[(k + 1 + tid) * ld + k] * scale;
+      vb[tid] = v;
+      hv[(size_t)k * TRI_MAX + tid] = v;
+    }
+    if (tid == 0) {
+      tau[k] = t;
+      eg[k] = beta;
+      dg[k] = A[k * ld + k];
+    }
+    __syncthreads();
+    if (t != 0.0) {
+      {  // p = t A22 v, eight threads per row
+        double acc = 0.0;
+        if (g8 < m) {
+          const double* row = A + (k + 1 + g8) * ld + (k + 1);
+          for (int c = l8; c < m; c += 8) acc += row[c] * vb[c];
+        }
+        acc = group8_sum(acc);
+        if (g8 < m && l8 == 0) pb[g8] = t * acc;
+      }
+      __syncthreads();
+      const double pv = (tid < m) ? pb[tid] * vb[tid] : 0.0;
+      const double a2 = -0.5 * t * tri_block_sum(pv, red);
+      if (tid < m) pb[tid] += a2 * vb[tid];  // w = p + a2 v
+      __syncthreads();
+      for (int r = r32; r < m; r += 32) {  // A22 -= v w^T + w v^T
+        const double vr = vb[r], wr = pb[r];
+        double* row = A + (k + 1 + r) * ld + (k + 1);
+        for (int c = c32; c < m; c += 32) row[c] -= vr * pb[c] + wr * vb[c];
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    dg[n - 2] = A[(n - 2) * ld + (n - 2)];
+    dg[n - 1] = A[(n - 1) * ld + (n - 1)];
+    eg[n - 2] = A[(n - 1) * ld + (n - 2)];
+  }
+  __syncthreads();
+  // Gershgorin interval and ||T|| (every thread alike)
+  double lo0 = HUGE_VAL, hi0 = -HUGE_VAL, tnorm = 0.0, emax2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? fabs(eg[i - 1]) : 0.0) + (i + 1 < n ? fabs(eg[i]) : 0.0);
+    lo0 = fmin(lo0, dg[i] - r);
+    hi0 = fmax(hi0, dg[i] + r);
+    tnorm = fmax(tnorm, fabs(dg[i]) + r);
+    if (i + 1 < n) emax2 = fmax(emax2, eg[i] * eg[i]);
+  }
+  tnorm = fmax(tnorm, DBL_MIN);
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  // zero pivots of T - lambda I become tiny: eps ||T||, floored so that a
+  // (near-)zero block still gives finite iterates
+  const double tiny = fmax(DBL_EPSILON * tnorm, 1e-280);
+  __syncthreads();
+  // T splits into unreduced blocks where |e_k| <= eps ||T|| (as dstebz
+  // splits it); each eigenpair is computed within its block, so vectors of
+  // different blocks are exactly orthogonal (the null space of a
+  // rank-deficient rho comes out as unit vectors)
+  if (tid + 1 < n && fabs(eg[tid]) <= DBL_EPSILON * tnorm) eg[tid] = 0.0;
+  __syncthreads();
+  if (tid < n) e2g[tid] = tid + 1 < n ? eg[tid] * eg[tid] : 0.0;
+  __syncthreads();
+  // position p of the output belongs to the block [s, e) that holds row p and
+  // takes eigenvalue p - s of that block (ascending within the block)
+  auto block_of = [&](int p, int& s, int& e) {
+    s = p;
+    e = p + 1;
+    while (s > 0 && eg[s - 1] != 0.0) --s;
+    while (e < n && eg[e - 1] != 0.0) ++e;
+  };
+  // 2. eigenvalues by multisection, the 8 lanes of a group evaluating the
+  //    Sturm count of the block at 8 interior points of the interval each step
+  {
+    const int p = g8;
+    int bs = 0, be = 1;
+    if (p < n) block_of(p, bs, be);
+    const int t = p - bs;
+    double lo = lo0, hi = hi0;
+    if (p < n && be - bs == 1) lo = hi = dg[p];
+    const double tol = 2.0 * DBL_EPSILON * tnorm;
+    for (int it = 0; it < 64; ++it) {
+      if (!(hi - lo > tol)) break;  // alike on the 8 lanes of a group
+      const double x = lo + (hi - lo) * (double)(l8 + 1) * (1.0 / 9.0);
+      int cnt = 0;
+      if (p < n) {
+        double q = dg[bs] - x;
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt = q < 0.0;
+        for (int i = bs + 1; i < be; ++i) {
+          q = dg[i] - x - e2g[i - 1] * tri_rcp(q);
+          if (fabs(q) < pivmin) q = -pivmin;
+          cnt += q < 0.0;
+        }
+      }
+      const bool above = cnt >= t + 1;  // eigenvalue t lies below x
+      const uint64_t bal = __ballot(above);
+      const unsigned bits = (unsigned)(bal >> (threadIdx.x & 56)) & 0xFFu;
+      const int f = bits ? __builtin_ctz(bits) : 8;  // first point above it
+      const double span = hi - lo;
+      const double nlo = f > 0 ? lo + span * (double)f * (1.0 / 9.0) : lo;
+      const double nhi = f < 8 ? lo + span * (double)(f + 1) * (1.0 / 9.0) : hi;
+      lo = nlo;
+      hi = fmax(nhi, nlo);
+    }
+    if (p < n && l8 == 0) lam[p] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+  double* Z = sm;  // A is no longer read: the reflectors are in hv, T in dg/eg
+  if (tid < n) {
+    // 3. inverse iteration for position tid within its block [bs, be):
+    //    LU with partial pivoting of T_block - sh I (local rows 0..nb-1)
+    int bs, be;
+    block_of(tid, bs, be);
+    const int nb = be - bs;
+    for (int i = 0; i < n; ++i)
+      if (i < bs || i >= be) Z[i * ld + tid] = 0.0;
+    if (nb == 1) {
+      Z[tid * ld + tid] = 1.0;
+    } else {
+      const double sh = lam[tid];
+      double* DL = lu + tid;
+      double* DD = lu + (size_t)TRI_MAX * TRI_MAX + tid;
+      double* DU = lu + (size_t)2 * TRI_MAX * TRI_MAX + tid;
+      double* DU2 = lu + (size_t)3 * TRI_MAX * TRI_MAX + tid;
+      double* PV = lu + (size_t)4 * TRI_MAX * TRI_MAX + tid;
+      const double* dgb = dg + bs;
+      const double* egb = eg + bs;
+      double* Zb = Z + (size_t)bs * ld + tid;  // local row i at Zb[i * ld]
+      double ddi = dgb[0] - sh, dui = egb[0];
+      for (int i = 0; i + 1 < nb; ++i) {
+        const double dli = egb[i];
+        double ddn = dgb[i + 1] - sh;
+        double dun = (i + 2 < nb) ? egb[i + 1] : 0.0;
+        double f, du2i = 0.0, pvi = 0.0;
+        if (fabs(ddi) >= fabs(dli)) {
+          if (ddi == 0.0) ddi = tiny;
+          f = dli / ddi;
+          ddn -= f * dui;
+        } else {
+          f = ddi / dli;
+          const double tmp = dui;
+          ddi = dli;
+          dui = ddn;
+          ddn = tmp - f * ddn;
+          if (i + 2 < nb) {
+            du2i = dun;
+            dun = -f * dun;
+          }
+          pvi = 1.0;
+        }
+        DL[(size_t)i * TRI_MAX] = f;
+        DD[(size_t)i * TRI_MAX] = ddi;
+        DU[(size_t)i * TRI_MAX] = dui;
+        DU2[(size_t)i * TRI_MAX] = du2i;
+        PV[(size_t)i * TRI_MAX] = pvi;
+        ddi = ddn;
+        dui = dun;
+      }
+      if (ddi == 0.0) ddi = tiny;
+      DD[(size_t)(nb - 1) * TRI_MAX] = ddi;
+      // start: uniform [-1, 1) from SplitMix64 of (vector, row)
+      for (int i = 0; i < nb; ++i) {
+        uint64_t z = (uint64_t)(tid * TRI_MAX + i + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        Zb[i * ld] = (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        double xi = Zb[0];
+        for (int i = 0; i + 1 < nb; ++i) {  // L, with the row interchanges
+          double xn = Zb[(i + 1) * ld];
+          if (PV[(size_t)i * TRI_MAX] != 0.0) {
+            const double t = xi;
+            xi = xn;
+            xn = t;
+          }
+          xn -= DL[(size_t)i * TRI_MAX] * xi;
+          Zb[i * ld] = xi;
+          xi = xn;
+        }
+        double x1 = xi / DD[(size_t)(nb - 1) * TRI_MAX];  // U (two superdiagonals)
+        Zb[(nb - 1) * ld] = x1;
+        double x0 = (Zb[(nb - 2) * ld] - DU[(size_t)(nb - 2) * TRI_MAX] * x1) / DD[(size_t)(nb - 2) * TRI_MAX];
+        Zb[(nb - 2) * ld] = x0;
+        double x2 = x1;
+        x1 = x0;
+        for (int i = nb - 3; i >= 0; --i) {
+          x0 = (Zb[i * ld] - DU[(size_t)i * TRI_MAX] * x1 - DU2[(size_t)i * TRI_MAX] * x2) / DD[(size_t)i * TRI_MAX];
+          Zb[i * ld] = x0;
+          x2 = x1;
+          x1 = x0;
+        }
+        double mx = 0.0;
+        for (int i = 0; i < nb; ++i) mx = fmax(mx, fabs(Zb[i * ld]));
+        double ss = 0.0;
+        for (int i = 0; i < nb; ++i) {
+          const double y = Zb[i * ld] / mx;
+          ss += y * y;
+        }
+        const double inv = 1.0 / (mx * sqrt(ss));
+        for (int i = 0; i < nb; ++i) Zb[i * ld] *= inv;
+      }
+    }
+  }
+  __syncthreads();
+  // 4. clusters: consecutive eigenvalues of one block within 1e-3 ||T|| of
+  //    each other (dstein's reorthogonalisation criterion); right-looking modified
+  //    Gram-Schmidt, two sweeps.  Step j: h_k = z_j . z_k for k >= j (eight
+  //    lanes per dot), then z_k -= (h_k / h_j) z_j for k > j and z_j /= |z_j|.
+  const double ctol = 1e-3 * tnorm;
+  for (int s0 = 0; s0 < n;) {
+    int e0 = s0 + 1;
+    while (e0 < n && eg[e0 - 1] != 0.0 && lam[e0] - lam[e0 - 1] <= ctol) ++e0;  // within one block
+    if (e0 - s0 > 1) {
+      for (int sweep = 0; sweep < 2; ++sweep)
+        for (int j = s0; j < e0; ++j) {
+          const int cnt = e0 - j;  // columns j .. e0-1
+          for (int kk = g8; kk < cnt; kk += TRI_THREADS / 8) {
+            double h = 0.0;
+            for (int i = l8; i < n; i += 8) h += Z[i * ld + j] * Z[i * ld + (j + kk)];
+            h = group8_sum(h);
+            if (l8 == 0) hb[kk] = h;
+          }
+          __syncthreads();
+          const double hj = hb[0];
+          const double inv = 1.0 / sqrt(hj);
+          for (int i = r32; i < n; i += 32) {
+            const double zj = Z[i * ld + j];
+            for (int kk = 1 + c32; kk < cnt; kk += 32) Z[i * ld + (j + kk)] -= (hb[kk] / hj) * zj;
+          }
+          __syncthreads();
+          if (tid < n) Z[tid * ld + j] *= inv;
+          __syncthreads();
+        }
+    }
+    s0 = e0;
+  }
+  // 5. Z <- H_0 ... H_{n-3} Z
+  for (int k = n - 3; k >= 0; --k) {
+    const double t = tau[k];
+    if (t == 0.0) continue;
+    const int m = n - k - 1;
+    if (tid < m) vb[tid] = hv[(size_t)k * TRI_MAX + tid];
+    __syncthreads();
+    for (int c = g8; c < n; c += TRI_THREADS / 8) {  // y_c = t v^T Z[k+1.., c], eight lanes per column
+      double acc = 0.0;
+      for (int r = l8; r < m; r += 8) acc += vb[r] * Z[(k + 1 + r) * ld + c];
+      acc = group8_sum(acc);
+      if (l8 == 0) pb[c] = t * acc;
+    }
+    __syncthreads();
+    for (int r = r32; r < m; r += 32) {
+      const double vr = vb[r];
+      double* row = Z + (k + 1 + r) * ld;
+      for (int c = c32; c < n; c += 32) row[c] -= vr * pb[c];
+    }
+    __syncthreads();
+  }
+  double* vo = vout + off[b];
+  for (int i = r32; i < n; i += 32)
+    for (int j = c32; j < n; j += 32) vo[(size_t)i * n + j] = Z[i * ld + j];
+  for (int j = tid; j < n; j += TRI_THREADS) wout[eoff[b] + j] = lam[j];
+}
+
+int rho_tri_max_dim(int device) {
+  static int cached = -1;
+  if (cached >= 0) return cached;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
+  const int budget = lds - 8192;  // static LDS of the kernel
+  int nmax = 0;
+  for (int n = 3; n <= TRI_MAX; ++n)
+    if (n * (n + 1) * (int)sizeof(double) <= budget) nmax = n;
+  if (nmax > 0)
+    (void)hipFuncSetAttribute((const void*)k_rho_eig_tri, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              nmax * (nmax + 1) * (int)sizeof(double));
+  cached = nmax;
+  return cached;
+}
+
+void launch_rho_eig_tri(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
+                        double* wout, double* ws, int nblk, int nmax, hipStream_t s) {
+  if (nblk <= 0) return;
+  const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_rho_eig_tri, dim3(nblk), dim3(TRI_THREADS), lds, s, a, off, dim, eoff, vout, wout, ws);
   NSS_HIP(hipGetLastError());
 }
 

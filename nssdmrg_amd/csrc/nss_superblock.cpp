@@ -470,12 +470,22 @@ void nss_release_rocblas(nss_ctx* ctx) {
   }
 }
 
-// Largest rho block sent to the batched Jacobi kernel (larger ones go to
-// rocSOLVER on the auxiliary streams): NSS_JACOBI_MAX, default 64.
+// Largest rho block sent to the batched Jacobi kernel: NSS_JACOBI_MAX,
+// default 32.  Blocks above it up to NSS_RHO_TRI_MAX (default 128, 0 for
+// none) go to the tridiagonal-path kernel, larger ones to rocSOLVER; both on
+// the auxiliary streams.
 static int jacobi_limit() {
   static const int v = [] {
     const char* e = std::getenv("NSS_JACOBI_MAX");
-    return e ? std::max(0, std::atoi(e)) : 64;
+    return e ? std::max(0, std::atoi(e)) : 32;
+  }();
+  return v;
+}
+
+static int rho_tri_limit() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_RHO_TRI_MAX");
+    return e ? std::max(0, std::atoi(e)) : 128;
   }();
   return v;
 }
@@ -529,7 +539,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   struct Item {
     int job, side, q, d;
     int64_t roff, eoff;
-    int kind;  // 1 jacobi, 2 rocsolver
+    int kind;  // 1 jacobi, 2 rocsolver, 3 tridiagonal path
   };
   std::vector<Item> items;
   int64_t tot = 0;
@@ -606,25 +616,29 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   pt.next("rdm.jacobi");
   const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
   if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
+  const int tmax = std::min(rho_tri_max_dim(ctx->device), rho_tri_limit());
   int64_t etot = 0;
-  std::vector<int64_t> joff, jeoff;
-  std::vector<int32_t> jdim;
-  int jn = 0;
+  std::vector<int64_t> joff, jeoff, toff, teoff;
+  std::vector<int32_t> jdim, tdim;
+  int jn = 0, tn = 0;
   for (Item& it : items) {
     it.eoff = etot;
     etot += it.d;
+    it.kind = it.d <= jmax ? 1 : it.d <= tmax ? 3 : 2;
     if (ctx->prof) {
-      ctx->timers[it.d <= jmax ? "rdm.count_jacobi" : "rdm.count_rocsolver"] += 1;
-      if (it.d > jmax) ctx->timers["rdm.sum_dim_rocsolver"] += it.d;
+      ctx->timers[it.kind == 1 ? "rdm.count_jacobi" : it.kind == 3 ? "rdm.count_tri" : "rdm.count_rocsolver"] += 1;
+      if (it.kind == 2) ctx->timers["rdm.sum_dim_rocsolver"] += it.d;
     }
-    if (it.d <= jmax) {
-      it.kind = 1;
+    if (it.kind == 1) {
       joff.push_back(it.roff);
       jeoff.push_back(it.eoff);
       jdim.push_back(it.d);
       jn = std::max(jn, it.d);
-    } else {
-      it.kind = 2;
+    } else if (it.kind == 3) {
+      toff.push_back(it.roff);
+      teoff.push_back(it.eoff);
+      tdim.push_back(it.d);
+      tn = std::max(tn, it.d);
     }
   }
   DevArr<double> dD;
@@ -640,6 +654,27 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     NSS_HIP(hipMemcpyAsync(djeoff.p, jeoff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     NSS_HIP(hipMemcpyAsync(djdim.p, jdim.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     launch_jacobi_eig(rho.p, djoff.p, djdim.p, djeoff.p, rho.p, dD.p, nb, jn, ctx->stream);
+  }
+  // blocks above the Jacobi limit up to tmax: the tridiagonal-path kernel on
+  // auxiliary stream 0, concurrently with the Jacobi launch (rho is
+  // complete: the symmetry check above synchronised the context stream)
+  bool used_aux[nss_ctx::NAUX] = {};
+  DevArr<int64_t> dtoff, dteoff;
+  DevArr<int32_t> dtdim;
+  DevArr<double> tws;
+  if (!tdim.empty()) {
+    const int nb = (int)tdim.size();
+    hipStream_t st = nullptr;
+    (void)aux_handle(ctx, 0, &st);
+    used_aux[0] = true;
+    dtoff.resize(ctx->pool, nb);
+    dteoff.resize(ctx->pool, nb);
+    dtdim.resize(ctx->pool, nb);
+    tws.resize(ctx->pool, rho_tri_workspace(nb));
+    NSS_HIP(hipMemcpyAsync(dtoff.p, toff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    NSS_HIP(hipMemcpyAsync(dteoff.p, teoff.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    NSS_HIP(hipMemcpyAsync(dtdim.p, tdim.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    launch_rho_eig_tri(rho.p, dtoff.p, dtdim.p, dteoff.p, rho.p, dD.p, tws.p, nb, tn, st);
   }
   // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
   // size class (dims rounded up to a multiple of 64, padded slabs).  The
@@ -679,12 +714,11 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     DevArr<double> slab, sD, sE;
   };
   std::vector<ClassBatch> batches(calls.size());
-  bool used_aux[nss_ctx::NAUX] = {};
   {
     size_t c = 0;
     for (auto& kv : calls) {
       ClassBatch& B = batches[c];
-      const int k = (int)(c % nss_ctx::NAUX);
+      const int k = (int)((c + !tdim.empty()) % nss_ctx::NAUX);  // after the tridiagonal-path launch's stream
       rocblas_handle hnd = aux_handle(ctx, k, &B.st);
       used_aux[k] = true;
       ++c;

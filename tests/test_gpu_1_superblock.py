@@ -219,6 +219,30 @@ def test_rdm_eigh_large_blocks(ctx):
             assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12
 
 
+def test_rdm_eigh_tridiagonal_path_blocks(ctx):
+    # blocks of 33..128 (the tridiagonal-path kernel): full rank, rank 1 and 7
+    # with large null clusters, the 33 and 128 ends, and an all-zero block
+    dims = [(100, 7), (7, 100), (128, 128), (33, 33), (120, 1), (50, 200), (60, 60)]
+    sb = _random_sb(11, dims, 6)
+    upload(ctx, sb)
+    psi = O.splitmix64_uniform_fast(sb.n, 9)
+    z = len(dims) - 1
+    psi[sb.offset[z]:sb.offset[z] + dims[z][0] * dims[z][1]] = 0.0
+    psi /= np.linalg.norm(psi)
+    for side in ("left", "right"):
+        evs, vecs = ctx.rdm_eigh(psi, side)
+        for k, (ev, vec) in enumerate(zip(evs, vecs)):
+            dl, dr, off = sb.Dl[k], sb.Dr[k], sb.offset[k]
+            p = psi[off:off + dl * dr].reshape(dl, dr)
+            rho = p @ p.T if side == "left" else p.T @ p
+            ref = np.linalg.eigvalsh(rho)
+            print(f"case tri side={side} k={k} d={len(ev)}", flush=True)
+            assert np.all(np.diff(ev) >= 0), (side, k)
+            assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-15), (side, k)
+            assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14, (side, k)
+            assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12, (side, k)
+
+
 def test_rdm_eigh_rank_deficient_blocks(ctx):
     # rank-deficient rho blocks on both eigensolver paths (Jacobi up to 64,
     # rocSOLVER above): large null spaces, rank-1 blocks, a 2x2 and a full block
