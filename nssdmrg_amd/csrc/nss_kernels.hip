@@ -1262,7 +1262,8 @@ void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, 
 //     (dgttrf), from a per-vector pseudo-random start, the factors in the
 //     global workspace and the iterate in column t of Z (LDS);
 //  4. Gram-Schmidt within clusters of eigenvalues closer than 1e-3 ||T||,
-//     as dstein does (right-looking modified Gram-Schmidt, two sweeps);
+//     as dstein does (right-looking modified Gram-Schmidt, a second sweep
+//     where the first met near-degenerate iterates);
 //  5. Z <- Q Z, one reflector at a time.
 // T is split into unreduced blocks at |e_k| <= eps ||T|| first (dstebz) and
 // every eigenpair is computed within its block.  Output as the Jacobi
@@ -1275,8 +1276,61 @@ constexpr int64_t TRI_WS = (int64_t)6 * TRI_MAX * TRI_MAX;  // workspace doubles
 
 int64_t rho_tri_workspace(int nblk) { return (int64_t)nblk * TRI_WS; }
 
+// A double moved across lanes by a DPP control (both 32-bit halves): lane
+// exchanges within rows of 16 lanes at register speed, where a shuffle goes
+// through the LDS crossbar.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over the aligned group of 8 lanes, in each of them: quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror
+__device__ __forceinline__ double group8_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  return v;
+}
+
+// sum over the wave (every lane active), in every lane: groups of 8, then
+// row_mirror for rows of 16, then the four rows
+__device__ __forceinline__ double tri_wave_sum(double v) {
+  v = group8_sum(v);
+  v += dpp_f64<0x140>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// max over the workgroup, in every thread (the barriers also order the
+// LDS work around it)
+__device__ __forceinline__ double tri_block_max(double v, double* red) {
+  v = fmax(v, dpp_f64<0xB1>(v));
+  v = fmax(v, dpp_f64<0x4E>(v));
+  v = fmax(v, dpp_f64<0x141>(v));
+  v = fmax(v, dpp_f64<0x140>(v));
+  v = fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < TRI_WAVES; ++w) r = fmax(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
 __device__ __forceinline__ double tri_block_sum(double v, double* red) {  // all threads get the sum
-  v = wave_sum(v);
+  v = tri_wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = 0.0;
@@ -1286,13 +1340,6 @@ __device__ __forceinline__ double tri_block_sum(double v, double* red) {  // all
   return r;
 }
 
-// sum over the aligned group of 8 lanes
-__device__ __forceinline__ double group8_sum(double v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
-}
 
 // 1/q to full precision: the hardware reciprocal and two Newton steps
 __device__ __forceinline__ double tri_rcp(double q) {
@@ -1365,12 +1412,15 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
       __syncthreads();
       const double pv = (tid < m) ? pb[tid] * vb[tid] : 0.0;
       const double a2 = -0.5 * t * tri_block_sum(pv, red);
-      if (tid < m) pb[tid] += a2 * vb[tid];  // w = p + a2 v
-      __syncthreads();
-      for (int r = r32; r < m; r += 32) {  // A22 -= v w^T + w v^T
-        const double vr = vb[r], wr = pb[r];
+      // A22 -= v w^T + w v^T with w = p + a2 v, i.e. v p^T + p v^T + 2 a2 v v^T
+      const double a22 = 2.0 * a2;
+      for (int r = r32; r < m; r += 32) {
+        const double vr = vb[r], pr = pb[r];
         double* row = A + (k + 1 + r) * ld + (k + 1);
-        for (int c = c32; c < m; c += 32) row[c] -= vr * pb[c] + wr * vb[c];
+        for (int c = c32; c < m; c += 32) {
+          const double vc = vb[c];
+          row[c] -= vr * (pb[c] + a22 * vc) + pr * vc;
+        }
       }
       __syncthreads();
     }
@@ -1550,16 +1600,20 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
   __syncthreads();
   // 4. clusters: consecutive eigenvalues of one block within 1e-3 ||T|| of
   //    each other (dstein's reorthogonalisation criterion); right-looking modified
-  //    Gram-Schmidt, two sweeps.  Step j: h_k = z_j . z_k for k >= j (eight
-  //    lanes per dot), then z_k -= (h_k / h_j) z_j for k > j and z_j /= |z_j|.
+  //    Gram-Schmidt, up to two sweeps.  Step j: h_k = z_j . z_k for k >= j (eight
+  //    lanes per dot), then z_k -= (h_k / h_j) z_j for k > j; z_j /= |z_j|
+  //    during step j+1, which reads neither z_j nor its scale.
   const double ctol = 1e-3 * tnorm;
   for (int s0 = 0; s0 < n;) {
     int e0 = s0 + 1;
     while (e0 < n && eg[e0 - 1] != 0.0 && lam[e0] - lam[e0 - 1] <= ctol) ++e0;  // within one block
     if (e0 - s0 > 1) {
-      for (int sweep = 0; sweep < 2; ++sweep)
+      for (int sweep = 0; sweep < 2; ++sweep) {
+        double inv = 1.0;   // scale of z_{j-1}
+        double ovl = 0.0;   // largest |h_k| / h_j this thread applied
         for (int j = s0; j < e0; ++j) {
           const int cnt = e0 - j;  // columns j .. e0-1
+          if (j > s0 && tid < n) Z[tid * ld + (j - 1)] *= inv;
           for (int kk = g8; kk < cnt; kk += TRI_THREADS / 8) {
             double h = 0.0;
             for (int i = l8; i < n; i += 8) h += Z[i * ld + j] * Z[i * ld + (j + kk)];
@@ -1568,15 +1622,23 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
           }
           __syncthreads();
           const double hj = hb[0];
-          const double inv = 1.0 / sqrt(hj);
+          inv = 1.0 / sqrt(hj);
           for (int i = r32; i < n; i += 32) {
             const double zj = Z[i * ld + j];
-            for (int kk = 1 + c32; kk < cnt; kk += 32) Z[i * ld + (j + kk)] -= (hb[kk] / hj) * zj;
+            for (int kk = 1 + c32; kk < cnt; kk += 32) {
+              const double c = hb[kk] / hj;
+              ovl = fmax(ovl, fabs(c));
+              Z[i * ld + (j + kk)] -= c * zj;
+            }
           }
           __syncthreads();
-          if (tid < n) Z[tid * ld + j] *= inv;
-          __syncthreads();
         }
+        if (tid < n) Z[tid * ld + (e0 - 1)] *= inv;
+        // a second sweep only where the first removed overlaps above 1e-6
+        // (near-degenerate iterates); below that one sweep leaves the
+        // cluster orthogonal to working precision
+        if (!(tri_block_max(ovl, red) > 1e-6)) break;
+      }
     }
     s0 = e0;
   }
