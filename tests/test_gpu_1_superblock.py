@@ -244,8 +244,9 @@ def test_rdm_eigh_tridiagonal_path_blocks(ctx):
 
 
 def test_rdm_eigh_rank_deficient_blocks(ctx):
-    # rank-deficient rho blocks on both eigensolver paths (Jacobi up to 64,
-    # rocSOLVER above): large null spaces, rank-1 blocks, a 2x2 and a full block
+    # rank-deficient rho blocks on the three eigensolver paths (Jacobi up to
+    # 32, the tridiagonal-path kernel up to 128, rocSOLVER above): large null
+    # spaces, rank-1 blocks, a 2x2 and a full block
     dims = [(135, 5), (5, 135), (2, 2), (64, 64), (135, 1), (1, 136)]
     sb = _random_sb(7, dims, 6)
     upload(ctx, sb)
