@@ -283,7 +283,7 @@ struct nss_ctx {
   // auxiliary streams (default priority), each with its own rocblas handle,
   // created lazily: independent batches of the rho eigensolve run on them
   // concurrently with the context stream
-  static constexpr int NAUX = 3;
+  static constexpr int NAUX = 4;  // [0, NAUX-1): rocSOLVER rho classes; NAUX-1: the tridiagonal-path rho kernel
   hipStream_t aux[NAUX] = {};
   void* aux_rocblas[NAUX] = {};
   nss::SbDevice* sb = nullptr;

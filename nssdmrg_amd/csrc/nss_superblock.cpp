@@ -656,8 +656,9 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     launch_jacobi_eig(rho.p, djoff.p, djdim.p, djeoff.p, rho.p, dD.p, nb, jn, ctx->stream);
   }
   // blocks above the Jacobi limit up to tmax: the tridiagonal-path kernel on
-  // auxiliary stream 0, concurrently with the Jacobi launch (rho is
-  // complete: the symmetry check above synchronised the context stream)
+  // the last auxiliary stream, concurrently with the Jacobi launch and the
+  // rocSOLVER classes (rho is complete: the symmetry check above
+  // synchronised the context stream)
   bool used_aux[nss_ctx::NAUX] = {};
   DevArr<int64_t> dtoff, dteoff;
   DevArr<int32_t> dtdim;
@@ -665,8 +666,8 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   if (!tdim.empty()) {
     const int nb = (int)tdim.size();
     hipStream_t st = nullptr;
-    (void)aux_handle(ctx, 0, &st);
-    used_aux[0] = true;
+    (void)aux_handle(ctx, nss_ctx::NAUX - 1, &st);
+    used_aux[nss_ctx::NAUX - 1] = true;
     dtoff.resize(ctx->pool, nb);
     dteoff.resize(ctx->pool, nb);
     dtdim.resize(ctx->pool, nb);
@@ -718,7 +719,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     size_t c = 0;
     for (auto& kv : calls) {
       ClassBatch& B = batches[c];
-      const int k = (int)((c + !tdim.empty()) % nss_ctx::NAUX);  // after the tridiagonal-path launch's stream
+      const int k = (int)(c % (nss_ctx::NAUX - 1));
       rocblas_handle hnd = aux_handle(ctx, k, &B.st);
       used_aux[k] = true;
       ++c;
