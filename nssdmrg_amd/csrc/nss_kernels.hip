@@ -1255,8 +1255,8 @@ void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, 
 //     H_0 ... H_{n-3}, H_k = I - tau_k v_k v_k^T, the v_k kept in a global
 //     workspace), the p = tau A v products eight lanes per row;
 //  2. every eigenvalue by multisection on Sturm counts (Wilkinson,
-//     The Algebraic Eigenvalue Problem, 5.37-5.39), eight lanes per
-//     eigenvalue, each evaluating one of eight interior points per step;
+//     The Algebraic Eigenvalue Problem, 5.37-5.39), TRI_BIS lanes per
+//     eigenvalue, each evaluating one of TRI_BIS interior points per step;
 //  3. its eigenvector by inverse iteration (dstein): two solves with
 //     T - lambda I, factored by Gaussian elimination with partial pivoting
 //     (dgttrf), from a per-vector pseudo-random start, the factors in the
@@ -1272,6 +1272,7 @@ void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, 
 constexpr int TRI_THREADS = 1024;
 constexpr int TRI_WAVES = TRI_THREADS / 64;
 constexpr int TRI_MAX = 128;
+constexpr int TRI_BIS = 4;  // lanes (and interior points) per eigenvalue in the multisection
 constexpr int64_t TRI_WS = (int64_t)6 * TRI_MAX * TRI_MAX;  // workspace doubles per block
 
 int64_t rho_tri_workspace(int nblk) { return (int64_t)nblk * TRI_WS; }
@@ -1357,7 +1358,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
   extern __shared__ double sm[];  // n x (n+1): A, then Z
   __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
   __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX];
-  __shared__ double red[TRI_WAVES];
+  __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
   const int b = blockIdx.x;
   const int n = dim[b];
   const int ld = n + 1;
@@ -1373,14 +1374,24 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
   __syncthreads();
   // 1. tridiagonalisation (every branch below is uniform: its inputs are
   //    LDS values or block sums that every thread reads alike)
+  //    Three barriers per reflector: the sums of squares of the next column
+  //    and p.v are reduced per wave before the barrier that follows their
+  //    producing phase, and summed over the waves after it.
+  bool have_s2 = false;  // red2 holds the wave partials of the next column's squares
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;  // x = A[k+1..n-1][k]
-    double s2 = 0.0;
-    if (tid + k + 2 < n) {
-      const double x = A[(tid + k + 2) * ld + k];
-      s2 = x * x;
+    double xn2 = 0.0;
+    if (have_s2) {
+#pragma unroll
+      for (int w = 0; w < TRI_WAVES; ++w) xn2 += red2[w];
+    } else {
+      double s2 = 0.0;
+      if (tid + k + 2 < n) {
+        const double x = A[(tid + k + 2) * ld + k];
+        s2 = x * x;
+      }
+      xn2 = tri_block_sum(s2, red);
     }
-    const double xn2 = tri_block_sum(s2, red);
     const double alpha = A[(k + 1) * ld + k];
     double t = 0.0, beta = alpha, scale = 0.0;
     if (xn2 > 0.0) {
@@ -1399,7 +1410,9 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
       dg[k] = A[k * ld + k];
     }
     __syncthreads();
+    have_s2 = false;
     if (t != 0.0) {
+      double pvl = 0.0;
       {  // p = t A22 v, eight threads per row
         double acc = 0.0;
         if (g8 < m) {
@@ -1407,22 +1420,37 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
           for (int c = l8; c < m; c += 8) acc += row[c] * vb[c];
         }
         acc = group8_sum(acc);
-        if (g8 < m && l8 == 0) pb[g8] = t * acc;
+        if (g8 < m && l8 == 0) {
+          const double pr = t * acc;
+          pb[g8] = pr;
+          pvl = pr * vb[g8];
+        }
       }
+      pvl = tri_wave_sum(pvl);
+      if ((tid & 63) == 0) red3[tid >> 6] = pvl;
       __syncthreads();
-      const double pv = (tid < m) ? pb[tid] * vb[tid] : 0.0;
-      const double a2 = -0.5 * t * tri_block_sum(pv, red);
-      // A22 -= v w^T + w v^T with w = p + a2 v, i.e. v p^T + p v^T + 2 a2 v v^T
+      double pv = 0.0;
+#pragma unroll
+      for (int w = 0; w < TRI_WAVES; ++w) pv += red3[w];
+      const double a2 = -0.5 * t * pv;
+      // A22 -= v w^T + w v^T with w = p + a2 v, i.e. v p^T + p v^T + 2 a2 v v^T;
+      // the new column k+1 below its subdiagonal feeds the next step's norm
       const double a22 = 2.0 * a2;
+      double s2n = 0.0;
       for (int r = r32; r < m; r += 32) {
         const double vr = vb[r], pr = pb[r];
         double* row = A + (k + 1 + r) * ld + (k + 1);
         for (int c = c32; c < m; c += 32) {
           const double vc = vb[c];
-          row[c] -= vr * (pb[c] + a22 * vc) + pr * vc;
+          const double nv = row[c] - (vr * (pb[c] + a22 * vc) + pr * vc);
+          row[c] = nv;
+          if (c == 0 && r >= 2) s2n += nv * nv;
         }
       }
+      s2n = tri_wave_sum(s2n);
+      if ((tid & 63) == 0) red2[tid >> 6] = s2n;
       __syncthreads();
+      have_s2 = true;
     }
   }
   if (tid == 0) {
@@ -1462,41 +1490,41 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
     while (s > 0 && eg[s - 1] != 0.0) --s;
     while (e < n && eg[e - 1] != 0.0) ++e;
   };
-  // 2. eigenvalues by multisection, the 8 lanes of a group evaluating the
-  //    Sturm count of the block at 8 interior points of the interval each step
+  // 2. eigenvalues by multisection: the TRI_BIS lanes of a group evaluate the
+  //    Sturm count of the block at TRI_BIS interior points of the interval
+  //    each step
   {
-    const int p = g8;
+    const int gb = tid / TRI_BIS, lb = tid % TRI_BIS;
+    const int p = gb;
     int bs = 0, be = 1;
     if (p < n) block_of(p, bs, be);
     const int t = p - bs;
     double lo = lo0, hi = hi0;
-    if (p < n && be - bs == 1) lo = hi = dg[p];
+    if (p >= n || be - bs == 1) lo = hi = (p < n ? dg[p] : 0.0);
     const double tol = 2.0 * DBL_EPSILON * tnorm;
-    for (int it = 0; it < 64; ++it) {
-      if (!(hi - lo > tol)) break;  // alike on the 8 lanes of a group
-      const double x = lo + (hi - lo) * (double)(l8 + 1) * (1.0 / 9.0);
-      int cnt = 0;
-      if (p < n) {
-        double q = dg[bs] - x;
+    constexpr double inv_parts = 1.0 / (TRI_BIS + 1);
+    for (int it = 0; it < 96; ++it) {
+      if (!(hi - lo > tol)) break;  // alike on the lanes of a group
+      const double x = lo + (hi - lo) * (double)(lb + 1) * inv_parts;
+      double q = dg[bs] - x;
+      if (fabs(q) < pivmin) q = -pivmin;
+      int cnt = q < 0.0;
+      for (int i = bs + 1; i < be; ++i) {
+        q = dg[i] - x - e2g[i - 1] * tri_rcp(q);
         if (fabs(q) < pivmin) q = -pivmin;
-        cnt = q < 0.0;
-        for (int i = bs + 1; i < be; ++i) {
-          q = dg[i] - x - e2g[i - 1] * tri_rcp(q);
-          if (fabs(q) < pivmin) q = -pivmin;
-          cnt += q < 0.0;
-        }
+        cnt += q < 0.0;
       }
       const bool above = cnt >= t + 1;  // eigenvalue t lies below x
       const uint64_t bal = __ballot(above);
-      const unsigned bits = (unsigned)(bal >> (threadIdx.x & 56)) & 0xFFu;
-      const int f = bits ? __builtin_ctz(bits) : 8;  // first point above it
+      const unsigned bits = (unsigned)(bal >> (threadIdx.x & (64 - TRI_BIS))) & ((1u << TRI_BIS) - 1u);
+      const int f = bits ? __builtin_ctz(bits) : TRI_BIS;  // first point above it
       const double span = hi - lo;
-      const double nlo = f > 0 ? lo + span * (double)f * (1.0 / 9.0) : lo;
-      const double nhi = f < 8 ? lo + span * (double)(f + 1) * (1.0 / 9.0) : hi;
+      const double nlo = f > 0 ? lo + span * (double)f * inv_parts : lo;
+      const double nhi = f < TRI_BIS ? lo + span * (double)(f + 1) * inv_parts : hi;
       lo = nlo;
       hi = fmax(nhi, nlo);
     }
-    if (p < n && l8 == 0) lam[p] = 0.5 * (lo + hi);
+    if (p < n && lb == 0) lam[p] = 0.5 * (lo + hi);
   }
   __syncthreads();
   double* Z = sm;  // A is no longer read: the reflectors are in hv, T in dg/eg
