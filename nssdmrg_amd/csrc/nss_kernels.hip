@@ -1264,7 +1264,7 @@ void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, 
 //  4. Gram-Schmidt within clusters of eigenvalues closer than 1e-3 ||T||,
 //     as dstein does (right-looking modified Gram-Schmidt, a second sweep
 //     where the first met near-degenerate iterates);
-//  5. Z <- Q Z, one reflector at a time.
+//  5. Z <- Q Z, two reflectors per pass.
 // T is split into unreduced blocks at |e_k| <= eps ||T|| first (dstebz) and
 // every eigenpair is computed within its block.  Output as the Jacobi
 // kernel's: eigenvectors as columns (row-major), eigenvalues at wout +
@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
                                                              double* __restrict__ ws) {
   extern __shared__ double sm[];  // n x (n+1): A, then Z
   __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
-  __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX];
+  __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX], yb[TRI_MAX];
   __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
   const int b = blockIdx.x;
   const int n = dim[b];
@@ -1670,23 +1670,70 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
     }
     s0 = e0;
   }
-  // 5. Z <- H_0 ... H_{n-3} Z
-  for (int k = n - 3; k >= 0; --k) {
-    const double t = tau[k];
-    if (t == 0.0) continue;
-    const int m = n - k - 1;
-    if (tid < m) vb[tid] = hv[(size_t)k * TRI_MAX + tid];
+  // 5. Z <- H_0 ... H_{n-3} Z, two reflectors per pass: with k1 = k0 + 1,
+  //    H_k0 H_k1 Z = Z - v1 y1^T - v0 y0^T, y1 = t1 Z^T v1 and
+  //    y0 = t0 (Z^T v0 - (v0.v1) y1), both over the rows k1.. (v1 has a zero
+  //    there); one load, one product and one update phase per pair
+  int k1 = n - 3;
+  for (; k1 >= 1; k1 -= 2) {
+    const int k0 = k1 - 1;
+    const int m0 = n - k1;  // rows k1 .. n-1
+    const double t1 = tau[k1], t0 = tau[k0];
+    double sl = 0.0;
+    if (tid < m0) {
+      const double v0 = hv[(size_t)k0 * TRI_MAX + tid];  // reflector k0 starts at row k0+1 = k1
+      const double v1 = tid == 0 ? 0.0 : hv[(size_t)k1 * TRI_MAX + tid - 1];
+      vb[tid] = v0;
+      hb[tid] = v1;
+      sl = v0 * v1;
+    }
+    sl = tri_wave_sum(sl);
+    if ((tid & 63) == 0) red2[tid >> 6] = sl;
     __syncthreads();
-    for (int c = g8; c < n; c += TRI_THREADS / 8) {  // y_c = t v^T Z[k+1.., c], eight lanes per column
+    for (int c = g8; c < n; c += TRI_THREADS / 8) {
+      double a1 = 0.0, a0 = 0.0;
+      for (int r = l8; r < m0; r += 8) {
+        const double z = Z[(k1 + r) * ld + c];
+        a1 += hb[r] * z;
+        a0 += vb[r] * z;
+      }
+      a1 = group8_sum(a1);
+      a0 = group8_sum(a0);
+      if (l8 == 0) {
+        pb[c] = t1 * a1;
+        yb[c] = a0;
+      }
+    }
+    __syncthreads();
+    double s01 = 0.0;
+#pragma unroll
+    for (int w = 0; w < TRI_WAVES; ++w) s01 += red2[w];
+    for (int r = r32; r < m0; r += 32) {
+      const double v0 = vb[r], v1 = hb[r];
+      double* row = Z + (k1 + r) * ld;
+      for (int c = c32; c < n; c += 32) {
+        const double y1 = pb[c];
+        const double y0 = t0 * (yb[c] - s01 * y1);
+        row[c] -= v1 * y1 + v0 * y0;
+      }
+    }
+    __syncthreads();
+  }
+  if (k1 == 0 && tau[0] != 0.0) {  // the last single reflector
+    const double t = tau[0];
+    const int m = n - 1;
+    if (tid < m) vb[tid] = hv[tid];
+    __syncthreads();
+    for (int c = g8; c < n; c += TRI_THREADS / 8) {
       double acc = 0.0;
-      for (int r = l8; r < m; r += 8) acc += vb[r] * Z[(k + 1 + r) * ld + c];
+      for (int r = l8; r < m; r += 8) acc += vb[r] * Z[(1 + r) * ld + c];
       acc = group8_sum(acc);
       if (l8 == 0) pb[c] = t * acc;
     }
     __syncthreads();
     for (int r = r32; r < m; r += 32) {
       const double vr = vb[r];
-      double* row = Z + (k + 1 + r) * ld;
+      double* row = Z + (1 + r) * ld;
       for (int c = c32; c < n; c += 32) row[c] -= vr * pb[c];
     }
     __syncthreads();
@@ -1702,7 +1749,7 @@ int rho_tri_max_dim(int device) {
   if (cached >= 0) return cached;
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
-  const int budget = lds - 8192;  // static LDS of the kernel
+  const int budget = lds - 16384;  // static LDS of the kernel (about 10 KB)
   int nmax = 0;
   for (int n = 3; n <= TRI_MAX; ++n)
     if (n * (n + 1) * (int)sizeof(double) <= budget) nmax = n;
