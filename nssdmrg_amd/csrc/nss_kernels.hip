@@ -1590,6 +1590,9 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
       }
       for (int pass = 0; pass < 2; ++pass) {
         double xi = Zb[0];
+        // (unrolled so that the factor loads of several rows are in flight
+        // at once: they do not depend on the iterate)
+#pragma unroll 8
         for (int i = 0; i + 1 < nb; ++i) {  // L, with the row interchanges
           double xn = Zb[(i + 1) * ld];
           if (PV[(size_t)i * TRI_MAX] != 0.0) {
@@ -1607,6 +1610,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
         Zb[(nb - 2) * ld] = x0;
         double x2 = x1;
         x1 = x0;
+#pragma unroll 8
         for (int i = nb - 3; i >= 0; --i) {
           x0 = (Zb[i * ld] - DU[(size_t)i * TRI_MAX] * x1 - DU2[(size_t)i * TRI_MAX] * x2) / DD[(size_t)i * TRI_MAX];
           Zb[i * ld] = x0;
