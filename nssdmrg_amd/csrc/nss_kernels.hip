@@ -1509,6 +1509,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
       double q = dg[bs] - x;
       if (fabs(q) < pivmin) q = -pivmin;
       int cnt = q < 0.0;
+#pragma unroll 8
       for (int i = bs + 1; i < be; ++i) {
         q = dg[i] - x - e2g[i - 1] * tri_rcp(q);
         if (fabs(q) < pivmin) q = -pivmin;
@@ -1648,6 +1649,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
           if (j > s0 && tid < n) Z[tid * ld + (j - 1)] *= inv;
           for (int kk = g8; kk < cnt; kk += TRI_THREADS / 8) {
             double h = 0.0;
+#pragma unroll 4
             for (int i = l8; i < n; i += 8) h += Z[i * ld + j] * Z[i * ld + (j + kk)];
             h = group8_sum(h);
             if (l8 == 0) hb[kk] = h;
