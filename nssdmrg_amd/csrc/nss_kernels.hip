@@ -1437,14 +1437,24 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
       // the new column k+1 below its subdiagonal feeds the next step's norm
       const double a22 = 2.0 * a2;
       double s2n = 0.0;
+      double vcq[4], wcq[4];  // this thread's columns c32 + 32q (m <= 127), read once
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = c32 + 32 * q;
+        vcq[q] = c < m ? vb[c] : 0.0;
+        wcq[q] = c < m ? pb[c] + a22 * vb[c] : 0.0;
+      }
       for (int r = r32; r < m; r += 32) {
         const double vr = vb[r], pr = pb[r];
         double* row = A + (k + 1 + r) * ld + (k + 1);
-        for (int c = c32; c < m; c += 32) {
-          const double vc = vb[c];
-          const double nv = row[c] - (vr * (pb[c] + a22 * vc) + pr * vc);
-          row[c] = nv;
-          if (c == 0 && r >= 2) s2n += nv * nv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = c32 + 32 * q;
+          if (c < m) {
+            const double nv = row[c] - (vr * wcq[q] + pr * vcq[q]);
+            row[c] = nv;
+            if (c == 0 && r >= 2) s2n += nv * nv;
+          }
         }
       }
       s2n = tri_wave_sum(s2n);
@@ -1714,13 +1724,20 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
     double s01 = 0.0;
 #pragma unroll
     for (int w = 0; w < TRI_WAVES; ++w) s01 += red2[w];
+    double y1q[4], y0q[4];  // this thread's columns c32 + 32q (n <= 128), read once
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = c32 + 32 * q;
+      y1q[q] = c < n ? pb[c] : 0.0;
+      y0q[q] = c < n ? t0 * (yb[c] - s01 * pb[c]) : 0.0;
+    }
     for (int r = r32; r < m0; r += 32) {
       const double v0 = vb[r], v1 = hb[r];
       double* row = Z + (k1 + r) * ld;
-      for (int c = c32; c < n; c += 32) {
-        const double y1 = pb[c];
-        const double y0 = t0 * (yb[c] - s01 * y1);
-        row[c] -= v1 * y1 + v0 * y0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = c32 + 32 * q;
+        if (c < n) row[c] -= v1 * y1q[q] + v0 * y0q[q];
       }
     }
     __syncthreads();
