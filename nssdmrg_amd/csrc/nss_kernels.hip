@@ -1667,12 +1667,19 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
           __syncthreads();
           const double hj = hb[0];
           inv = 1.0 / sqrt(hj);
+          double cq[4];  // this thread's coefficients h_k / h_j, k = j + 1 + c32 + 32q
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kk = 1 + c32 + 32 * q;
+            cq[q] = kk < cnt ? hb[kk] / hj : 0.0;
+            ovl = fmax(ovl, fabs(cq[q]));
+          }
           for (int i = r32; i < n; i += 32) {
             const double zj = Z[i * ld + j];
-            for (int kk = 1 + c32; kk < cnt; kk += 32) {
-              const double c = hb[kk] / hj;
-              ovl = fmax(ovl, fabs(c));
-              Z[i * ld + (j + kk)] -= c * zj;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int kk = 1 + c32 + 32 * q;
+              if (kk < cnt) Z[i * ld + (j + kk)] -= cq[q] * zj;
             }
           }
           __syncthreads();
