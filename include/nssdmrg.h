@@ -76,6 +76,17 @@ int nss_sb_upload_blocks(nss_ctx* ctx, const double* const* HL, const double* co
 int nss_sb_hv(nss_ctx* ctx, const double* v, double* hv, int64_t n);
 /* Same on device pointers, enqueued on the context stream (no host sync). */
 int nss_sb_hv_device(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n);
+/* Largest term-product scratch (doubles, 128 MiB) of nss_sb_hv_device_noalloc. */
+#define NSS_NOALLOC_W_MAX (16LL * 1024 * 1024)
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The result of
+ * nss_sb_hv_device, bit for bit, with no hipMalloc/hipFree, no pool
+ * allocation and no timing events: the term products W_t go to a file-scope
+ * device array of NSS_NOALLOC_W_MAX doubles that each device holds until the
+ * process ends.  A superblock whose term products need more is refused
+ * (NSS_E_ARG).  Every call on one device shares that array, so calls that
+ * overlap on one device (several contexts or streams) race. */
+int nss_sb_hv_device_noalloc(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n);
 /* H*v timing recorded with HIP events on the context stream, on every k-th
  * H*v of nss_sb_hv*, nss_lanczos and the engine (k from the environment
  * variable NSS_HV_TIMING when the context is created: default 4, 1 times

@@ -73,6 +73,7 @@ SIGNATURES = {
     "nss_sb_upload_blocks": (C.c_int, [_P, _DPP, _DPP, C.c_int, _I32, _I32, _DPP, _DPP, C.c_int, C.c_int]),
     "nss_sb_hv": (C.c_int, [_P, _D, _D, C.c_int64]),
     "nss_sb_hv_device": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "nss_sb_hv_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int64]),
     "nss_ctx_hv_stats": (C.c_int, [_P, C.c_int, _D]),
     "nss_ctx_timers": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "nss_sb_hv_fp32": (C.c_int, [_P, _D, _D, C.c_int64]),

@@ -313,6 +313,9 @@ struct nss_ctx {
   // engines created on this context: destroyed (their device memory
   // returned) before the context's pool goes away
   std::vector<struct nss_dmrg*> engines;
+  // this device's copy of the file-scope scratch array of the allocation-free
+  // H*v entry point (nss_sb_hv_device_noalloc), resolved once per device
+  double* noalloc_scratch = nullptr;
   // phase timers (NSS_PROFILE=1): host wall time with a stream sync at the
   // end of each timed phase; off by default (no extra syncs)
   bool prof = false;

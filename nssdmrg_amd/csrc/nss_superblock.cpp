@@ -861,6 +861,34 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
 
 using namespace nss;
 
+// ------------------------------------------------------------------ scratch
+// Device scratch of the allocation-free H*v entry point: the term products
+// W_t = Psi_src B_t^T of one call (the role of the W buffer that
+// nss_sb_upload_blocks allocates for nss_sb_hv_device).  Every device the
+// library runs on holds one copy of NSS_NOALLOC_W_MAX doubles (128 MiB) from
+// module load until the process ends; all calls on a device share it.  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// its per-device address comes from hipGetSymbolAddress (noalloc_scratch).
+static __device__ __align__(256) double scratch[NSS_NOALLOC_W_MAX];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_addr;  // device -> address of its copy of `scratch`
+}  // namespace
+
+// Address of `scratch` on `device`, which must be the current device;
+// resolved once per device.
+static double* noalloc_scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) return it->second;
+  void* p = nullptr;
+  NSS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  NSS_REQUIRE(p != nullptr, NSS_E_HIP, "hipGetSymbolAddress returned no address for the H*v scratch");
+  g_scratch_addr[device] = static_cast<double*>(p);
+  return static_cast<double*>(p);
+}
+
 // ------------------------------------------------------------------ C ABI
 #define NSS_API_BEGIN try {
 #define NSS_API_END                                  \
@@ -902,6 +930,15 @@ extern "C" int nss_ctx_create(int device, nss_ctx** out) {
   NSS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->pool = new DevicePool(device);
   c->sb = sb_create(c);
+  try {
+    c->noalloc_scratch = noalloc_scratch_address(device);
+  } catch (...) {
+    (void)hipStreamDestroy(c->stream);
+    sb_destroy(c->sb);
+    delete c->pool;
+    delete c;
+    throw;
+  }
   const char* pe = std::getenv("NSS_PROFILE");
   c->prof = pe && pe[0] == '1';
   const char* te = std::getenv("NSS_HV_TIMING");  // period of the timed H*v calls (0: none)
@@ -1055,6 +1092,29 @@ extern "C" int nss_sb_hv_device(nss_ctx* ctx, const double* d_v, double* d_hv, i
   NSS_REQUIRE(n == ctx->sb->n && n > 0, NSS_E_ARG, "nss_sb_hv_device: n does not match the layout");
   NSS_HIP(hipSetDevice(ctx->device));
   sb_hv(ctx, ctx->sb, d_v, d_hv);
+  NSS_API_END
+}
+
+// H*v of the current superblock on device vectors with no device allocation
+// and no timing events: the term products go to this device's copy of the
+// file-scope scratch array.  Enqueued on the context stream like
+// nss_sb_hv_device; calls that overlap on one device (other contexts or
+// streams) race on the shared scratch.
+extern "C" int nss_sb_hv_device_noalloc(nss_ctx* ctx, const double* d_v, double* d_hv, int64_t n) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(ctx && d_v && d_hv, NSS_E_ARG, "nss_sb_hv_device_noalloc: NULL arguments");
+  NSS_REQUIRE(n == ctx->sb->n && n > 0, NSS_E_ARG, "nss_sb_hv_device_noalloc: n does not match the layout");
+  NSS_REQUIRE(ctx->noalloc_scratch != nullptr, NSS_E_STATE, "nss_sb_hv_device_noalloc: scratch not resolved");
+  const SbDevice* sb = ctx->sb;
+  NSS_REQUIRE(!sb->W64.htiles.empty() || !sb->Y64.htiles.empty(), NSS_E_STATE,
+              "nss_sb_hv_device_noalloc: call nss_sb_upload_blocks first");
+  NSS_REQUIRE(sb->wsize <= NSS_NOALLOC_W_MAX, NSS_E_ARG,
+              "nss_sb_hv_device_noalloc: the superblock's term products need " + std::to_string(sb->wsize) +
+                  " doubles, above the scratch's " + std::to_string((long long)NSS_NOALLOC_W_MAX));
+  NSS_HIP(hipSetDevice(ctx->device));
+  const Bases b{d_v, ctx->noalloc_scratch, d_hv};
+  launch_gemm_f64(sb->W64, b, ctx->stream, GEMM_HV_W);
+  launch_gemm_f64(sb->Y64, b, ctx->stream, GEMM_HV_Y);
   NSS_API_END
 }
 
