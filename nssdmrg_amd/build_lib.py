@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnssdmrg.so")
-SOURCES = ["nss_kernels.hip", "nss_superblock.cpp", "nss_lanczos.cpp", "nss_eig.cpp", "nss_engine.cpp"]
+SOURCES = ["nss_kernels.hip", "nss_rho_big.hip", "nss_superblock.cpp", "nss_lanczos.cpp", "nss_eig.cpp", "nss_engine.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 (v_pk_*_f32) generation is disabled for all device code.

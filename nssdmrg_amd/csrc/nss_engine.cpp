@@ -678,7 +678,7 @@ struct Engine {
   }
 
   // renormalize_block + operators_renormalization (src/DMRG_RDM.f90:172-404)
-  Block renormalize(const Block& E, const std::vector<int>& sbsec, const Spectrum& sp, int mstates,
+  Block renormalize(const Block& E, const std::vector<int>& sbsec, Spectrum& sp, int mstates,
                     double estates, int& ms_out, double& trunc, double& cut,
                     std::vector<std::pair<QN, std::vector<double>>>& kept) {
     Block N;
@@ -694,6 +694,19 @@ struct Engine {
     for (size_t k = 0; k < N.sec.size(); ++k) {
       uoff[k] = utot;
       utot += (int64_t)E.sec[src_sector[k]].dim * N.sec[k].dim;
+    }
+    // rho blocks of the large-block path: eigenvectors of the kept states only
+    {
+      std::vector<RhoVec*> dv;
+      std::vector<int> cnt;
+      for (size_t k = 0; k < N.sec.size(); ++k) {
+        RhoVec& V = sp.dv[blk_of_new[k]];
+        if (V.def && !V.base) {
+          dv.push_back(&V);
+          cnt.push_back(N.sec[k].dim);
+        }
+      }
+      if (!dv.empty()) rho_materialize(ctx, dv, cnt);
     }
     // gathered on the device from the eigenvector blocks of the rho solve
     DevArr<double> du;
@@ -1383,6 +1396,7 @@ struct Engine {
     J.evals = o.evs;
     J.evecs = o.unused;
     J.dvecs = o.dvs;
+    J.defer = true;
     return J;
   }
 
@@ -1391,7 +1405,7 @@ struct Engine {
     RdmOut o;
     {
       PhaseTimer pt(ctx, "step.rdm");
-      sb_rdm_eigh2(ctx, sbd(), psi.p, rdm_sides(S), o.evs, o.unused, o.dvs);
+      sb_rdm_eigh2(ctx, sbd(), psi.p, rdm_sides(S), o.evs, o.unused, o.dvs, true);
     }
     finish_step(S, o);
   }

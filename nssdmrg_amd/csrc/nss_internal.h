@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -237,6 +238,48 @@ void launch_rho_eig_tri(const double* a, const int64_t* off, const int32_t* dim,
 void launch_jacobi_eig(const double* a, const int64_t* off, const int32_t* dim, const int64_t* eoff, double* vout,
                        double* wout, int nblk, int nmax, hipStream_t s);
 
+// Large-block tridiagonal path (nss_rho_big.hip) for symmetric blocks of
+// dimension up to RHO_BIG_MAX, batched over every block of a call.
+constexpr int RHO_BIG_MAX = 2048;
+struct TredBlk {
+  double* a;    // n x n row-major symmetric block, reduced in place: afterwards
+                // row k holds reflector k at columns k+1..n-1 (v[k+1] = 1)
+  double* ws;   // tred_ws_size(n) doubles: v[2][n], p[2][n], d[n], e[n], tau[n]
+  int32_t n, pad;
+};
+int64_t tred_ws_size(int n);
+// Column j of the Householder tridiagonalisation of every block (launch j = 0
+// .. nmax-1 in order; nmax = the largest n of the batch).
+void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t s);
+// Every eigenvalue of every T (d, e in ws): position p to w[eoff[b] + p],
+// ascending within each unreduced block of T.
+void launch_tri_eigvals(const TredBlk* blks, const int64_t* eoff, double* w, int nblk, int nmax, hipStream_t s);
+struct InvReq {        // one inverse-iteration vector
+  const double* d;     // T's diagonal / off-diagonal
+  const double* e;
+  double* z;           // slab: row i, slot s at z[i * ldz + s] (zeroed by the caller)
+  int64_t ldz;
+  int32_t rs, re;      // the unreduced block holding the position
+  int32_t slot, seedpos;
+  double lam, tiny;    // shift, replacement for a zero pivot
+};
+void launch_tri_invit(const InvReq* req, int nreq, double* fac, int nmax, hipStream_t s);  // fac: 5 nmax nreq
+struct GsCluster {     // slots s0..s1-1 of a slab, nonzero in rows rs..re-1
+  double* z;
+  int64_t ldz;
+  int32_t rs, re, s0, s1;
+};
+void launch_tri_cluster_gs(const GsCluster* cl, int ncl, hipStream_t s);
+struct BtTask {        // back-transformation of slots s0..s0+ns-1 of a slab
+  const double* a;     // the reduced block (reflectors in its rows)
+  const double* tau;
+  double* z;
+  int64_t ldz;
+  int32_t n, s0, ns, pad;
+};
+int backtr_vectors_per_task(int n);
+void launch_tri_backtr(const BtTask* tasks, int ntask, int nmax, hipStream_t s);
+
 // Padding of blocks into ncls x ncls slabs for a strided-batched dense
 // eigensolve, and the inverse copy of the eigenpairs (see nss_kernels.hip).
 void launch_pad_blocks(const double* src, const int64_t* off, const int32_t* dim, int nb, int ncls, double* dst,
@@ -283,7 +326,11 @@ struct nss_ctx {
   // auxiliary streams (default priority), each with its own rocblas handle,
   // created lazily: independent batches of the rho eigensolve run on them
   // concurrently with the context stream
-  static constexpr int NAUX = 4;  // [0, NAUX-1): rocSOLVER rho classes; NAUX-1: the tridiagonal-path rho kernel
+  // [0, AUX_TRI): rocSOLVER rho classes; AUX_TRI: the one-workgroup
+  // tridiagonal-path rho kernel; AUX_BIG: the large-block rho path
+  static constexpr int NAUX = 5;
+  static constexpr int AUX_TRI = 3;
+  static constexpr int AUX_BIG = 4;
   hipStream_t aux[NAUX] = {};
   void* aux_rocblas[NAUX] = {};
   nss::SbDevice* sb = nullptr;
@@ -442,12 +489,22 @@ void sb_rdm_eigh(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int side,
 // both sides at once (sides bit 0: left, bit 1: right)
 // Device view of one rho block's eigenvectors: component i of eigenvector
 // j at base[i*si + j*sj]; order[k] = index j of the k-th smallest eigenvalue.
+struct RhoDeferred;
 struct RhoVec {
   const double* base = nullptr;
   int d = 0;
   int64_t si = 0, sj = 0;
   std::vector<int> order;
+  // a block of the large-block path whose eigenvectors are not computed yet
+  // (base == nullptr): rho_materialize computes those of the largest
+  // eigenvalues on request
+  std::shared_ptr<RhoDeferred> def;
+  int def_blk = -1;
 };
+// Computes the eigenvectors of the count[i] largest eigenvalues of each
+// deferred view vecs[i] (count 0: left deferred) and points the views at
+// them (valid while the views live).  Enqueued on the context stream.
+void rho_materialize(nss_ctx* ctx, const std::vector<RhoVec*>& vecs, const std::vector<int>& count);
 // rho_L / rho_R blocks of psi (sides: bit 0 left, bit 1 right) and their
 // eigenpairs, eigenvalues ascending on the host.  With dvecs == nullptr the
 // eigenvectors come to the host (evecs: row-major, eigenvectors as columns);
@@ -455,7 +512,7 @@ struct RhoVec {
 // call) and dvecs[side][q] describes them.
 void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
                   std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
-                  std::vector<RhoVec>* dvecs = nullptr);
+                  std::vector<RhoVec>* dvecs = nullptr, bool defer = false);
 // Several superblocks at once (independent chains): job j's rho blocks of
 // psi_j, its eigenvalues to evals[2], eigenvectors to evecs[2] (host) or
 // dvecs[2] (views into rho, valid until rho is reused).  One GEMM launch,
@@ -467,6 +524,9 @@ struct RdmJob {
   std::vector<std::vector<double>>* evals = nullptr;  // [2]
   std::vector<std::vector<double>>* evecs = nullptr;  // [2], or nullptr with dvecs
   std::vector<RhoVec>* dvecs = nullptr;               // [2], or nullptr
+  // with dvecs: blocks of the large-block path may stay deferred (their
+  // eigenvectors computed later, for the kept states only)
+  bool defer = false;
 };
 void sb_rdm_eigh_multi(::nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& rho, GemmPlan& R);
 struct GatherTask {  // dst[r*ncol + c] = src[r*si + cols[c]*sj]

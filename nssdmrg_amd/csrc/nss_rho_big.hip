@@ -1,0 +1,601 @@
+// Large rho blocks: the symmetric eigensolve of blocks_matrix%eigh
+// (src/MATRIX/MATRIX_BLOCKS.f90:589-657, LAPACK dsyev through SciFortran's
+// eigh) for blocks above the one-workgroup tridiagonal-path kernel, batched
+// over every block of a step and spread over many workgroups per block.
+//
+//  1. Householder tridiagonalisation T = Q^T A Q (dsytd2's lower form,
+//     Q = H_0 H_1 ... H_{n-3}, H_k = I - tau_k v_k v_k^T, v_k[k+1] = 1), one
+//     launch per column j for all blocks.  Each workgroup owns TRED_ROWS rows
+//     of the trailing matrix.  It recomputes, redundantly, the previous
+//     step's w = p - (tau/2)(p.v) v and the updated row j (the column j by
+//     symmetry), forms v_j, then in one pass over its rows applies the
+//     previous rank-2 update A -= v w^T + w v^T and accumulates its share of
+//     p_j = tau_j A v_j.  The kernel boundary is the only global
+//     synchronisation.  Reflector k is kept in row k of the block
+//     (columns k+1..n-1), written by the launch after the one that formed it
+//     (nothing reads row k then).
+//  2. Every eigenvalue of T by multisection on Sturm counts within its
+//     unreduced block (T split where |e_k| <= eps ||T||, as dstebz splits),
+//     EIG_BIS lanes per eigenvalue.
+//  3. On request, for chosen positions only (the kept states of a
+//     truncation): inverse iteration with T - lambda I (dstein: two solves
+//     with the dgttrf factors from a pseudo-random start), one lane per
+//     vector; modified Gram-Schmidt within clusters of eigenvalues closer
+//     than 1e-3 ||T|| (dstein's criterion), one workgroup per cluster; the
+//     back-transformation x = Q z, one wave per group of vectors held in
+//     registers while the reflectors stream past.
+#include <cfloat>
+
+#include "nss_internal.h"
+
+namespace nss {
+
+namespace {
+
+constexpr int TRED_THREADS = 256;
+constexpr int TRED_WAVES = TRED_THREADS / 64;
+constexpr int TRED_ROWS = 16;  // trailing-matrix rows per workgroup
+constexpr int EIG_THREADS = 256;
+constexpr int EIG_BIS = 4;     // lanes (interior points) per eigenvalue
+constexpr int EIG_PER_WG = EIG_THREADS / EIG_BIS;
+constexpr int GS_THREADS = 1024;
+constexpr int GS_WAVES = GS_THREADS / 64;
+constexpr int GS_TILE = 512;   // dot products per LDS round in the cluster Gram-Schmidt
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double rdlane(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)u, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over the wave (every lane active), returned in every lane
+__device__ __forceinline__ double wsum(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  v += dpp_d<0x140>(v);  // row_mirror
+  return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
+}
+
+__device__ __forceinline__ double wmax(double v) {
+  v = fmax(v, dpp_d<0xB1>(v));
+  v = fmax(v, dpp_d<0x4E>(v));
+  v = fmax(v, dpp_d<0x141>(v));
+  v = fmax(v, dpp_d<0x140>(v));
+  return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
+}
+
+// workgroup sum / max with NW waves, in every thread (red: NW doubles of LDS)
+template <int NW>
+__device__ __forceinline__ double bsum(double v, double* red) {
+  v = wsum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+template <int NW>
+__device__ __forceinline__ double bmax(double v, double* red) {
+  v = wmax(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) r = fmax(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+// 1/q to full precision: the hardware reciprocal and two Newton steps
+__device__ __forceinline__ double rcp_d(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return fma(fma(-q, r, 1.0), r, r);
+}
+
+}  // namespace
+
+int64_t tred_ws_size(int n) { return (int64_t)7 * n; }
+
+// ------------------------------------------------------------ 1. tridiagonal
+__global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __restrict__ blks, int nblk, int j,
+                                                           int nmax) {
+  extern __shared__ double sm[];  // vp, wp over [j, n); vj over [j, n)
+  __shared__ double red[TRED_WAVES];
+  const int b = blockIdx.x % nblk, t = blockIdx.x / nblk;
+  const TredBlk B = blks[b];
+  const int n = B.n;
+  if (j >= n) return;
+  const int r0 = j + 1 + t * TRED_ROWS;
+  if (t > 0 && r0 >= n) return;
+  const int tid = threadIdx.x;
+  const int m = n - j;  // the trailing range [j, n)
+  double* vp = sm;
+  double* wp = sm + nmax;
+  double* vj = sm + 2 * nmax;
+  double* A = B.a;
+  double* ws = B.ws;
+  double* vbuf = ws;          // [2][n]
+  double* pbuf = ws + 2 * n;  // [2][n]
+  double* dd = ws + 4 * n;
+  double* ee = ws + 5 * n;
+  double* tau = ws + 6 * n;
+  // previous step: v_{j-1}, p_{j-1} over [j, n); w = p - (tau/2)(p.v) v
+  const double taup = j > 0 ? tau[j - 1] : 0.0;
+  double pv = 0.0;
+  if (j > 0) {
+    const double* vq = vbuf + ((j - 1) & 1) * n;
+    const double* pq = pbuf + ((j - 1) & 1) * n;
+    for (int i = tid; i < m; i += TRED_THREADS) {
+      const double v = vq[j + i], p = pq[j + i];
+      vp[i] = v;
+      wp[i] = p;
+      pv += p * v;
+    }
+  } else {
+    for (int i = tid; i < m; i += TRED_THREADS) vp[i] = wp[i] = 0.0;
+  }
+  pv = bsum<TRED_WAVES>(pv, red);  // (its barriers also publish vp, wp)
+  const double K = 0.5 * taup * pv;
+  for (int i = tid; i < m; i += TRED_THREADS) wp[i] -= K * vp[i];
+  __syncthreads();
+  // row j after the previous update (the column j by symmetry), into vj
+  const double v0 = vp[0], w0 = wp[0];
+  double s2 = 0.0;
+  const double* rowj = A + (int64_t)j * n + j;
+  for (int i = tid; i < m; i += TRED_THREADS) {
+    const double r = rowj[i] - v0 * wp[i] - w0 * vp[i];
+    vj[i] = r;
+    if (i >= 2) s2 += r * r;
+  }
+  s2 = bsum<TRED_WAVES>(s2, red);
+  const double dj = vj[0];
+  const double alpha = m > 1 ? vj[1] : 0.0;
+  double tj = 0.0, beta = alpha, scale = 0.0;
+  if (s2 > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+    tj = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  __syncthreads();  // every thread has read vj[0], vj[1]
+  for (int i = tid + 1; i < m; i += TRED_THREADS) vj[i] = i == 1 ? 1.0 : vj[i] * scale;
+  __syncthreads();
+  if (t == 0) {
+    if (tid == 0) {
+      dd[j] = dj;
+      ee[j] = m > 1 ? beta : 0.0;
+      tau[j] = tj;
+    }
+    double* vo = vbuf + (j & 1) * n;
+    for (int i = tid + 1; i < m; i += TRED_THREADS) vo[j + i] = vj[i];
+    if (j > 0) {  // reflector j-1 to its row (columns j..n-1)
+      double* keep = A + (int64_t)(j - 1) * n;
+      for (int i = tid; i < m; i += TRED_THREADS) keep[j + i] = vp[i];
+    }
+  }
+  // owned rows: the previous update, then this row's share of p_j = tau_j A v_j
+  const int wv = tid >> 6, lane = tid & 63;
+  double* po = pbuf + (j & 1) * n;
+  const int r1 = min(r0 + TRED_ROWS, n);
+  for (int i = r0 + wv; i < r1; i += TRED_WAVES) {
+    const int li = i - j;
+    const double vi = vp[li], wi = wp[li];
+    double* row = A + (int64_t)i * n;
+    double acc = 0.0;
+    if (j > 0) {
+      for (int c = j + 1 + lane; c < n; c += 64) {
+        const int lc = c - j;
+        const double a = row[c] - vi * wp[lc] - wi * vp[lc];
+        row[c] = a;
+        acc += a * vj[lc];
+      }
+    } else {
+      for (int c = 1 + lane; c < n; c += 64) acc += row[c] * vj[c];
+    }
+    acc = wsum(acc);
+    if (lane == 0) po[i] = tj * acc;
+  }
+}
+
+void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t s) {
+  const int tiles = std::max(1, (nmax - j - 1 + TRED_ROWS - 1) / TRED_ROWS);
+  hipLaunchKernelGGL(k_tred_col, dim3(nblk * tiles), dim3(TRED_THREADS), (size_t)3 * nmax * sizeof(double), s,
+                     blks, nblk, j, nmax);
+  NSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ 2. eigenvalues
+// Position p of block b belongs to the unreduced block [s, e) of T that holds
+// row p and takes eigenvalue p - s of that block (ascending within it).
+__global__ __launch_bounds__(EIG_THREADS) void k_tri_eigvals(const TredBlk* __restrict__ blks,
+                                                             const int64_t* __restrict__ eoff,
+                                                             double* __restrict__ wout, int nblk, int nmax) {
+  extern __shared__ double sm[];  // dg, eg, e2 (nmax each)
+  __shared__ double red[EIG_THREADS / 64];
+  __shared__ int sbeg[EIG_PER_WG], send[EIG_PER_WG];
+  const int b = blockIdx.x % nblk, g = blockIdx.x / nblk;
+  const TredBlk B = blks[b];
+  const int n = B.n;
+  const int p0 = g * EIG_PER_WG;
+  if (p0 >= n) return;
+  const int tid = threadIdx.x;
+  double* dg = sm;
+  double* eg = sm + nmax;
+  double* e2 = sm + 2 * nmax;
+  const double* dsrc = B.ws + 4 * n;
+  const double* esrc = B.ws + 5 * n;
+  double tn = 0.0, lo = HUGE_VAL, hi = -HUGE_VAL, em = 0.0;
+  for (int i = tid; i < n; i += EIG_THREADS) {
+    const double d = dsrc[i];
+    const double el = i > 0 ? fabs(esrc[i - 1]) : 0.0;
+    const double er = i + 1 < n ? fabs(esrc[i]) : 0.0;
+    dg[i] = d;
+    eg[i] = i + 1 < n ? esrc[i] : 0.0;
+    tn = fmax(tn, fabs(d) + el + er);
+    lo = fmin(lo, d - el - er);
+    hi = fmax(hi, d + el + er);
+    if (i + 1 < n) em = fmax(em, er * er);
+  }
+  tn = fmax(bmax<EIG_THREADS / 64>(tn, red), DBL_MIN);
+  lo = -bmax<EIG_THREADS / 64>(-lo, red);
+  hi = bmax<EIG_THREADS / 64>(hi, red);
+  em = bmax<EIG_THREADS / 64>(em, red);
+  const double pivmin = DBL_MIN * fmax(1.0, em);
+  for (int i = tid; i + 1 < n; i += EIG_THREADS)
+    if (fabs(eg[i]) <= DBL_EPSILON * tn) eg[i] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < n; i += EIG_THREADS) e2[i] = i + 1 < n ? eg[i] * eg[i] : 0.0;
+  // the unreduced blocks of this workgroup's positions (wave 0: ballots over
+  // 64 consecutive positions, then a scan in chunks of 64 beyond them)
+  if (tid < 64) {
+    const int p = p0 + tid;
+    const bool starts = p < n && (p == 0 || eg[p - 1] == 0.0);
+    const bool ends = p < n && (p == n - 1 || eg[p] == 0.0);
+    const uint64_t bs = __ballot(starts), be = __ballot(ends);
+    // start before p0: nearest k <= p0 with k == 0 or eg[k-1] == 0
+    int sprev = p0;
+    if (!(bs & 1ull)) {
+      for (int base = p0 - 1; base >= 0; base -= 64) {  // k = base - lane
+        const int k = base - tid;
+        const bool z = k >= 0 && (k == 0 || eg[k - 1] == 0.0);
+        const uint64_t bz = __ballot(z);
+        if (bz) {
+          sprev = base - (int)__builtin_ctzll(bz);
+          break;
+        }
+      }
+    }
+    // end after the last position: nearest k >= p0+63 with eg[k] == 0 or k == n-1
+    const int plast = min(p0 + 63, n - 1);
+    int enext = plast + 1;
+    if (!((be >> (plast - p0)) & 1ull)) {
+      for (int base = plast + 1; base < n; base += 64) {
+        const int k = base + tid;
+        const bool z = k < n && (k == n - 1 || eg[k] == 0.0);
+        const uint64_t bz = __ballot(z);
+        if (bz) {
+          enext = base + (int)__builtin_ctzll(bz) + 1;
+          break;
+        }
+      }
+    }
+    if (p < n) {
+      const uint64_t below = bs & ((tid == 63) ? ~0ull : ((2ull << tid) - 1ull));
+      sbeg[tid] = below ? p0 + 63 - (int)__builtin_clzll(below) : sprev;
+      const uint64_t above = be & ~((1ull << tid) - 1ull);
+      send[tid] = above ? p0 + (int)__builtin_ctzll(above) + 1 : enext;
+    }
+  }
+  __syncthreads();
+  const int lp = tid / EIG_BIS, lb = tid % EIG_BIS;
+  const int p = p0 + lp;
+  if (p >= n) return;  // (no barrier follows)
+  const int bs = sbeg[lp], be = send[lp];
+  const int tix = p - bs;
+  double l = lo, h = hi;
+  if (be - bs == 1) l = h = dg[p];
+  const double tol = 2.0 * DBL_EPSILON * tn;
+  constexpr double inv_parts = 1.0 / (EIG_BIS + 1);
+  for (int it = 0; it < 96; ++it) {
+    if (!(h - l > tol)) break;  // alike on the lanes of a group
+    const double x = l + (h - l) * (double)(lb + 1) * inv_parts;
+    double q = dg[bs] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < 0.0;
+#pragma unroll 8
+    for (int i = bs + 1; i < be; ++i) {
+      q = dg[i] - x - e2[i - 1] * rcp_d(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+    const bool above = cnt >= tix + 1;  // eigenvalue tix lies below x
+    const uint64_t bal = __ballot(above);
+    const unsigned bits = (unsigned)(bal >> (threadIdx.x & (64 - EIG_BIS))) & ((1u << EIG_BIS) - 1u);
+    const int f = bits ? __builtin_ctz(bits) : EIG_BIS;  // first point above it
+    const double span = h - l;
+    const double nl = f > 0 ? l + span * (double)f * inv_parts : l;
+    const double nh = f < EIG_BIS ? l + span * (double)(f + 1) * inv_parts : h;
+    l = nl;
+    h = fmax(nh, nl);
+  }
+  if (lb == 0) wout[eoff[b] + p] = 0.5 * (l + h);
+}
+
+void launch_tri_eigvals(const TredBlk* blks, const int64_t* eoff, double* w, int nblk, int nmax, hipStream_t s) {
+  const int groups = (nmax + EIG_PER_WG - 1) / EIG_PER_WG;
+  hipLaunchKernelGGL(k_tri_eigvals, dim3(nblk * groups), dim3(EIG_THREADS), (size_t)3 * nmax * sizeof(double), s,
+                     blks, eoff, w, nblk, nmax);
+  NSS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------ 3. eigenvectors
+// One lane per requested vector: LU with partial pivoting of T[rs..re) -
+// lam I (its factors at fac[(f * nmax + i) * nreq + r]), two solves from a
+// SplitMix64 start, the normalised iterate to rows rs..re-1 of column slot
+// of z (rows outside stay as the caller zeroed them).
+__global__ __launch_bounds__(256) void k_tri_invit(const InvReq* __restrict__ rq, int nreq,
+                                                   double* __restrict__ fac, int nmax) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nreq) return;
+  const InvReq R = rq[r];
+  const int nb = R.re - R.rs;
+  double* zb = R.z + (int64_t)R.rs * R.ldz + R.slot;  // local row i at zb[i * ldz]
+  const int64_t ldz = R.ldz;
+  if (nb == 1) {
+    zb[0] = 1.0;
+    return;
+  }
+  const int64_t fs = (int64_t)nmax * nreq;
+  double* DL = fac + r;
+  double* DD = fac + fs + r;
+  double* DU = fac + 2 * fs + r;
+  double* DU2 = fac + 3 * fs + r;
+  double* PV = fac + 4 * fs + r;
+  const double* dgb = R.d + R.rs;
+  const double* egb = R.e + R.rs;
+  const double sh = R.lam, tiny = R.tiny;
+  double ddi = dgb[0] - sh, dui = egb[0];
+  for (int i = 0; i + 1 < nb; ++i) {
+    const double dli = egb[i];
+    double ddn = dgb[i + 1] - sh;
+    double dun = (i + 2 < nb) ? egb[i + 1] : 0.0;
+    double f, du2i = 0.0, pvi = 0.0;
+    if (fabs(ddi) >= fabs(dli)) {
+      if (ddi == 0.0) ddi = tiny;
+      f = dli / ddi;
+      ddn -= f * dui;
+    } else {
+      f = ddi / dli;
+      const double tmp = dui;
+      ddi = dli;
+      dui = ddn;
+      ddn = tmp - f * ddn;
+      if (i + 2 < nb) {
+        du2i = dun;
+        dun = -f * dun;
+      }
+      pvi = 1.0;
+    }
+    DL[(int64_t)i * nreq] = f;
+    DD[(int64_t)i * nreq] = ddi;
+    DU[(int64_t)i * nreq] = dui;
+    DU2[(int64_t)i * nreq] = du2i;
+    PV[(int64_t)i * nreq] = pvi;
+    ddi = ddn;
+    dui = dun;
+  }
+  if (ddi == 0.0) ddi = tiny;
+  DD[(int64_t)(nb - 1) * nreq] = ddi;
+  for (int i = 0; i < nb; ++i) {  // start: uniform [-1, 1) from SplitMix64 of (position, row)
+    uint64_t z = ((uint64_t)R.seedpos << 24) + (uint64_t)(i + 1);
+    z *= 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    zb[i * ldz] = (double)(z >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    double xi = zb[0];
+#pragma unroll 8
+    for (int i = 0; i + 1 < nb; ++i) {  // L, with the row interchanges
+      double xn = zb[(i + 1) * ldz];
+      if (PV[(int64_t)i * nreq] != 0.0) {
+        const double t = xi;
+        xi = xn;
+        xn = t;
+      }
+      xn -= DL[(int64_t)i * nreq] * xi;
+      zb[i * ldz] = xi;
+      xi = xn;
+    }
+    double x1 = xi / DD[(int64_t)(nb - 1) * nreq];  // U (two superdiagonals)
+    zb[(nb - 1) * ldz] = x1;
+    double x0 = (zb[(nb - 2) * ldz] - DU[(int64_t)(nb - 2) * nreq] * x1) / DD[(int64_t)(nb - 2) * nreq];
+    zb[(nb - 2) * ldz] = x0;
+    double x2 = x1;
+    x1 = x0;
+#pragma unroll 8
+    for (int i = nb - 3; i >= 0; --i) {
+      x0 = (zb[i * ldz] - DU[(int64_t)i * nreq] * x1 - DU2[(int64_t)i * nreq] * x2) / DD[(int64_t)i * nreq];
+      zb[i * ldz] = x0;
+      x2 = x1;
+      x1 = x0;
+    }
+    double mx = 0.0;
+    for (int i = 0; i < nb; ++i) mx = fmax(mx, fabs(zb[i * ldz]));
+    double ss = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const double y = zb[i * ldz] / mx;
+      ss += y * y;
+    }
+    const double inv = 1.0 / (mx * sqrt(ss));
+    for (int i = 0; i < nb; ++i) zb[i * ldz] *= inv;
+  }
+}
+
+void launch_tri_invit(const InvReq* req, int nreq, double* fac, int nmax, hipStream_t s) {
+  if (nreq <= 0) return;
+  hipLaunchKernelGGL(k_tri_invit, dim3((nreq + 255) / 256), dim3(256), 0, s, req, nreq, fac, nmax);
+  NSS_HIP(hipGetLastError());
+}
+
+// Right-looking modified Gram-Schmidt over the vectors s0..s1-1 of a cluster
+// (nonzero in rows rs..re-1 only), up to two sweeps.  Step j: h_k = z_j . z_k
+// for k >= j (waves over rows, lanes over k: the slots of a row are
+// contiguous), then z_k -= (h_k / h_j) z_j; z_j is scaled by 1/sqrt(h_j) at
+// the start of step j+1, which reads neither z_j nor its scale.
+__global__ __launch_bounds__(GS_THREADS) void k_tri_cluster_gs(const GsCluster* __restrict__ cls) {
+  __shared__ double part[GS_WAVES][64];
+  __shared__ double hb[GS_TILE];
+  __shared__ double red[GS_WAVES];
+  const GsCluster C = cls[blockIdx.x];
+  const int c = C.s1 - C.s0;
+  if (c < 2) return;
+  const int nr = C.re - C.rs;
+  const int64_t ldz = C.ldz;
+  double* Z = C.z + (int64_t)C.rs * ldz + C.s0;  // row i, vector k at Z[i * ldz + k]
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    double inv = 1.0;
+    double ovl = 0.0;
+    for (int j = 0; j < c; ++j) {
+      if (j > 0)
+        for (int i = tid; i < nr; i += GS_THREADS) Z[i * ldz + (j - 1)] *= inv;
+      // the coefficients of k = j + 1 .. c-1 in tiles of GS_TILE, h_j first
+      double hj = 0.0;
+      for (int k0 = j; k0 < c; k0 += GS_TILE) {
+        const int kt = min(GS_TILE, c - k0);
+        for (int q = 0; q < kt; q += 64) {
+          const int k = k0 + q + lane;
+          double acc = 0.0;
+          if (k < c)
+            for (int i = wv; i < nr; i += GS_WAVES) acc += Z[i * ldz + j] * Z[i * ldz + k];
+          part[wv][lane] = acc;
+          __syncthreads();
+          if (wv == 0) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < GS_WAVES; ++w) s += part[w][lane];
+            if (q + lane < kt) hb[q + lane] = s;
+          }
+          __syncthreads();
+        }
+        if (k0 == j) hj = hb[0];
+        const int first = k0 == j ? 1 : 0;
+        // z_k -= (h_k / h_j) z_j for the tile's k
+        for (int q = first; q < kt; q += 64) {
+          const int kk = q + lane;
+          const double cf = kk < kt ? hb[kk] / hj : 0.0;
+          ovl = fmax(ovl, fabs(cf));
+          if (kk < kt)
+            for (int i = wv; i < nr; i += GS_WAVES) Z[i * ldz + k0 + kk] -= cf * Z[i * ldz + j];
+        }
+        __syncthreads();
+      }
+      inv = 1.0 / sqrt(hj);
+    }
+    for (int i = tid; i < nr; i += GS_THREADS) Z[i * ldz + (c - 1)] *= inv;
+    __syncthreads();
+    // a second sweep only where the first removed overlaps above 1e-6
+    if (!(bmax<GS_WAVES>(ovl, red) > 1e-6)) break;
+  }
+}
+
+void launch_tri_cluster_gs(const GsCluster* cl, int ncl, hipStream_t s) {
+  if (ncl <= 0) return;
+  hipLaunchKernelGGL(k_tri_cluster_gs, dim3(ncl), dim3(GS_THREADS), 0, s, cl);
+  NSS_HIP(hipGetLastError());
+}
+
+// x = H_0 (H_1 (... (H_{n-3} z))) for V vectors per wave, row lane + 64 t of
+// vector v in register z[t][v] (NPL rows per lane: n <= 64 NPL).  The next
+// reflector's entries are loaded before the current one is applied.
+template <int NPL, int V>
+__global__ __launch_bounds__(256) void k_tri_backtr(const BtTask* __restrict__ tasks, int ntask) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= ntask) return;  // wave-uniform
+  const BtTask T = tasks[w];
+  const int lane = threadIdx.x & 63;
+  const int n = T.n;
+  double z[NPL][V];
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    const int i = lane + 64 * t;
+#pragma unroll
+    for (int v = 0; v < V; ++v) z[t][v] = (i < n && v < T.ns) ? T.z[(int64_t)i * T.ldz + T.s0 + v] : 0.0;
+  }
+  double vr[NPL];
+  int k = n - 3;
+  if (k >= 0) {
+    const double* vk = T.a + (int64_t)k * n;
+#pragma unroll
+    for (int t = 0; t < NPL; ++t) {
+      const int i = lane + 64 * t;
+      vr[t] = (i > k && i < n) ? vk[i] : 0.0;
+    }
+  }
+  for (; k >= 0; --k) {
+    const double tk = T.tau[k];
+    double vn[NPL];  // reflector k-1, loaded ahead
+    if (k > 0) {
+      const double* vk = T.a + (int64_t)(k - 1) * n;
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) {
+        const int i = lane + 64 * t;
+        vn[t] = (i > k - 1 && i < n) ? vk[i] : 0.0;
+      }
+    }
+    if (tk != 0.0) {
+      double d[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] = 0.0;
+#pragma unroll
+      for (int t = 0; t < NPL; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) d[v] = fma(vr[t], z[t][v], d[v]);
+#pragma unroll
+      for (int v = 0; v < V; ++v) d[v] = tk * wsum(d[v]);
+#pragma unroll
+      for (int t = 0; t < NPL; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) z[t][v] = fma(-vr[t], d[v], z[t][v]);
+    }
+    if (k > 0) {
+#pragma unroll
+      for (int t = 0; t < NPL; ++t) vr[t] = vn[t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NPL; ++t) {
+    const int i = lane + 64 * t;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (i < n && v < T.ns) T.z[(int64_t)i * T.ldz + T.s0 + v] = z[t][v];
+  }
+}
+
+int backtr_vectors_per_task(int n) { return n <= 1024 ? 4 : 2; }
+
+void launch_tri_backtr(const BtTask* tasks, int ntask, int nmax, hipStream_t s) {
+  if (ntask <= 0) return;
+  NSS_REQUIRE(nmax <= RHO_BIG_MAX, NSS_E_ARG, "back-transformation: block above the large-block limit");
+  const dim3 grid((ntask + 3) / 4);
+  if (nmax <= 1024) hipLaunchKernelGGL((k_tri_backtr<16, 4>), grid, dim3(256), 0, s, tasks, ntask);
+  else hipLaunchKernelGGL((k_tri_backtr<32, 2>), grid, dim3(256), 0, s, tasks, ntask);
+  NSS_HIP(hipGetLastError());
+}
+
+}  // namespace nss

@@ -482,6 +482,16 @@ static int jacobi_limit() {
   return v;
 }
 
+// Largest rho block of the large-block tridiagonal path (NSS_RHO_BIG_MAX,
+// default and cap RHO_BIG_MAX, 0 for none); larger blocks go to rocSOLVER.
+static int rho_big_limit() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_RHO_BIG_MAX");
+    return e ? std::clamp(std::atoi(e), 0, RHO_BIG_MAX) : RHO_BIG_MAX;
+  }();
+  return v;
+}
+
 static int rho_tri_limit() {
   static const int v = [] {
     const char* e = std::getenv("NSS_RHO_TRI_MAX");
@@ -531,6 +541,136 @@ static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
   return static_cast<rocblas_handle>(ctx->aux_rocblas[k]);
 }
 
+// ---------------------------------------------------------------- large rho blocks
+// Blocks of the large-block path after tridiagonalisation: their eigenvalues
+// are known, their eigenvectors are computed on request (rho_materialize).
+// The reduced blocks (reflectors in their rows) stay in the rho buffer of
+// the call that made them.
+struct RhoDeferred {
+  nss_ctx* ctx = nullptr;
+  std::vector<TredBlk> blk;                // device addresses of the blocks and workspaces
+  std::vector<int64_t> hwo;                // offset of block k's workspace in ws / hws
+  std::vector<double> hws;                 // host copy of the workspaces (d at 4n, e at 5n)
+  std::vector<std::vector<double>> lam;    // eigenvalue at each position of block k
+  DevArr<double> ws;
+  // materialised eigenvectors and the tables of their launches (kept until
+  // the views go away)
+  struct Batch {
+    std::vector<DevArr<double>> slabs;
+    DevArr<double> fac;
+    DevArr<InvReq> dreq;
+    DevArr<GsCluster> dcl;
+    DevArr<BtTask> dbt;
+    std::vector<InvReq> hreq;
+    std::vector<GsCluster> hcl;
+    std::vector<BtTask> hbt;
+  };
+  std::vector<std::unique_ptr<Batch>> done;
+};
+
+void rho_materialize(nss_ctx* ctx, const std::vector<RhoVec*>& vecs, const std::vector<int>& count) {
+  auto B = std::make_unique<RhoDeferred::Batch>();
+  RhoDeferred* owner = nullptr;
+  int nmax = 0;
+  for (size_t i = 0; i < vecs.size(); ++i) {
+    RhoVec& V = *vecs[i];
+    const int k = count[i];
+    if (!V.def || V.base || k <= 0) continue;
+    NSS_REQUIRE(k <= V.d && V.def_blk >= 0, NSS_E_STATE, "rho_materialize: bad request");
+    RhoDeferred& D = *V.def;
+    owner = owner ? owner : &D;
+    const int b = V.def_blk, n = V.d;
+    const double* d = D.hws.data() + D.hwo[b] + 4 * (int64_t)n;
+    const double* e = D.hws.data() + D.hwo[b] + 5 * (int64_t)n;
+    const std::vector<double>& lam = D.lam[b];
+    // ||T|| and the unreduced blocks, as the eigenvalue kernel splits T
+    double tn = 0.0;
+    for (int r = 0; r < n; ++r)
+      tn = std::max(tn, std::fabs(d[r]) + (r > 0 ? std::fabs(e[r - 1]) : 0.0) + (r + 1 < n ? std::fabs(e[r]) : 0.0));
+    tn = std::max(tn, DBL_MIN);
+    std::vector<int> bs(n), be(n);
+    for (int r = 0; r < n; ++r) bs[r] = (r > 0 && !(std::fabs(e[r - 1]) <= DBL_EPSILON * tn)) ? bs[r - 1] : r;
+    for (int r = n - 1; r >= 0; --r) be[r] = (r + 1 < n && !(std::fabs(e[r]) <= DBL_EPSILON * tn)) ? be[r + 1] : r + 1;
+    // the k largest eigenvalues' positions, ascending: slot s holds pos[s]
+    std::vector<int> pos(V.order.end() - k, V.order.end());
+    std::sort(pos.begin(), pos.end());
+    DevArr<double> slab;
+    slab.resize(ctx->pool, (size_t)n * k);
+    NSS_HIP(hipMemsetAsync(slab.p, 0, (size_t)n * k * sizeof(double), ctx->stream));
+    const double tiny = std::max(DBL_EPSILON * tn, 1e-280);
+    const double ctol = 1e-3 * tn;
+    const TredBlk& T = D.blk[b];
+    const double* dd = T.ws + 4 * (int64_t)n;
+    const double* de = T.ws + 5 * (int64_t)n;
+    int c0 = 0;
+    for (int sl = 0; sl < k; ++sl) {
+      const int p = pos[sl];
+      InvReq q{};
+      q.d = dd;
+      q.e = de;
+      q.z = slab.p;
+      q.ldz = k;
+      q.rs = bs[p];
+      q.re = be[p];
+      q.slot = sl;
+      q.seedpos = p;
+      q.lam = lam[p];
+      q.tiny = tiny;
+      B->hreq.push_back(q);
+      // clusters: consecutive kept positions of one unreduced block whose
+      // eigenvalues lie within 1e-3 ||T|| (dstein's criterion)
+      const bool joins = sl > 0 && bs[pos[sl - 1]] == bs[p] && lam[p] - lam[pos[sl - 1]] <= ctol;
+      if (!joins) {
+        if (sl - c0 > 1) B->hcl.push_back({slab.p, (int64_t)k, bs[pos[c0]], be[pos[c0]], c0, sl});
+        c0 = sl;
+      }
+    }
+    if (k - c0 > 1) B->hcl.push_back({slab.p, (int64_t)k, bs[pos[c0]], be[pos[c0]], c0, k});
+    const int vpt = backtr_vectors_per_task(n);
+    for (int s0 = 0; s0 < k; s0 += vpt) {
+      BtTask t{};
+      t.a = T.a;
+      t.tau = T.ws + 6 * (int64_t)n;
+      t.z = slab.p;
+      t.ldz = k;
+      t.n = n;
+      t.s0 = s0;
+      t.ns = std::min(vpt, k - s0);
+      B->hbt.push_back(t);
+    }
+    nmax = std::max(nmax, n);
+    // the view: slot of the c-th largest eigenvalue
+    V.base = slab.p;
+    V.si = k;
+    V.sj = 1;
+    for (int c = 0; c < k; ++c) {
+      const int p = V.order[n - 1 - c];
+      V.order[n - 1 - c] = (int)(std::lower_bound(pos.begin(), pos.end(), p) - pos.begin());
+    }
+    B->slabs.push_back(std::move(slab));
+  }
+  if (!owner) return;
+  hipStream_t s = ctx->stream;
+  const int nreq = (int)B->hreq.size(), ncl = (int)B->hcl.size(), nbt = (int)B->hbt.size();
+  // one backtransformation launch serves one register shape: n <= 1024 or above
+  std::vector<BtTask> lo, hi;
+  for (const BtTask& t : B->hbt) (t.n <= 1024 ? lo : hi).push_back(t);
+  B->hbt = lo;
+  B->hbt.insert(B->hbt.end(), hi.begin(), hi.end());
+  B->fac.resize(ctx->pool, (size_t)5 * nmax * nreq);
+  B->dreq.resize(ctx->pool, nreq);
+  B->dcl.resize(ctx->pool, std::max(1, ncl));
+  B->dbt.resize(ctx->pool, nbt);
+  NSS_HIP(hipMemcpyAsync(B->dreq.p, B->hreq.data(), nreq * sizeof(InvReq), hipMemcpyHostToDevice, s));
+  if (ncl) NSS_HIP(hipMemcpyAsync(B->dcl.p, B->hcl.data(), ncl * sizeof(GsCluster), hipMemcpyHostToDevice, s));
+  NSS_HIP(hipMemcpyAsync(B->dbt.p, B->hbt.data(), nbt * sizeof(BtTask), hipMemcpyHostToDevice, s));
+  launch_tri_invit(B->dreq.p, nreq, B->fac.p, nmax, s);
+  launch_tri_cluster_gs(B->dcl.p, ncl, s);
+  if (!lo.empty()) launch_tri_backtr(B->dbt.p, (int)lo.size(), 1024, s);
+  if (!hi.empty()) launch_tri_backtr(B->dbt.p + lo.size(), (int)hi.size(), RHO_BIG_MAX, s);
+  owner->done.push_back(std::move(B));
+}
+
 // rho blocks of both sides of one or several superblocks in one batch:
 // items (job, side, sector) share one rho buffer, one GEMM launch, one
 // symmetry check, one Jacobi launch and one strided-batched rocSOLVER call
@@ -539,7 +679,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   struct Item {
     int job, side, q, d;
     int64_t roff, eoff;
-    int kind;  // 1 jacobi, 2 rocsolver, 3 tridiagonal path
+    int kind;  // 1 jacobi, 2 rocsolver, 3 tridiagonal path (one workgroup), 4 large-block path
   };
   std::vector<Item> items;
   int64_t tot = 0;
@@ -617,6 +757,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
   if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
   const int tmax = std::min(rho_tri_max_dim(ctx->device), rho_tri_limit());
+  const int bmax = rho_big_limit();
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff, toff, teoff;
   std::vector<int32_t> jdim, tdim;
@@ -624,9 +765,12 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   for (Item& it : items) {
     it.eoff = etot;
     etot += it.d;
-    it.kind = it.d <= jmax ? 1 : it.d <= tmax ? 3 : 2;
+    it.kind = it.d <= jmax ? 1 : it.d <= tmax ? 3 : it.d <= bmax ? 4 : 2;
     if (ctx->prof) {
-      ctx->timers[it.kind == 1 ? "rdm.count_jacobi" : it.kind == 3 ? "rdm.count_tri" : "rdm.count_rocsolver"] += 1;
+      ctx->timers[it.kind == 1   ? "rdm.count_jacobi"
+                  : it.kind == 3 ? "rdm.count_tri"
+                  : it.kind == 4 ? "rdm.count_big"
+                                 : "rdm.count_rocsolver"] += 1;
       if (it.kind == 2) ctx->timers["rdm.sum_dim_rocsolver"] += it.d;
     }
     if (it.kind == 1) {
@@ -659,15 +803,26 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   // the last auxiliary stream, concurrently with the Jacobi launch and the
   // rocSOLVER classes (rho is complete: the symmetry check above
   // synchronised the context stream)
-  bool used_aux[nss_ctx::NAUX] = {};
+  // every auxiliary stream used here is drained before this function
+  // returns or throws: the pool does not track streams, so buffers freed on
+  // the way out must not be in use there
+  struct AuxDrain {
+    nss_ctx* c;
+    bool used[nss_ctx::NAUX] = {};
+    ~AuxDrain() {
+      for (int k = 0; k < nss_ctx::NAUX; ++k)
+        if (used[k] && c->aux[k]) (void)hipStreamSynchronize(c->aux[k]);
+    }
+  } drain{ctx};
+  bool* used_aux = drain.used;
   DevArr<int64_t> dtoff, dteoff;
   DevArr<int32_t> dtdim;
   DevArr<double> tws;
   if (!tdim.empty()) {
     const int nb = (int)tdim.size();
     hipStream_t st = nullptr;
-    (void)aux_handle(ctx, nss_ctx::NAUX - 1, &st);
-    used_aux[nss_ctx::NAUX - 1] = true;
+    (void)aux_handle(ctx, nss_ctx::AUX_TRI, &st);
+    used_aux[nss_ctx::AUX_TRI] = true;
     dtoff.resize(ctx->pool, nb);
     dteoff.resize(ctx->pool, nb);
     dtdim.resize(ctx->pool, nb);
@@ -683,6 +838,48 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   // launch on the context stream (rho is complete: the symmetry check above
   // synchronised the context stream; every batch writes disjoint rows of
   // rho and of the eigenvalue array).
+  // blocks above tmax up to bmax: the large-block path on its own auxiliary
+  // stream (Householder tridiagonalisation, one launch per column over all of
+  // them, then every eigenvalue); their eigenvectors come later
+  std::vector<int> bigs;
+  for (int i = 0; i < ni; ++i)
+    if (items[i].kind == 4) bigs.push_back(i);
+  auto def = std::make_shared<RhoDeferred>();
+  DevArr<TredBlk> dtb;
+  DevArr<int64_t> dbeo;
+  std::vector<int64_t> hbeo;
+  if (!bigs.empty()) {
+    pt.next("rdm.big");
+    hipStream_t st = nullptr;
+    (void)aux_handle(ctx, nss_ctx::AUX_BIG, &st);
+    used_aux[nss_ctx::AUX_BIG] = true;
+    const int nb = (int)bigs.size();
+    int64_t wtot = 0;
+    int nbmax = 0;
+    for (int i : bigs) {
+      def->hwo.push_back(wtot);
+      wtot += tred_ws_size(items[i].d);
+      nbmax = std::max(nbmax, items[i].d);
+    }
+    def->ws.resize(ctx->pool, wtot);
+    for (int k = 0; k < nb; ++k) {
+      const Item& it = items[bigs[k]];
+      TredBlk t{};
+      t.a = rho.p + it.roff;
+      t.ws = def->ws.p + def->hwo[k];
+      t.n = it.d;
+      def->blk.push_back(t);
+      hbeo.push_back(it.eoff);
+    }
+    dtb.resize(ctx->pool, nb);
+    dbeo.resize(ctx->pool, nb);
+    NSS_HIP(hipMemcpyAsync(dtb.p, def->blk.data(), nb * sizeof(TredBlk), hipMemcpyHostToDevice, st));
+    NSS_HIP(hipMemcpyAsync(dbeo.p, hbeo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    for (int j = 0; j < nbmax; ++j) launch_tred_col(dtb.p, nb, j, nbmax, st);
+    launch_tri_eigvals(dtb.p, dbeo.p, dD.p, nb, nbmax, st);
+    def->hws.resize(wtot);
+    NSS_HIP(hipMemcpyAsync(def->hws.data(), def->ws.p, wtot * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
   pt.next("rdm.rocsolver");
   std::map<int, std::vector<int>> classes;
   int dlarge = 0;  // largest block above the single-kernel range
@@ -719,7 +916,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     size_t c = 0;
     for (auto& kv : calls) {
       ClassBatch& B = batches[c];
-      const int k = (int)(c % (nss_ctx::NAUX - 1));
+      const int k = (int)(c % nss_ctx::AUX_TRI);
       rocblas_handle hnd = aux_handle(ctx, k, &B.st);
       used_aux[k] = true;
       ++c;
@@ -771,6 +968,46 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   pt.next("rdm.download");
   std::vector<double> hD(std::max<int64_t>(1, etot));
   if (etot) NSS_HIP(hipMemcpyAsync(hD.data(), dD.p, etot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  // large blocks: their eigenvalues to the deferred record; eigenvectors
+  // computed now (all of them) unless the job lets them wait for the
+  // kept-state selection, then the block of rho holds them as the Jacobi
+  // kernel leaves its output
+  std::vector<RhoVec> bigvec(bigs.size());
+  if (!bigs.empty()) {
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+    def->ctx = ctx;
+    std::vector<RhoVec*> now;
+    std::vector<int> cnt;
+    for (size_t k = 0; k < bigs.size(); ++k) {
+      const Item& it = items[bigs[k]];
+      def->lam.emplace_back(hD.begin() + it.eoff, hD.begin() + it.eoff + it.d);
+      RhoVec& v = bigvec[k];
+      v.d = it.d;
+      v.def = def;
+      v.def_blk = (int)k;
+      v.order.resize(it.d);
+      for (int j = 0; j < it.d; ++j) v.order[j] = j;
+      const std::vector<double>& w = def->lam[k];
+      std::stable_sort(v.order.begin(), v.order.end(), [&](int x, int y) { return w[x] < w[y]; });
+      const RdmJob& J = jobs[it.job];
+      if (!(J.dvecs && J.defer)) {
+        now.push_back(&v);
+        cnt.push_back(it.d);
+      }
+    }
+    if (!now.empty()) {
+      PhaseTimer pm(ctx, "rdm.big_vectors");
+      rho_materialize(ctx, now, cnt);
+      for (size_t k = 0; k < bigs.size(); ++k) {
+        const Item& it = items[bigs[k]];
+        if (!bigvec[k].base) continue;
+        // every position kept: slot = position, so the slab is the
+        // row-major eigenvector matrix
+        NSS_HIP(hipMemcpyAsync(rho.p + it.roff, bigvec[k].base, (size_t)it.d * it.d * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+      }
+    }
+  }
   // eigenvectors to the host only when asked for (host_vectors); otherwise
   // they stay in rho and are handed out as device views
   bool host_vecs = false;
@@ -804,7 +1041,15 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
       rv->order.resize(d);
       for (int j = 0; j < d; ++j) rv->order[j] = j;
     }
-    if (it.kind == 2) {
+    const bool deferred = it.kind == 4 && rv && J.defer;
+    if (deferred) {
+      // eigenvalues by position (ascending within each unreduced block of
+      // T); the view carries the ascending order and stays deferred
+      const size_t k = std::find(bigs.begin(), bigs.end(), (int)(&it - items.data())) - bigs.begin();
+      const std::vector<double>& w = def->lam[k];
+      for (int j = 0; j < d; ++j) ev[j] = w[bigvec[k].order[j]];
+      *rv = bigvec[k];
+    } else if (it.kind == 2) {
       // rocSOLVER: eigenvector j contiguous (component i at j*d + i), ascending
       std::copy(hD.begin() + it.eoff, hD.begin() + it.eoff + d, ev.begin());
       if (vec)
@@ -838,11 +1083,12 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
 
 void sb_rdm_eigh2(nss_ctx* ctx, SbDevice* sb, const double* psi_dev, int sides,
                   std::vector<std::vector<double>> evals[2], std::vector<std::vector<double>> evecs[2],
-                  std::vector<RhoVec>* dvecs) {
+                  std::vector<RhoVec>* dvecs, bool defer) {
   std::vector<RdmJob> jobs(1);
   jobs[0].sb = sb;
   jobs[0].psi = psi_dev;
   jobs[0].sides = sides;
+  jobs[0].defer = defer;
   jobs[0].evals = evals;
   jobs[0].evecs = evecs;
   jobs[0].dvecs = dvecs;

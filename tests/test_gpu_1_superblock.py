@@ -263,3 +263,35 @@ def test_rdm_eigh_rank_deficient_blocks(ctx):
             assert np.allclose(np.maximum(ev, 0), np.maximum(ref, 0), atol=1e-15), (side, k)
             assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-14, (side, k)
             assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-13, (side, k)
+
+
+def test_rdm_eigh_large_block_path(ctx):
+    # blocks of the large-block path (129..2048 states): graded spectra like a
+    # DMRG rho (Schmidt values 10^(-k/8)), a flat random one, low rank, and
+    # the 2048 end; every eigenpair against numpy, scaled by the block's norm
+    print("case large-block rho path", flush=True)
+    dims = [(900, 900), (1500, 40), (2048, 3), (129, 600), (700, 700), (300, 1)]
+    sb = _random_sb(13, dims, 0)
+    upload(ctx, sb)
+    rng = np.random.default_rng(17)
+    psi = np.zeros(sb.n)
+    for k, (dl, dr) in enumerate(dims):
+        r = min(dl, dr)
+        ul = np.linalg.qr(rng.standard_normal((dl, r)))[0]
+        vr = np.linalg.qr(rng.standard_normal((dr, r)))[0]
+        sv = 10.0 ** (-np.arange(r) / 8.0) if k != 4 else rng.uniform(0.5, 1.0, r)
+        psi[sb.offset[k]:sb.offset[k] + dl * dr] = ((ul * sv) @ vr.T).ravel()
+    psi /= np.linalg.norm(psi)
+    for side in ("left", "right"):
+        evs, vecs = ctx.rdm_eigh(psi, side)
+        for k, (ev, vec) in enumerate(zip(evs, vecs)):
+            dl, dr, off = sb.Dl[k], sb.Dr[k], sb.offset[k]
+            p = psi[off:off + dl * dr].reshape(dl, dr)
+            rho = p @ p.T if side == "left" else p.T @ p
+            ref = np.linalg.eigvalsh(rho)
+            nrm = max(np.abs(ref).max(), 1e-300)
+            print(f"  side={side} k={k} d={len(ev)}", flush=True)
+            assert np.all(np.diff(ev) >= 0), (side, k)
+            assert np.abs(np.maximum(ev, 0) - np.maximum(ref, 0)).max() < 1e-13 * nrm, (side, k)
+            assert np.abs(vec @ np.diag(ev) @ vec.T - rho).max() < 1e-13 * nrm, (side, k)
+            assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-12, (side, k)
