@@ -1498,9 +1498,9 @@ struct Engine {
     I.t_hv = nss_hv_timer_flush(ctx);
     I.t_step = secs(S.t0, t5);
     if (par.verbose > 0) {
-      std::printf("step %c%d len %d+%d  m_sb %lld nsect %d  E/L %.15f  S %.12f  m %d,%d  nop %d  t %.3fs\n", type,
-                  label, I.left_len, I.right_len, (long long)msb, I.nsect, I.energies[0] / I.L, I.entropy_left,
-                  I.m_left, I.m_right, I.numop, I.t_step);
+      std::printf("step %c%d len %d+%d  m_sb %lld nsect %d  E/L %.15f  S %.12f  m %d,%d  nop %d (fp32 %d)  t %.3fs\n",
+                  type, label, I.left_len, I.right_len, (long long)msb, I.nsect, I.energies[0] / I.L, I.entropy_left,
+                  I.m_left, I.m_right, I.numop, I.numop_fp32, I.t_step);
       std::fflush(stdout);
     }
     steps.push_back(std::move(st));

@@ -427,6 +427,9 @@ struct LanczosJob {
 // Lowest eigenpair of every job, the H*v of all chains in one grouped launch
 // per phase (nss_lanczos.cpp).
 void lanczos_batch(::nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax, int maxiter, double tol);
+// Mixed-precision cycles end when their residual estimate has fallen by
+// this factor below the cycle's first FP64 residual (NSS_MIXED_INNER, default 1e-5).
+double lz_mixed_inner();
 // Superblock dimension from which a single engine's FP64 lowest-eigenpair
 // solve uses lanczos_batch (partial reorthogonalisation) instead of the
 // thick-restart solver with a Gram-Schmidt pass every iteration
@@ -473,10 +476,14 @@ void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);     // 
 // Batch H*v: merged W and Y plans of several superblocks whose vectors sit at
 // xoff[c] of shared X / Y arenas and whose term scratch sits at woff[c] of a
 // shared W arena (sb_wsize elements each); hv_plans launches them (timed).
+// f32: the plans of the FP32 operand copies (arenas of floats at the same
+// offsets), launched by hv_plans32.
 int64_t sb_wsize(const SbDevice* sb);
 void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
-                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s);
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s,
+                       bool f32 = false);
 void hv_plans(::nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b);
+void hv_plans32(::nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b);
 int64_t sb_n(const SbDevice* sb);
 double sb_flops(const SbDevice* sb);
 int sb_nq(const SbDevice* sb);

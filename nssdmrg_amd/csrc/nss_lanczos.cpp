@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <vector>
 
@@ -302,6 +303,208 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
   }
 }
 
+double lz_mixed_inner() {
+  static const double v = [] {
+    const char* e = std::getenv("NSS_MIXED_INNER");
+    const double x = e ? std::atof(e) : 1e-5;
+    return (x > 0.0 && x < 1.0) ? x : 1e-5;
+  }();
+  return v;
+}
+
+// Ritz values within this fraction of |theta_1| of the lowest join the FP64
+// block of the next cycle (NSS_MIXED_BLOCK_WIN, default 1e-4).
+static double lz_mixed_window() {
+  static const double v = [] {
+    const char* e = std::getenv("NSS_MIXED_BLOCK_WIN");
+    const double x = e ? std::atof(e) : 1e-4;
+    return x >= 0.0 ? x : 1e-4;
+  }();
+  return v;
+}
+
+// Lowest eigenpair in mixed precision: FP32-MFMA H*v for the bulk of the
+// Krylov work, FP64 for everything that decides the answer.  Each cycle:
+//  1. FP64 block step: W = H X for the block X of p <= 3 orthonormal vectors
+//     (V[0..p)), the exact Rayleigh-Ritz G = X^T W = Q Theta Q^T, X <- X Q,
+//     W <- W Q and the exact residuals R = W - X Theta.  The run ends here
+//     when |R_1| <= tol max(1, |theta_1|), the FP64 criterion of the
+//     thick-restart solver.
+//  2. A Lanczos space V[p..) built with FP32-MFMA H*v from R_1 (FP64
+//     vectors, recurrence and a Gram-Schmidt pass against X and V every
+//     iteration), together with the exact couplings C = R^T V (X^T H v =
+//     R^T v for v orthogonal to X).
+//  3. Rayleigh-Ritz of H over [X V]: M = [[Theta, C], [C^T, T]], with the
+//     FP32 rounding confined to the tridiagonal T of the correction space.
+//     The cycle ends when the lowest pair's residual estimate has fallen by
+//     lz_mixed_inner() below |R_1|; the Ritz vectors of M whose values lie
+//     within lz_mixed_window() |theta_1| of the lowest become the next block.
+// A near-degenerate partner (a state closer to the ground state than the
+// FP32 rounding of H*v, which an FP32 Krylov space mixes with it at O(1))
+// thereby enters the block and is separated exactly by the FP64 Rayleigh-Ritz.
+// A cycle whose |R_1| does not fall below a quarter of the previous one (with
+// an unchanged block size) hands over to the FP64 thick-restart solver from X_1.
+// NSS_MIXED_TRACE=1: one line per cycle of the mixed solver on stderr.
+static bool lz_mixed_trace() {
+  static const bool v = [] {
+    const char* e = std::getenv("NSS_MIXED_TRACE");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
+static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double tol, double* x) {
+  nss_ctx* ctx = L.ctx;
+  SbDevice* sb = L.sb;
+  const int64_t n = L.n;
+  const int kmax = L.kmax, S = L.S;
+  hipStream_t s = ctx->stream;
+  constexpr int PMAX = 3;
+  if (kmax < PMAX + 4) return trlan(L, v0, 1, maxiter, tol, false, x)[0];
+  const int64_t gch = gemv_t_chunks(n);
+  DevArr<double> W, R, Tmp, Cd, Gd, hd, gpart;
+  W.resize(ctx->pool, (size_t)PMAX * n);
+  R.resize(ctx->pool, (size_t)PMAX * n);
+  Tmp.resize(ctx->pool, (size_t)PMAX * n);
+  Cd.resize(ctx->pool, (size_t)PMAX * (kmax + 1));
+  Gd.resize(ctx->pool, (size_t)PMAX * PMAX);
+  hd.resize(ctx->pool, (size_t)(kmax + 1) * PMAX);
+  gpart.resize(ctx->pool, (size_t)PMAX * gch + 1);
+  launch_norm(v0, n, L.part.p, L.scal.p, s);
+  launch_div(v0, L.vec(0), n, L.scal.p, s);
+  const double inner = lz_mixed_inner(), window = lz_mixed_window();
+  int p = 1, prev_p = 0;
+  double prev_res = -1.0;
+  std::vector<double> G, th, nrm(PMAX), sl, cc, M, ev;
+  for (int cycle = 0; cycle < std::max(8, maxiter); ++cycle) {
+    // 1. FP64 block step
+    for (int i = 0; i < p; ++i) {
+      sb_hv(ctx, sb, L.vec(i), W.p + (size_t)i * n);
+      ++L.numop;
+    }
+    for (int j = 0; j < p; ++j) launch_gemv_t(L.V.p, n, p, W.p + (size_t)j * n, n, gpart.p, Gd.p + j * p, s);
+    G.resize((size_t)p * p);
+    NSS_HIP(hipMemcpyAsync(G.data(), Gd.p, G.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipStreamSynchronize(s));
+    for (int a = 0; a < p; ++a)  // G[j*p + i] = X_i . W_j; symmetrise
+      for (int b = 0; b < a; ++b) G[(size_t)a * p + b] = G[(size_t)b * p + a] = 0.5 * (G[(size_t)a * p + b] + G[(size_t)b * p + a]);
+    th.assign(p, 0.0);
+    sym_eig(p, G.data(), th.data());  // eigenvectors as columns
+    if (p > 1) {
+      std::vector<double> q((size_t)p * p);
+      for (int i = 0; i < p; ++i)
+        for (int k = 0; k < p; ++k) q[(size_t)i * p + k] = G[(size_t)k * p + i];  // column i contiguous
+      NSS_HIP(hipMemcpyAsync(hd.p, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      for (int pass = 0; pass < 2; ++pass) {  // X <- X Q, then W <- W Q
+        double* base = pass == 0 ? L.V.p : W.p;
+        for (int i = 0; i < p; ++i) launch_gemv_n(base, n, p, hd.p + (size_t)i * p, Tmp.p + (size_t)i * n, n, 1, s);
+        NSS_HIP(hipMemcpyAsync(base, Tmp.p, (size_t)p * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      }
+    }
+    // R_i = W_i - theta_i X_i and |R_i|
+    NSS_HIP(hipMemcpyAsync(R.p, W.p, (size_t)p * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    NSS_HIP(hipMemcpyAsync(hd.p + (size_t)PMAX * PMAX, th.data(), p * sizeof(double), hipMemcpyHostToDevice, s));
+    for (int i = 0; i < p; ++i)
+      launch_gemv_n(L.vec(i), n, 1, hd.p + (size_t)PMAX * PMAX + i, R.p + (size_t)i * n, n, 0, s);
+    launch_norm(R.p, n, L.part.p, L.scal.p, s);
+    double res1 = 0.0;
+    NSS_HIP(hipMemcpyAsync(&res1, L.scal.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    NSS_HIP(hipStreamSynchronize(s));
+    const double lim = tol * std::max(1.0, std::fabs(th[0]));
+    if (lz_mixed_trace())
+      std::fprintf(stderr, "mixed n %lld cycle %d p %d theta %.15g res %.3e\n", (long long)n, cycle, p, th[0], res1);
+    if (res1 <= lim) {
+      NSS_HIP(hipMemcpyAsync(x, L.vec(0), n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      NSS_HIP(hipStreamSynchronize(s));
+      return th[0];
+    }
+    // no progress from the FP32 space: FP64 from here (a cycle that grew the
+    // block has only just separated a near-degenerate pair, and is exempt)
+    if (prev_res > 0.0 && p == prev_p && res1 > 0.25 * prev_res) break;
+    prev_res = res1;
+    prev_p = p;
+    // 2. FP32 Lanczos space from R_1, orthogonal to X
+    NSS_HIP(hipMemcpyAsync(L.w.p, R.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    launch_orth_pass(L.V.p, n, p, L.w.p, n, L.part.p, hd.p, L.sq.p, s);
+    launch_normalize(L.w.p, L.vec(p), n, L.sq.p, L.scal.p + 1, s);
+    launch_gemv_t(R.p, n, p, L.vec(p), n, gpart.p, Cd.p, s);  // C[k*p + i] = R_i . v_{p+k}
+    NSS_HIP(hipMemsetAsync(L.slots.p, 0, (size_t)kmax * S * sizeof(double), s));
+    L.restart_at = p;
+    const double target = std::max(0.5 * lim, inner * res1);
+    std::vector<double> dV(kmax + 1, 0.0), eV(kmax + 1, 0.0), bV(kmax + 1, 0.0);
+    int j = p, read_upto = p, next_check = p + 1, m = 0;
+    bool breakdown = false;
+    for (;;) {
+      L.iterate(j, true);
+      ++j;
+      launch_gemv_t(R.p, n, p, L.vec(j), n, gpart.p, Cd.p + (size_t)(j - p) * p, s);
+      const bool at_end = j == kmax || j == n;
+      if (j < next_check && !at_end) continue;
+      const int cnt = j - read_upto;
+      sl.resize((size_t)cnt * S);
+      NSS_HIP(hipMemcpyAsync(sl.data(), L.slots.p + (size_t)read_upto * S, sl.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+      cc.resize((size_t)(j - p + 1) * p);
+      NSS_HIP(hipMemcpyAsync(cc.data(), Cd.p, cc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      NSS_HIP(hipStreamSynchronize(s));
+      double hn = std::fabs(th[0]);
+      for (int it = read_upto; it < j; ++it) {
+        const double* q = sl.data() + (size_t)(it - read_upto) * S;
+        const int k = it - p;
+        dV[k] = q[it] + q[(kmax + 1) + it];
+        if (k > 0) eV[k - 1] = q[it - 1] + q[(kmax + 1) + it - 1];
+        bV[k] = q[2 * (kmax + 1)];
+        hn = std::max(hn, std::fabs(dV[k]));
+        m = k + 1;
+        if (bV[k] <= 1e-14 * std::max(1.0, hn) || !std::isfinite(bV[k])) {
+          breakdown = true;
+          break;
+        }
+      }
+      read_upto = j;
+      // 3. Rayleigh-Ritz over [X, V[p..p+m)]
+      const int D = p + m;
+      M.assign((size_t)D * D, 0.0);
+      for (int i = 0; i < p; ++i) M[(size_t)i * D + i] = th[i];
+      for (int k = 0; k < m; ++k) {
+        for (int i = 0; i < p; ++i) M[(size_t)i * D + p + k] = M[(size_t)(p + k) * D + i] = cc[(size_t)k * p + i];
+        M[(size_t)(p + k) * D + p + k] = dV[k];
+        if (k + 1 < m) M[(size_t)(p + k) * D + p + k + 1] = M[(size_t)(p + k + 1) * D + p + k] = eV[k];
+      }
+      ev.assign(D, 0.0);
+      sym_eig(D, M.data(), ev.data());
+      const double est = std::fabs(bV[m - 1] * M[(size_t)(D - 1) * D + 0]);
+      if (!(est <= target || at_end || breakdown)) {
+        next_check = std::min(kmax, j + (m < 8 ? 1 : 4));
+        continue;
+      }
+      // the next block: the lowest Ritz vector and those within the window
+      int np = 1;
+      while (np < PMAX && np < D && ev[np] - ev[0] <= window * std::max(1.0, std::fabs(ev[0]))) ++np;
+      if (lz_mixed_trace())
+        std::fprintf(stderr, "mixed   fp32 m %d est %.3e target %.3e%s%s gaps %.3e %.3e next p %d\n", m, est, target,
+                     at_end ? " at_end" : "", breakdown ? " breakdown" : "", D > 1 ? ev[1] - ev[0] : 0.0,
+                     D > 2 ? ev[2] - ev[0] : 0.0, np);
+      std::vector<double> y((size_t)np * D);
+      for (int i = 0; i < np; ++i)
+        for (int a = 0; a < D; ++a) y[(size_t)i * D + a] = M[(size_t)a * D + i];
+      NSS_HIP(hipMemcpyAsync(hd.p, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      for (int i = 0; i < np; ++i) launch_gemv_n(L.V.p, n, D, hd.p + (size_t)i * D, Tmp.p + (size_t)i * n, n, 1, s);
+      NSS_HIP(hipMemcpyAsync(L.V.p, Tmp.p, (size_t)np * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      for (int i = 0; i < np; ++i) {
+        launch_norm(L.vec(i), n, L.part.p, L.scal.p, s);
+        launch_div(L.vec(i), L.vec(i), n, L.scal.p, s);
+      }
+      NSS_HIP(hipStreamSynchronize(s));  // y lives on the host
+      p = np;
+      break;
+    }
+  }
+  // hand-over: FP64 thick restart from the current lowest block vector
+  if (lz_mixed_trace()) std::fprintf(stderr, "mixed n %lld: FP64 hand-over\n", (long long)n);
+  return trlan(L, L.vec(0), 1, maxiter, tol, false, x)[0];
+}
+
 LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int maxiter, double tol,
                           uint64_t seed, int v0_mode, int precision, double* xdev) {
   LzClock clk(ctx);
@@ -347,8 +550,10 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   }
   clk.lap("lz.setup");
   LanczosOut out;
-  if (precision == NSS_FP32_MIXED) {
-    // FP32-MFMA H*v to an FP32-attainable residual, then the FP64 restart
+  if (precision == NSS_FP32_MIXED && neigen == 1) {
+    out.evals.assign(1, lanczos_mixed_block(L, v0.p, maxiter, tol, xdev));
+  } else if (precision == NSS_FP32_MIXED) {
+    // several eigenpairs: FP32-MFMA H*v to an FP32-attainable residual, then the FP64 restart
     trlan(L, v0.p, neigen, 2, std::max(tol, 1e-5), true, xdev, true);
     // start vector of the FP64 run: sum of the FP32 Ritz vectors
     NSS_HIP(hipMemcpyAsync(v0.p, xdev, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -415,11 +620,12 @@ static int lzb_full_reorth() {
 // iteration is one merged W launch and one merged Y launch over all active
 // chains (their vectors in shared X / Y arenas) and five batched vector
 // launches; the host reads every chain's new slots in one copy at the
-// checks (iterations 1..8, then every 4th).  A chain leaves the batch when
-// its residual |beta_{m-1} y_{m-1}| meets tol max(1, |theta|) (the
-// single-chain criterion), at an invariant subspace, or at its kmax; one
-// that is still unconverged there is finished by the single-chain
-// thick-restart solver from its current Ritz vector.
+// checks (iterations 1..8, then every 4th).
+//
+// A chain leaves the batch when its residual |beta_{m-1} y_{m-1}| meets
+// tol max(1, |theta|) (the single-chain criterion), at an invariant
+// subspace, or at its kmax; one that is still unconverged there is finished
+// by the single-chain thick-restart solver from its current Ritz vector.
 void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, int maxiter, double tol) {
   const int K = (int)jobs.size();
   if (K == 0) return;
@@ -475,8 +681,6 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     NSS_HIP(hipStreamSynchronize(s));
   }
   const int full = lzb_full_reorth();
-  GemmPlan Wp, Yp;
-  sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx->pool, s);
   std::vector<LzChainDev> hc(K);
   std::vector<int32_t> bmap(btot);
   for (int c = 0; c < K; ++c) {
@@ -525,6 +729,8 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     launch_div(v0, v0, n[c], scal.p + c, s);
     NSS_HIP(hipMemcpyAsync(hc[c].x, v0, n[c] * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  GemmPlan Wp, Yp;
+  sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx->pool, s);
   // host side of every chain: the projected tridiagonal
   std::vector<std::vector<double>> d(K, std::vector<double>(kb1, 0.0)), e(K, std::vector<double>(kb1, 0.0)),
       beta(K, std::vector<double>(kb1, 0.0));

@@ -32,9 +32,9 @@ namespace nss {
 
 namespace {
 
-constexpr int TRED_THREADS = 256;
+constexpr int TRED_THREADS = 1024;
 constexpr int TRED_WAVES = TRED_THREADS / 64;
-constexpr int TRED_ROWS = 16;  // trailing-matrix rows per workgroup
+constexpr int TRED_ROWS = 16;  // trailing-matrix rows per workgroup: one per wave
 constexpr int EIG_THREADS = 256;
 constexpr int EIG_BIS = 4;     // lanes (interior points) per eigenvalue
 constexpr int EIG_PER_WG = EIG_THREADS / EIG_BIS;
@@ -196,6 +196,7 @@ __global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __rest
     double* row = A + (int64_t)i * n;
     double acc = 0.0;
     if (j > 0) {
+#pragma unroll 4
       for (int c = j + 1 + lane; c < n; c += 64) {
         const int lc = c - j;
         const double a = row[c] - vi * wp[lc] - wi * vp[lc];
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __rest
         acc += a * vj[lc];
       }
     } else {
+#pragma unroll 4
       for (int c = 1 + lane; c < n; c += 64) acc += row[c] * vj[c];
     }
     acc = wsum(acc);

@@ -386,12 +386,14 @@ static void plan_append(GemmPlan& dst, const GemmPlan& src, int64_t xoff, int64_
 }
 
 void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
-                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s) {
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s,
+                       bool f32) {
   W.clear();
   Y.clear();
   for (size_t c = 0; c < sbs.size(); ++c) {
-    plan_append(W, sbs[c]->W64, xoff[c], woff[c]);
-    plan_append(Y, sbs[c]->Y64, xoff[c], woff[c]);
+    NSS_REQUIRE(!f32 || sbs[c]->has32, NSS_E_STATE, "FP32 H*v requested but blocks were uploaded for FP64 only");
+    plan_append(W, f32 ? sbs[c]->W32 : sbs[c]->W64, xoff[c], woff[c]);
+    plan_append(Y, f32 ? sbs[c]->Y32 : sbs[c]->Y64, xoff[c], woff[c]);
   }
   W.upload(pool, s);
   Y.upload(pool, s);
@@ -399,6 +401,7 @@ void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int6
 
 // H*v of merged plans, timed like sb_hv.
 void hv_plans(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b) { hv_timed(ctx, W, Y, b, 0); }
+void hv_plans32(nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b) { hv_timed(ctx, W, Y, b, 1); }
 
 double nss_hv_timer_flush(nss_ctx* ctx) {
   NSS_HIP(hipStreamSynchronize(ctx->stream));
