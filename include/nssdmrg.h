@@ -213,6 +213,17 @@ typedef struct {
    * nss_dmrg_measure refuse PBC runs, as the reference's finite_DMRG does
    * (src/DMRG_MAIN.f90:179-180). */
   int32_t pbc;
+  /* QNtype (src/INPUT_VARS.f90:223-226, sb_set_current_qn
+   * src/DMRG_SUPERBLOCK.f90:717-726): 0 local, the target at L sites is
+   * int(target_qn L) (Norb = 1); 1 global, it is min(L, int(target_qn)). */
+  int32_t qn_type;
+  /* QNtruncation_error / QNtruncation_dim (src/INPUT_VARS.f90:211-217,
+   * eigh_blocks_matrix src/MATRIX/MATRIX_BLOCKS.f90:627-629): every rho
+   * eigenvalue of a QN block whose weight sum is below qn_trunc_error, or
+   * whose dimension is below qn_trunc_dim, becomes tiny(1d0) (so its states
+   * are kept only after all others).  0 and 0 (or 1) leave rho untouched. */
+  double qn_trunc_error;
+  int32_t qn_trunc_dim;
 } nss_params;
 
 typedef struct {
@@ -233,6 +244,11 @@ typedef struct {
   double hv_flops;            /* algorithmic FLOPs of one H*v */
   double t_step, t_diag, t_hv, t_rdm, t_renorm, t_enlarge;  /* seconds */
   int32_t link;               /* 'n', or 'p' for the PBC_prev step of a PBC iteration */
+  /* write_entanglement (src/DMRG_MAIN.f90:501-562): with P_q the weight of
+   * rho's QN block q and S_q the entropy of its normalised spectrum, the
+   * number entropy S_N = -sum_q P_q ln P_q and the configurational entropy
+   * S_c = sum_q P_q S_q, for rho_L and rho_R (S = S_N + S_c). */
+  double sn_left, sc_left, sn_right, sc_right;
 } nss_step_info;
 
 int nss_dmrg_create(nss_ctx* ctx, const nss_model* model, const nss_params* params,
@@ -266,6 +282,16 @@ int nss_dmrg_step_info(nss_dmrg* d, int i, nss_step_info* out);
  * *nout = number of kept sectors. */
 int nss_dmrg_kept_qn(nss_dmrg* d, int i, int side, double min_weight, int max, double* qn,
                      int32_t* count, int32_t* nout);
+/* rho%lambda() of step i (get_evals_blocks_matrix,
+ * src/MATRIX/MATRIX_BLOCKS.f90:660-687, written as LambdaQ_left/right by
+ * write_entanglement, src/DMRG_MAIN.f90:523-561): the eigenvalues of every
+ * QN block of rho_L (side 0) or rho_R (side 1), ascending within a block,
+ * negative ones as tiny(1d0).  Blocks in the superblock's sector order.
+ * Writes up to maxb blocks (qn[2] as doubles, count) and up to maxv values
+ * back to back; *nblocks and *nvals = the totals.  NULL buffers with 0
+ * capacities query the sizes. */
+int nss_dmrg_lambda(nss_dmrg* d, int i, int side, int maxb, double* qn, int32_t* count, int64_t maxv,
+                    double* vals, int32_t* nblocks, int64_t* nvals);
 /* Superblock of the most recent step (for checks and CPU baselines): sector
  * dims and dense host copies of its operator blocks in the layout of
  * nss_sb_upload_blocks (row-major).  Call with NULL buffers to get sizes. */

@@ -227,7 +227,8 @@ DEFAULTS = {  # src/INPUT_VARS.f90:141-339
     "DEG_EVALS_THRESHOLD": 0.1, "VERBOSE": 3, "SPARSE_H": False, "LANC_NEIGEN": 2,
     "LANC_NCV_FACTOR": 10, "LANC_NCV_ADD": 0, "LANC_NITER": 512, "LANC_TOLERANCE": 1e-12,
     "LANC_DIM_THRESHOLD": 1024, "LANC_V0_DBLE": False, "SUN": 2, "HVEC": [0.0, 0.0, 0.0],
-    "TS": [-1.0, -1.0], "MH": [0.0, 0.0],
+    "TS": [-1.0, -1.0], "MH": [0.0, 0.0], "QNTYPE": "local", "QNTRUNCATION_ERROR": 0.0,
+    "QNTRUNCATION_DIM": 1,
 }
 
 
@@ -321,6 +322,9 @@ class DMRGConfig:
     verbose: int = 0
     predict: bool = False   # start finite-step Lanczos from the transformed previous ground state
     pbc: bool = False       # PBCdmrg: periodic ring (infinite algorithm; an even ldmrg becomes odd)
+    qn_type: str = "local"  # QNtype: "local" (target int(q L)) or "global" (target min(L, int(q)))
+    qn_trunc_error: float = 0.0  # QNtruncation_error
+    qn_trunc_dim: int = 1        # QNtruncation_dim
 
 
 class DMRG:
@@ -354,6 +358,11 @@ class DMRG:
         p.verbose = cfg.verbose
         p.predict = int(bool(cfg.predict))
         p.pbc = int(bool(cfg.pbc))
+        if cfg.qn_type[:1].lower() not in ("l", "g"):
+            raise ValueError("DMRG_MAIN error: QNtype != [local,global]")
+        p.qn_type = 1 if cfg.qn_type[:1].lower() == "g" else 0
+        p.qn_trunc_error = float(cfg.qn_trunc_error)
+        p.qn_trunc_dim = int(cfg.qn_trunc_dim)
         h = C.c_void_p()
         F.check(F.lib().nss_dmrg_create(ctx.handle, C.byref(m), C.byref(p), C.byref(h)))
         self._h = h
@@ -422,6 +431,23 @@ class DMRG:
         F.check(F.lib().nss_dmrg_kept_qn(self._h, i, sd, min_weight, n.value, F.dptr(qn), F.iptr(cnt),
                                          C.byref(n)))
         return {(qn[2 * j], qn[2 * j + 1]): int(cnt[j]) for j in range(n.value) if cnt[j] > 0}
+
+    def lambdas(self, i: int, side: str = "left") -> list:
+        """rho%lambda() of step i (the LambdaQ_left / LambdaQ_right lists of
+        write_entanglement): [(qn, eigenvalues ascending), ...] per QN block."""
+        sd = 0 if side == "left" else 1
+        nb, nv = C.c_int32(), C.c_int64()
+        F.check(F.lib().nss_dmrg_lambda(self._h, i, sd, 0, None, None, 0, None, C.byref(nb), C.byref(nv)))
+        qn = np.zeros(2 * max(nb.value, 1))
+        cnt = np.zeros(max(nb.value, 1), dtype=np.int32)
+        vals = np.zeros(max(nv.value, 1))
+        F.check(F.lib().nss_dmrg_lambda(self._h, i, sd, nb.value, F.dptr(qn), F.iptr(cnt), nv.value, F.dptr(vals),
+                                        C.byref(nb), C.byref(nv)))
+        out, o = [], 0
+        for j in range(nb.value):
+            out.append(((qn[2 * j], qn[2 * j + 1]), vals[o:o + cnt[j]].copy()))
+            o += cnt[j]
+        return out
 
     def last_superblock(self) -> dict:
         nq, nt, n = C.c_int(), C.c_int(), C.c_int64()
@@ -493,7 +519,8 @@ def init_dmrg(Hij, ModelDot, cfg: dict | None = None, ctx: Context | None = None
                       neigen=int(c["LANC_NEIGEN"]), lanc_dim_threshold=int(c["LANC_DIM_THRESHOLD"]),
                       lanc_maxiter=int(c["LANC_NITER"]), lanc_tol=float(c["LANC_TOLERANCE"]),
                       v0_mode=1 if c["LANC_V0_DBLE"] else 0, deg_thr=float(c["DEG_EVALS_THRESHOLD"]),
-                      pbc=bool(c["PBCDMRG"]))
+                      pbc=bool(c["PBCDMRG"]), qn_type=str(c["QNTYPE"]).lower(),
+                      qn_trunc_error=float(c["QNTRUNCATION_ERROR"]), qn_trunc_dim=int(c["QNTRUNCATION_DIM"]))
     for k, v in over.items():
         setattr(conf, k, v)
     if c["DMRGTYPE"].lower().startswith("i"):

@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("lanc_kmax", C.c_int32), ("lanc_maxiter", C.c_int32), ("lanc_tol", C.c_double),
                 ("seed", C.c_uint64), ("v0_mode", C.c_int32), ("deg_thr", C.c_double),
                 ("precision", C.c_int32), ("verbose", C.c_int32), ("predict", C.c_int32),
-                ("pbc", C.c_int32)]
+                ("pbc", C.c_int32), ("qn_type", C.c_int32), ("qn_trunc_error", C.c_double),
+                ("qn_trunc_dim", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -55,7 +56,8 @@ class StepInfo(C.Structure):
                 ("numop", C.c_int32), ("numop_fp32", C.c_int32), ("hv_flops", C.c_double),
                 ("t_step", C.c_double), ("t_diag", C.c_double), ("t_hv", C.c_double),
                 ("t_rdm", C.c_double), ("t_renorm", C.c_double), ("t_enlarge", C.c_double),
-                ("link", C.c_int32)]
+                ("link", C.c_int32), ("sn_left", C.c_double), ("sc_left", C.c_double),
+                ("sn_right", C.c_double), ("sc_right", C.c_double)]
 
 
 _P = C.c_void_p
@@ -102,6 +104,7 @@ SIGNATURES = {
     "nss_dmrg_nsteps": (C.c_int, [_P]),
     "nss_dmrg_step_info": (C.c_int, [_P, C.c_int, C.POINTER(StepInfo)]),
     "nss_dmrg_kept_qn": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_int, _D, _I32, _I32]),
+    "nss_dmrg_lambda": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _D, _I32, C.c_int64, _D, _I32, _I64]),
     "nss_dmrg_last_sb_dims": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64]),
     "nss_dmrg_last_sb_export": (C.c_int, [_P, _I32, _I32, _I32, _I32, _D, _D, _D, _D, _D]),
     "nss_dmrg_destroy": (C.c_int, [_P]),

@@ -185,6 +185,8 @@ struct Step {
   nss_step_info info{};
   // kept sectors: QN and the rho eigenvalues of their kept states
   std::vector<std::pair<QN, std::vector<double>>> kept_left, kept_right;
+  // rho%lambda(): every QN block's eigenvalues (ascending), superblock sector order
+  std::vector<std::pair<QN, std::vector<double>>> lam_left, lam_right;
 };
 
 struct Engine {
@@ -1266,11 +1268,56 @@ struct Engine {
 
   // ------------------------------------------------------------ step
   QN target_of(int L) const {
-    // QNtype=local: int(target_qn*L*Norb), src/DMRG_SUPERBLOCK.f90:722
+    // sb_set_current_qn (src/DMRG_SUPERBLOCK.f90:717-726), Norb = 1:
+    // QNtype=local int(target_qn*L), QNtype=global min(L, int(target_qn))
+    auto one = [&](double q) { return par.qn_type == 1 ? std::min(L, (int)q) : (int)(q * L); };
     QN t;
-    t.a = 2 * (int)(par.target_qn[0] * L);
-    t.b = site.fermion ? 2 * (int)(par.target_qn[1] * L) : 0;
+    t.a = 2 * one(par.target_qn[0]);
+    t.b = site.fermion ? 2 * one(par.target_qn[1]) : 0;
     return t;
+  }
+
+  // QNtruncation_error / QNtruncation_dim of eigh_blocks_matrix
+  // (src/MATRIX/MATRIX_BLOCKS.f90:627-629): a QN block whose eigenvalues sum
+  // below the error threshold, or whose dimension is below the dimension
+  // threshold, gets tiny(1d0) for every eigenvalue
+  void qn_truncate(std::vector<std::vector<double>>& ev) const {
+    if (par.qn_trunc_error <= 0.0 && par.qn_trunc_dim <= 1) return;
+    for (std::vector<double>& b : ev) {
+      if (b.empty()) continue;
+      double w = 0.0;
+      for (double x : b) w += x;
+      if (w < par.qn_trunc_error || (int)b.size() < par.qn_trunc_dim) std::fill(b.begin(), b.end(), DBL_MIN);
+    }
+  }
+
+  // write_entanglement (src/DMRG_MAIN.f90:519-560): the QN blocks' weights
+  // P_q and entropies S_q of the normalised spectra give S_N = -sum P_q ln P_q
+  // and S_c = sum P_q S_q, with the same 1d-32 regularisation as S.  A block of
+  // zero weight (the reference would divide by it) contributes nothing.
+  static void entropy_split(const std::vector<std::vector<double>>& ev, double& sn, double& sc) {
+    sn = sc = 0.0;
+    for (const std::vector<double>& b : ev) {
+      if (b.empty()) continue;
+      double P = 0.0;
+      for (double x : b) P += x;
+      sn -= P * std::log(P + 1e-32);
+      if (!(P > 0.0)) continue;
+      double Sq = 0.0;
+      for (double x : b) {
+        const double o = x / P;
+        Sq += o * std::log(o + 1e-32);
+      }
+      sc -= P * Sq;
+    }
+  }
+
+  std::vector<std::pair<QN, std::vector<double>>> lambdas(const std::vector<std::vector<double>>& ev,
+                                                          const Block& E, const std::vector<int>& sbsec) const {
+    std::vector<std::pair<QN, std::vector<double>>> out;
+    for (size_t q = 0; q < ev.size(); ++q)
+      if (!ev[q].empty()) out.push_back({E.sec[sbsec[q]].q, ev[q]});
+    return out;
   }
 
   // One DMRG step (step_dmrg, src/DMRG_MAIN.f90:265-460) in three phases:
@@ -1423,6 +1470,12 @@ struct Engine {
     PhaseTimer pt(ctx, "step.spectrum");
     const int farside = S.far ? -1 : (label == 1 ? 1 : 0);
     if (farside >= 0) o.evs[farside] = far_spectrum(o.evs[1 - farside], farside);
+    qn_truncate(o.evs[0]);
+    qn_truncate(o.evs[1]);
+    entropy_split(o.evs[0], I.sn_left, I.sc_left);
+    entropy_split(o.evs[1], I.sn_right, I.sc_right);
+    st.lam_left = lambdas(o.evs[0], left, sbL);
+    st.lam_right = lambdas(o.evs[1], right, sbR);
     Spectrum sl = sort_spectrum(std::move(o.evs[0]), std::move(o.dvs[0]));
     Spectrum sr = sort_spectrum(std::move(o.evs[1]), std::move(o.dvs[1]));
     const auto t4 = clk::now();
@@ -1880,6 +1933,33 @@ extern "C" int nss_dmrg_kept_qn(nss_dmrg* d, int i, int side, double min_weight,
   NSS_API_END
 }
 
+extern "C" int nss_dmrg_lambda(nss_dmrg* d, int i, int side, int maxb, double* qn, int32_t* count, int64_t maxv,
+                               double* vals, int32_t* nblocks, int64_t* nvals) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && nblocks && nvals, NSS_E_ARG, "nss_dmrg_lambda: NULL arguments");
+  NSS_REQUIRE(i >= 0 && i < (int)d->e->steps.size(), NSS_E_ARG, "nss_dmrg_lambda: step out of range");
+  NSS_REQUIRE(side == 0 || side == 1, NSS_E_ARG, "nss_dmrg_lambda: side must be 0 (left) or 1 (right)");
+  const auto& L = side == 0 ? d->e->steps[i].lam_left : d->e->steps[i].lam_right;
+  *nblocks = (int32_t)L.size();
+  int64_t tot = 0, w = 0;
+  for (int j = 0; j < (int)L.size(); ++j) {
+    const auto& b = L[j];
+    if (j < maxb) {
+      if (qn) {
+        qn[2 * j] = 0.5 * b.first.a;
+        qn[2 * j + 1] = 0.5 * b.first.b;
+      }
+      if (count) count[j] = (int32_t)b.second.size();
+    }
+    for (double x : b.second) {
+      if (vals && w < maxv) vals[w++] = x;
+      ++tot;
+    }
+  }
+  *nvals = tot;
+  NSS_API_END
+}
+
 extern "C" int nss_dmrg_last_sb_dims(nss_dmrg* d, int* nq, int* nterms, int64_t* n) {
   NSS_API_BEGIN
   NSS_REQUIRE(d && d->e, NSS_E_ARG, "NULL handle");
@@ -2001,6 +2081,10 @@ extern "C" int nss_dmrg_destroy(nss_dmrg* d) {
     sb_reset(c->sb);
     c->engines.erase(std::remove(c->engines.begin(), c->engines.end(), d), c->engines.end());
     delete d->e;
+    // an engine's blocks can hold a large share of the device: hand the
+    // pool's cache back once no queued work can use it
+    NSS_HIP(hipStreamSynchronize(c->stream));
+    c->pool->trim();
   }
   delete d;
   NSS_API_END

@@ -48,9 +48,13 @@ class DevicePool {
   int device() const { return device_; }
   size_t bytes_in_use() const { return in_use_; }
   size_t bytes_reserved() const { return reserved_; }
+  // Returns every cached (free) block to HIP.  Callers make sure no pending
+  // device work still uses memory freed into the pool.
+  void trim();
 
  private:
   int device_;
+  size_t cache_cap_ = 0;  // cached bytes above which free() trims (a quarter of the device memory)
   std::map<size_t, std::vector<void*>> free_;
   std::map<void*, size_t> live_;
   size_t in_use_ = 0, reserved_ = 0;

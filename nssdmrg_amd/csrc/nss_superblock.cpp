@@ -66,6 +66,22 @@ void DevicePool::free(void* p) {
   free_[it->second].push_back(p);
   in_use_ -= it->second;
   live_.erase(it);
+  if (cache_cap_ == 0) {
+    size_t fr = 0, tot = 0;
+    cache_cap_ = hipMemGetInfo(&fr, &tot) == hipSuccess && tot > 0 ? tot / 4 : (size_t(16) << 30);
+  }
+  // the runtime allocates memory of its own at kernel launches; a cache that
+  // keeps most of the device would starve it
+  if (reserved_ - in_use_ > cache_cap_) trim();
+}
+
+void DevicePool::trim() {
+  for (auto& kv : free_)
+    for (void* q : kv.second) {
+      (void)hipFree(q);
+      reserved_ -= kv.first;
+    }
+  free_.clear();
 }
 
 DevicePool::~DevicePool() {

@@ -43,6 +43,9 @@ module nss_dmrg
      real(c_double) :: deg_thr = 0.1d0
      integer(c_int32_t) :: precision = 0, verbose = 0, predict = 0
      integer(c_int32_t) :: pbc = 0
+     integer(c_int32_t) :: qn_type = 0
+     real(c_double) :: qn_trunc_error = 0d0
+     integer(c_int32_t) :: qn_trunc_dim = 1
   end type nss_params
 
   type, bind(C), public :: nss_step_info
@@ -56,6 +59,7 @@ module nss_dmrg
      real(c_double) :: hv_flops
      real(c_double) :: t_step, t_diag, t_hv, t_rdm, t_renorm, t_enlarge
      integer(c_int32_t) :: link
+     real(c_double) :: sn_left, sc_left, sn_right, sc_right
   end type nss_step_info
 
   !> dense operator block of a superblock sector (column-major, Fortran)

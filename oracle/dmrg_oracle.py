@@ -26,7 +26,12 @@ full chains, and invariants (symmetry of H_sb, trace of rho, lambda_L=lambda_R).
 The PBC path (link-'p' steps) is pinned by periodic-ring closed forms
 (Heisenberg rings of 4 and 6 sites, free-fermion rings), a Fock-space ED of
 the Hubbard ring, a brute-force spin-1 ring, the fermion algebra of both
-block ends and the symmetry of every PBC H_sb.  Under PBC the spin terms are
+block ends and the symmetry of every PBC H_sb.  The entanglement split
+(``entropy_split``, ``lambda_lists``) is pinned by the exact reduced density
+matrix of an ED ground state built from explicit Kronecker products and by
+S = S_N + S_c; QNtype=global by ED of the targeted sector; the QN-sector
+truncation knobs by their limits (every eigenvalue tiny(1d0) above every
+sector's weight or dimension, nothing changed below).  Under PBC the spin terms are
 the Hermitian ones of connect_spin_blocks(link='p'); the reference's direct
 spin setup differs there (see setup_direct).
 """
@@ -297,8 +302,11 @@ class Superblock:
         return int((self.Dl * self.Dr).sum())
 
 
-def current_target(length: int, target_qn, norb: int = 1) -> tuple:
-    """sb_set_current_qn, QNtype=local: int(target*L*Norb), src/DMRG_SUPERBLOCK.f90:717-726."""
+def current_target(length: int, target_qn, norb: int = 1, qn_type: str = "local") -> tuple:
+    """sb_set_current_qn, src/DMRG_SUPERBLOCK.f90:717-726: QNtype=local
+    int(target*L*Norb), QNtype=global min(L, int(target*Norb))."""
+    if qn_type[:1].lower() == "g":
+        return tuple(float(min(length, int(t * norb))) for t in target_qn)
     return tuple(float(int(t * length * norb)) for t in target_qn)
 
 
@@ -500,7 +508,8 @@ class RhoBlocks:
     eorder: np.ndarray = None       # (block, position) per sorted eval
 
 
-def build_rho(sb: Superblock, psi: np.ndarray, side: str) -> RhoBlocks:
+def build_rho(sb: Superblock, psi: np.ndarray, side: str, qn_trunc_error: float = 0.0,
+              qn_trunc_dim: int = 1) -> RhoBlocks:
     """sb_get_rdm + build_density_matrix, src/DMRG_RDM.f90:18-162."""
     qns, maps, blocks = [], [], []
     for k, q in enumerate(sb.qns):
@@ -517,14 +526,19 @@ def build_rho(sb: Superblock, psi: np.ndarray, side: str) -> RhoBlocks:
         if np.any(np.abs(rho - rho.T) > 1e-12):
             raise RuntimeError("build_density_matrix error: rho not Symmetric")
         blocks.append(rho)
-    return eigh_blocks(qns, maps, blocks)
+    return eigh_blocks(qns, maps, blocks, qn_trunc_error, qn_trunc_dim)
 
 
-def eigh_blocks(qns, maps, blocks) -> RhoBlocks:
-    """eigh_blocks_matrix(sort=T, reverse=T), src/MATRIX/MATRIX_BLOCKS.f90:589-657."""
+def eigh_blocks(qns, maps, blocks, qn_trunc_error: float = 0.0, qn_trunc_dim: int = 1) -> RhoBlocks:
+    """eigh_blocks_matrix(sort=T, reverse=T), src/MATRIX/MATRIX_BLOCKS.f90:589-657,
+    with the QN-sector truncation of :627-629 (a block whose eigenvalues sum
+    below QNtruncation_error, or whose dimension is below QNtruncation_dim,
+    has all its eigenvalues set to tiny(1d0)) before negatives become tiny."""
     ev_blk, vec_blk, allv, order = [], [], [], []
     for ib, rho in enumerate(blocks):
         e, u = np.linalg.eigh(rho)
+        if np.sum(e) < qn_trunc_error or len(e) < qn_trunc_dim:
+            e = np.full_like(e, TINY)
         e = np.where(e < 0.0, TINY, e)
         ev_blk.append(e)
         vec_blk.append(u)
@@ -582,6 +596,30 @@ def entropy(evals: np.ndarray) -> float:
     return float(-np.sum(ev * np.log(ev + EPS_LOG)))
 
 
+def entropy_split(rho: RhoBlocks) -> tuple:
+    """write_entanglement S_N and S_c, src/DMRG_MAIN.f90:519-560: with
+    P_q = sum_i lambda_q(i) (rho%lambda(), MATRIX_BLOCKS.f90:660-687) and
+    S_q = sum_i (lambda/P_q) log(lambda/P_q + 1e-32),
+    S_N = -sum_q P_q log(P_q + 1e-32), S_c = -sum_q P_q S_q.  A zero-weight
+    block (a division by zero in the reference) contributes nothing."""
+    sn = sc = 0.0
+    for e in rho.evals_blk:
+        if len(e) == 0:
+            continue
+        P = float(np.sum(e))
+        sn -= P * math.log(P + EPS_LOG)
+        if P > 0.0:
+            o = e / P
+            sc -= P * float(np.sum(o * np.log(o + EPS_LOG)))
+    return sn, sc
+
+
+def lambda_lists(rho: RhoBlocks) -> list:
+    """rho%lambda() per QN block (LambdaQ_left/right of write_entanglement):
+    [(qn, eigenvalues ascending)], blocks in superblock sector order."""
+    return [(tuple(float(x) for x in q), np.sort(e)) for q, e in zip(rho.qns, rho.evals_blk) if len(e)]
+
+
 # ---------------------------------------------------------------------------
 # DMRG driver (src/DMRG_MAIN.f90)
 # ---------------------------------------------------------------------------
@@ -600,6 +638,9 @@ class Params:
     norb: int = 1
     ncv_max: int = 600
     pbc: bool = False               # PBCdmrg (infinite algorithm only)
+    qn_type: str = "local"          # QNtype
+    qn_trunc_error: float = 0.0     # QNtruncation_error
+    qn_trunc_dim: int = 1           # QNtruncation_dim
 
 
 @dataclass
@@ -626,6 +667,12 @@ class StepRecord:
     kept_w_right: dict = None
     cut_left: float = 0.0        # lambda_{m_s}
     cut_right: float = 0.0
+    sn_left: float = 0.0         # number / configurational entanglement entropies
+    sc_left: float = 0.0
+    sn_right: float = 0.0
+    sc_right: float = 0.0
+    lam_left: list = None        # rho%lambda() lists
+    lam_right: list = None
 
 
 class DMRG:
@@ -655,7 +702,7 @@ class DMRG:
         self.left = enlarge_block(self.left, self.dot, "left", p.hij, link)
         self.right = enlarge_block(self.right, self.dot, "right", p.hij, link)
         L = self.left.length + self.right.length
-        target = current_target(L, p.target_qn, p.norb)
+        target = current_target(L, p.target_qn, p.norb, p.qn_type)
         sb = sb_get_states(self.left, self.right, target)
         setup_direct(sb, self.left, self.right, p.hij, p.pbc)
         msb = sb.n
@@ -670,8 +717,8 @@ class DMRG:
                                                p.lanc_tol, p.ncv_max)
         psi = evecs[:, 0]
         self.last_sb, self.last_psi = sb, psi
-        rl = build_rho(sb, psi, "left")
-        rr = build_rho(sb, psi, "right")
+        rl = build_rho(sb, psi, "left", p.qn_trunc_error, p.qn_trunc_dim)
+        rr = build_rho(sb, psi, "right", p.qn_trunc_error, p.qn_trunc_dim)
         if kind == "i":
             do_ren = self.left.length != p.ldmrg
         else:
@@ -690,7 +737,8 @@ class DMRG:
             cl, cr = float(lw[-1]), float(rw[-1])
         rec = StepRecord(kind, self.left.length, self.right.length, L, evals,
                          evals / L / p.norb, entropy(rl.evalues), entropy(rr.evalues),
-                         ml, mr, tl, tr, kl, kr, msb, len(sb.qns), nop, hv_flops(sb), wl, wr, cl, cr)
+                         ml, mr, tl, tr, kl, kr, msb, len(sb.qns), nop, hv_flops(sb), wl, wr, cl, cr,
+                         *entropy_split(rl), *entropy_split(rr), lambda_lists(rl), lambda_lists(rr))
         self.records.append(rec)
         return rec
 

@@ -494,3 +494,67 @@ def test_pbc_restart_files_carry_both_ends(ctx, tmp_path, model):
     d4 = N.DMRG(ctx, N.DMRGConfig(pbc=True, **kw))
     with pytest.raises(N.NssError, match="operator missing"):
         d4.load_blocks(op, orp)
+
+
+# ------------------------------------------------------------ entanglement split, QN options
+@pytest.mark.parametrize("model", ["spin1", "hubbard"])
+def test_entanglement_split_and_lambdas_vs_oracle(ctx, model):
+    # write_entanglement's S_N, S_c and rho%lambda() per QN block against the
+    # oracle, step by step (infinite and finite steps)
+    print(f"case entanglement split {model}", flush=True)
+    if model == "hubbard":
+        osite, site, hij, tq, U, L, M = O.electron_site(3.0, True), N.electron_site(), HUB_HIJ, (0.5, 0.5), 3.0, 5, 40
+    else:
+        osite, site, hij, tq, U, L, M = O.spin_site(3), N.spin_site(3), SPIN_HIJ, (0.0, 0.0), 0.0, 6, 30
+    p = O.Params(ldmrg=L, mdmrg=M, target_qn=tq[:2] if model == "hubbard" else (0.0,), hij=hij, neigen=1,
+                 lanc_dim_threshold=64, msweep=[M])
+    od = O.DMRG(osite, p)
+    od.finite()
+    d, st = run(ctx, "f", site=site, hij=np.array(hij), ldmrg=L, mdmrg=M, msweep=[M], target_qn=tq, neigen=1,
+                lanc_dim_threshold=64, uloc=U, hfmode=True)
+    assert len(st) == len(od.records)
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert abs(s["sn_left"] - r.sn_left) < 1e-9 and abs(s["sc_left"] - r.sc_left) < 1e-9, i
+        assert abs(s["sn_right"] - r.sn_right) < 1e-9 and abs(s["sc_right"] - r.sc_right) < 1e-9, i
+        # per QN block (the engine's sector order is its own)
+        got = {(q[0], q[1]): v for q, v in d.lambdas(i, "left")}
+        ref = {(float(q[0]), float(q[1]) if len(q) > 1 else 0.0): v for q, v in r.lam_left}
+        assert set(got) == set(ref), i
+        for q, rv in ref.items():
+            gv = got[q]
+            big = rv > W_MIN
+            assert len(gv) == len(rv) and np.allclose(gv[big], rv[big], rtol=1e-8, atol=1e-14), (i, q)
+
+
+def test_qntype_global_and_qn_truncation_vs_oracle(ctx):
+    print("case QNtype=global, QN truncation knobs", flush=True)
+    # global targets: N_up = N_dw = min(L, 3) at every length, untruncated (a
+    # truncated run keeps rho null-space states of the early rank-deficient
+    # steps, whose basis is arbitrary).  As in the reference, the states of
+    # a block sector with no complement in the superblock have no rho block
+    # and are dropped (4 sites at N_up = N_dw = 3: 9 of 16), so from 6 sites
+    # on the run follows the oracle, not the exact sector ground state
+    dot = O.electron_site(2.0, True)
+    p = O.Params(ldmrg=5, mdmrg=4096, target_qn=(3.0, 3.0), hij=HUB_HIJ, neigen=1, lanc_dim_threshold=64,
+                 qn_type="global")
+    od = O.DMRG(dot, p)
+    od.infinite()
+    d, st = run(ctx, "i", site=N.electron_site(), hij=np.array(HUB_HIJ), ldmrg=5, mdmrg=4096,
+                target_qn=(3.0, 3.0), qn_type="global", neigen=1, lanc_dim_threshold=64, uloc=2.0, hfmode=True)
+    assert [s["L"] for s in st] == [r.L for r in od.records] == [4, 6, 8, 10]
+    assert abs(st[0]["energies"][0] - O.ed_ground(dot, 4, np.array(HUB_HIJ), (3.0, 3.0))[0]) < 1e-10
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert s["m_sb"] == r.m_sb and s["m_left"] == r.m_left, i
+        assert abs(s["energies"][0] - r.energies[0]) < 1e-10 * max(1.0, abs(r.energies[0])), i
+    # QNtruncation_dim: QN blocks of fewer than 3 states weigh tiny(1d0)
+    p = O.Params(ldmrg=6, mdmrg=16, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1, lanc_dim_threshold=64,
+                 qn_trunc_dim=3)
+    od = O.DMRG(O.spin_site(2), p)
+    od.infinite()
+    d, st = run(ctx, "i", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=6, mdmrg=16, target_qn=(0.0, 0.0),
+                neigen=1, lanc_dim_threshold=64, qn_trunc_dim=3)
+    for i, (s, r) in enumerate(zip(st, od.records)):
+        assert abs(s["energies"][0] - r.energies[0]) < 1e-10 * max(1.0, abs(r.energies[0])), i
+        assert abs(s["entropy_left"] - r.entropy_left) < 1e-9, i
+        small = [ev for q, ev in d.lambdas(i, "left") if len(ev) < 3]
+        assert all(np.all(ev == np.finfo(np.float64).tiny) for ev in small), i

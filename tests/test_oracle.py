@@ -440,3 +440,101 @@ def test_pbc_superblock_hamiltonian_is_symmetric():
                 d.step("i", link=link)
                 h = O.sb_dense(d.last_sb)
                 assert np.allclose(h, h.T, atol=1e-12)
+
+
+# ------------------------------------------------------------ entanglement split
+def _spin_chain_ed(L):
+    """Heisenberg chain H = sum Sz Sz + (S+S- + S-S+)/2 by explicit Kronecker
+    products (no oracle code): H, per-state Sz."""
+    sz = np.diag([0.5, -0.5])
+    sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    eye = np.eye(2)
+
+    def op(o, i):
+        m = np.array([[1.0]])
+        for k in range(L):
+            m = np.kron(m, o if k == i else eye)
+        return m
+    h = np.zeros((2 ** L, 2 ** L))
+    for i in range(L - 1):
+        h += op(sz, i) @ op(sz, i + 1) + 0.5 * (op(sp, i) @ op(sp.T, i + 1) + op(sp.T, i) @ op(sp, i + 1))
+    tot = sum(np.diag(op(sz, i)) for i in range(L))
+    return h, tot
+
+
+def test_lambda_lists_equal_exact_reduced_density_matrix():
+    # untruncated L=6 step: rho_L's spectrum per Sz sector of the left half
+    # equals that of the exact ground state's reduced density matrix
+    L = 6
+    h, tot = _spin_chain_ed(L)
+    sel = np.isclose(tot, 0.0)
+    e, u = np.linalg.eigh(h[np.ix_(sel, sel)])
+    psi = np.zeros(2 ** L)
+    psi[sel] = u[:, 0]
+    m = psi.reshape(2 ** (L // 2), 2 ** (L // 2))
+    rho = m @ m.T
+    _, totl = _spin_chain_ed(L // 2)
+    p = O.Params(ldmrg=L // 2, mdmrg=4096, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1, lanc_dim_threshold=4096)
+    d = O.DMRG(O.spin_site(2), p)
+    d.infinite()
+    rec = d.records[-1]
+    assert rec.L == L and abs(rec.energies[0] - e[0]) < 1e-12
+    got = {q[0]: ev for q, ev in rec.lam_left}
+    assert set(got) == set(np.unique(totl).tolist())
+    for q, ev in got.items():
+        s = np.isclose(totl, q)
+        ref = np.linalg.eigvalsh(rho[np.ix_(s, s)])
+        assert np.allclose(np.sort(np.maximum(ev, 0)), np.maximum(ref, 0), atol=1e-13), q
+
+
+@pytest.mark.parametrize("model", ["spin", "hubbard"])
+def test_entropy_is_number_plus_configurational(model):
+    # S = S_N + S_c for every rho (the split of write_entanglement); a
+    # dropped weight or a sign slip in S_N / S_c breaks the identity
+    if model == "spin":
+        d = O.DMRG(O.spin_site(3), O.Params(ldmrg=5, mdmrg=20, target_qn=(0.0,), hij=SPIN_HIJ, msweep=[20]))
+    else:
+        d = O.DMRG(O.electron_site(3.0, True), O.Params(ldmrg=4, mdmrg=24, target_qn=(0.5, 0.5), hij=HUB_HIJ,
+                                                         msweep=[24]))
+    d.finite()
+    for r in d.records:
+        assert abs(r.sn_left + r.sc_left - r.entropy_left) < 1e-12
+        assert abs(r.sn_right + r.sc_right - r.entropy_right) < 1e-12
+        tr = sum(float(np.sum(ev)) for _, ev in r.lam_left)
+        assert abs(tr - 1.0) < 1e-12
+    assert max(r.sn_left for r in d.records) > 1e-3 and max(r.sc_left for r in d.records) > 1e-3
+
+
+def test_qntype_global_matches_exact_sector():
+    # QNtype=global: the target at every length is min(L, int(q)); an
+    # untruncated 6-site Hubbard run at N_up = N_dw = 2 equals the exact
+    # ground energy of that sector (4 sites: min(4, 2) = 2 as well)
+    dot = O.electron_site(4.0, True)
+    p = O.Params(ldmrg=3, mdmrg=4096, target_qn=(2.0, 2.0), hij=HUB_HIJ, neigen=1, lanc_dim_threshold=4096,
+                 qn_type="global")
+    d = O.DMRG(dot, p)
+    d.infinite()
+    for r in d.records:
+        ref = O.ed_ground(dot, r.L, np.array(HUB_HIJ), (2.0, 2.0))[0]
+        assert abs(r.energies[0] - ref) < 1e-10, r.L
+
+
+def test_qn_truncation_knobs():
+    # QNtruncation_dim above every block dimension (or QNtruncation_error
+    # above every block weight) sets every rho eigenvalue to tiny(1d0): the
+    # entropies vanish and the kept states carry no weight
+    for kw in (dict(qn_trunc_dim=10 ** 6), dict(qn_trunc_error=2.0)):
+        p = O.Params(ldmrg=5, mdmrg=8, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1, **kw)
+        d = O.DMRG(O.spin_site(2), p)
+        d.infinite()
+        for r in d.records:
+            assert abs(r.entropy_left) < 1e-300 and abs(r.sn_left) < 1e-300
+            assert all(np.all(ev == O.TINY) for _, ev in r.lam_left)
+    # a threshold below every block's weight changes nothing
+    base = O.DMRG(O.spin_site(2), O.Params(ldmrg=5, mdmrg=8, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1))
+    base.infinite()
+    d = O.DMRG(O.spin_site(2), O.Params(ldmrg=5, mdmrg=8, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1,
+                                        qn_trunc_error=1e-300))
+    d.infinite()
+    for a, b in zip(base.records, d.records):
+        assert a.energies[0] == b.energies[0] and a.entropy_left == b.entropy_left
