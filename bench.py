@@ -1,25 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the superblock hot path on one MI355X.
 
-Workload (default): 1D Hubbard chain, t=1, U=4, half filling, [N_up,N_dw]
-quantum numbers, OBC, L=64 sites in the finite sweeps, m=256 kept states,
-plain FP64 H_sb.psi.  The reference's finite algorithm sweeps a superblock of
-2*LDMRG+2 sites (src/DMRG_MAIN.f90:168-241, fNlat=2*LDMRG), so L=64 is
-LDMRG=31 (the iDMRG growth ends at 62 sites).  HFMODE=T (the reference
-default), LANC_NEIGEN=1, LANC_TOLERANCE=1e-12, LANC_DIM_THRESHOLD=128.
-Lanczos start: SplitMix64 uniform[-1,1) with seed 1 for the iDMRG steps and
-the first finite step of each sweep direction; every later finite step starts
-from the previous ground state carried into the new basis (wavefunction
-prediction).  The same sweeps with a random start in every step (the
-reference's behaviour) are timed beside it under "fp64_random_start".
+Workload (default, spin1_L100_m1024): spin-1 Heisenberg chain, J=1
+(Hij = diag(1, 1/2)), OBC, Sz=0, L=100 sites in the finite sweeps, m=1024
+kept states, 4 finite sweeps.  The reference's finite algorithm sweeps a
+superblock of 2*LDMRG+2 sites (src/DMRG_MAIN.f90:168-241), so L=100 is
+LDMRG=49.  LANC_NEIGEN=1, LANC_TOLERANCE=1e-12, LANC_DIM_THRESHOLD=128.
+Lanczos start: SplitMix64 uniform[-1,1) from seed 2 for the iDMRG steps and
+the first finite step of each sweep direction; every later finite step
+starts from the previous ground state carried into the new basis
+(wavefunction prediction).
 
-A step is one finite sweep (left->right, right->left, back to the middle).
-The iDMRG growth and --warmup sweeps run untimed; --steps sweeps are timed.
-value = H_sb.psi GFLOP/s: algorithmic FLOPs of every H.v in the timed sweeps
-(dense sector blocks, reference evaluation order) over their GPU time measured
-with HIP events on the context stream; ms_per_step = finite-sweep wall time.
-The FP32-MFMA path (FP64 Lanczos recurrence, rho and final FP64 restart)
-is timed on the same workload and reported under "fp32_mixed".
+A step is one finite DMRG step (step_dmrg, src/DMRG_MAIN.f90:262-433: the
+enlargement, the Lanczos ground state through H_sb.psi, the rho eigensolve
+and the truncation of one superblock).  After the iDMRG growth, --warmup
+finite steps run untimed, --steps finite steps are timed (each ends with
+the device synchronised), and the run then completes the workload's finite
+sweeps, whose wall times are reported as finite_sweep_wall_s.
+value = H_sb.psi GFLOP/s of the FP64 path over the timed steps: algorithmic
+FLOPs of every H.v (dense sector blocks, reference evaluation order) over
+their GPU time measured with HIP events on the context stream.  The same
+workload is then run with the FP32-MFMA H.v (FP64 Lanczos recurrence, FP64
+Rayleigh-Ritz blocks, FP64 rho) and reported under "fp32_mixed", with its
+sweep times and |dE0/L| against the FP64 run.
+
+--workload hubbard_L64_m256: the Hubbard chain t=1, U=4, half filling, OBC,
+L=64, m=256, 3 finite sweeps, the same definitions.
 
 --workload hubbard_batch64_L32: 64 independent Hubbard chains of L=32 (m,
 U and filling per chain from the seed) stepped in lockstep, every step's
@@ -30,6 +36,11 @@ phase over all chains; a step is one sweep of all chains.
 filling, m=2048).  The reference refuses finite sweeps under PBC, so the
 iDMRG growth through the 48-site ring is timed (a step is one iDMRG step),
 in FP64 and with the FP32-MFMA H.v, whose |dE0/L| against FP64 is reported.
+
+--workload spin1_L100_m1024_hv_replay: the iDMRG growth of the default
+workload, then --steps FP64 H.v products on its last superblock (1.5M
+states, 13 sectors) and nothing else: the kernel replay that hardware
+counter passes (rocprofv3 --pmc) are taken on.
 """
 from __future__ import annotations
 
@@ -51,12 +62,15 @@ PEAK_FP32_MFMA = 157.3e12
 METRIC = "finite-sweep wall time + H·v GFLOP/s, FP64 vs FP32-MFMA at |ΔE0/L|≤1e-9"
 
 WORKLOADS = {
-    # the headline workload: Hubbard U=4, OBC, L=64, m=256, finite sweeps
-    "hubbard_L64_m256": dict(model="electron", L=64, m=256, U=4.0, seed=1),
-    # smaller variants for quick checks
-    "hubbard_L32_m128": dict(model="electron", L=32, m=128, U=4.0, seed=1),
-    "spin12_L40_m64": dict(model="spin12", L=40, m=64, U=0.0, seed=0),
-    "spin1_L100_m1024": dict(model="spin1", L=100, m=1024, U=0.0, seed=2),
+    # the headline workload: spin-1 Heisenberg, L=100, m=1024, 4 finite sweeps
+    "spin1_L100_m1024": dict(model="spin1", L=100, m=1024, U=0.0, seed=2, sweeps=4),
+    # Hubbard U=4, OBC, L=64, m=256, 3 finite sweeps
+    "hubbard_L64_m256": dict(model="electron", L=64, m=256, U=4.0, seed=1, sweeps=3),
+    # spin-1/2 Heisenberg, L=40, m=64, 2 finite sweeps (the CPU-sized workload)
+    "spin12_L40_m64": dict(model="spin12", L=40, m=64, U=0.0, seed=0, sweeps=2),
+    "hubbard_L32_m128": dict(model="electron", L=32, m=128, U=4.0, seed=1, sweeps=2),
+    # kernel replay for counter passes: the default workload's growth, then H.v only
+    "spin1_L100_m1024_hv_replay": dict(model="spin1", L=100, m=1024, U=0.0, seed=2, sweeps=1, replay=True),
     # 64 independent Hubbard chains, L=32, per chain from the seed m in
     # [128, 512], U in [0, 8), N in {16, 14, 18} (N_up = N_dw = N/2)
     "hubbard_batch64_L32": dict(model="electron", L=32, batch=64, m_lo=128, m_hi=512, U_hi=8.0,
@@ -137,48 +151,92 @@ def make_engine(N, ctx, w, nsweeps, precision, verbose=0, predict=True):
 
 
 def run_path(N, ctx, w, warmup, steps, precision, verbose, predict=True):
-    d = make_engine(N, ctx, w, warmup + steps, precision, verbose, predict)
+    """iDMRG growth, `warmup` untimed finite steps, `steps` timed finite steps,
+    then the rest of the workload's finite sweeps."""
+    d = make_engine(N, ctx, w, w["sweeps"], precision, verbose, predict)
     t0 = time.perf_counter()
     d.finite_begin()
     t_idmrg = time.perf_counter() - t0
-    for im in range(warmup):
-        d.sweep(im)
-    n0 = len(d.steps())
+    n_growth = len(d.steps())
+    for _ in range(warmup):
+        if not d.finite_step():
+            raise SystemExit("bench.py: --warmup exceeds the workload's finite steps")
     ctx.hv_stats(reset=True)
-    walls = []
-    for im in range(warmup, warmup + steps):
-        t = time.perf_counter()
-        d.sweep(im)
-        walls.append(time.perf_counter() - t)
+    t = time.perf_counter()
+    for _ in range(steps):  # each finite step returns with the device synchronised
+        if not d.finite_step():
+            raise SystemExit("bench.py: --warmup + --steps exceed the workload's finite steps")
+    timed = time.perf_counter() - t
     st = ctx.hv_stats(reset=True)
-    timed = d.steps()[n0:]
-    return d, dict(t_idmrg=t_idmrg, walls=walls, stats=st, steps=timed)
+    while d.finite_step():
+        pass
+    st_rest = ctx.hv_stats(reset=True)
+    allsteps = d.steps()
+    fin = allsteps[n_growth:]
+    walls = [float(sum(x["t_step"] for x in fin if x["sweep"] == k)) for k in range(1, w["sweeps"] + 1)]
+    return d, dict(t_idmrg=t_idmrg, timed=timed, stats=st, stats_rest=st_rest, growth=allsteps[:n_growth],
+                   finite=fin, timed_steps=fin[warmup:warmup + steps], sweep_walls=walls)
 
 
-def cpu_baseline(d, budget_s=15.0):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def sb_flops(sb):
+    Dl, Dr = sb["Dl"].astype(float), sb["Dr"].astype(float)
+    f = float(np.sum(2.0 * Dl * Dr * (Dl + Dr)))
+    for t, s_ in zip(sb["dst"], sb["src"]):
+        f += 2.0 * Dr[t] * (Dl[s_] * Dr[s_] + Dl[t] * Dl[s_])
+    return f
+
+
+def cpu_baseline(d, finite=None, budget_s=15.0):
     """Scalar C restatement of spMatVec_direct_main (oracle/hv_direct.c) on the
-    last superblock of the timed sweeps, one core."""
+    last superblock of the run, one core (this process pinned to it for the
+    timing); with the finite steps' records, the FP64 sweep time such a CPU
+    H.v would need (H.v calls x FLOPs of each step at the measured rate)."""
     from oracle import hv_ref
     sb = d.last_superblock()
     c = hv_ref.CsrSuperblock(sb["Dl"], sb["Dr"], sb["HL"], sb["HR"], sb["dst"], sb["src"], sb["A"], sb["B"])
-    flops = float(np.sum(2.0 * sb["Dl"].astype(float) * sb["Dr"] * (sb["Dl"] + sb["Dr"].astype(float))))
-    for t, s_ in zip(sb["dst"], sb["src"]):
-        flops += 2.0 * sb["Dr"][t] * (float(sb["Dl"][s_]) * sb["Dr"][s_] + float(sb["Dl"][t]) * sb["Dl"][s_])
+    flops = sb_flops(sb)
     v = np.random.default_rng(0).uniform(-1, 1, c.n)
     out = np.empty_like(v)
-    c.hv(v, out)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        c.hv(v, out)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 2000:
-            break
-    return {"value": flops * n / el / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"{n} calls of the scalar C restatement of spMatVec_direct_main on the last superblock "
-                      f"of the timed sweeps (m_sb={c.n}, {c.nq} sectors, {c.nt} terms), {el:.1f} s, "
-                      f"host {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
-            "per_hv_ms": 1e3 * el / n}
+    aff = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+    core = min(aff) if aff else None
+    if core is not None:
+        os.sched_setaffinity(0, {core})
+    try:
+        n, t0 = 0, time.perf_counter()
+        while True:
+            c.hv(v, out)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget_s or n >= 2000:
+                break
+    finally:
+        if core is not None:
+            os.sched_setaffinity(0, aff)
+    rate = flops * n / el
+    res = {"value": rate / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+           "sample": f"{n} calls of the scalar C restatement of spMatVec_direct_main (row-list sparse blocks, "
+                     f"the reference's loop order) on the last superblock of the run (m_sb={c.n}, {c.nq} sectors, "
+                     f"{c.nt} terms), {el:.1f} s on one core pinned with sched_setaffinity, CPU {cpu_model()}, "
+                     f"nproc={os.cpu_count()}; the reference's 250 ms per-step sleep is not timed",
+           "per_hv_ms": 1e3 * el / n}
+    if finite:
+        est = {}
+        for k in sorted({x["sweep"] for x in finite}):
+            est[k] = sum(x["numop"] * x["hv_flops"] for x in finite if x["sweep"] == k) / rate
+        res["finite_sweep_hv_estimate_s"] = [est[k] for k in sorted(est)]
+        res["finite_sweep_hv_estimate_note"] = ("H.v time alone of each FP64 sweep on one core: every step's H.v "
+                                                "calls x its algorithmic FLOPs at the rate measured above")
+    return res
 
 
 PROFILES = os.path.join(ROOT, "profiles")
@@ -334,16 +392,49 @@ def main_pbc(N, ctx, w, a):
     print(json.dumps(out), flush=True)
 
 
+def main_replay(N, ctx, w, a):
+    """The default workload's iDMRG growth, then a.steps FP64 H.v products on
+    its last superblock (kernel replay for counter passes)."""
+    d = make_engine(N, ctx, w, 1, "fp64", a.verbose)
+    last = d.run("i")[-1]
+    # the context keeps the engine's last superblock: ctx.hv applies its H_sb
+    v = np.random.default_rng(0).uniform(-1.0, 1.0, int(last["m_sb"]))
+    ctx.hv_stats(reset=True)
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.hv(v)
+    wall = time.perf_counter() - t
+    st = ctx.hv_stats(reset=True)
+    calls = max(1, st["calls"])
+    y_avg_s, y_flops = st["t_y"] / calls, st["flops_y"] / calls
+    out = {"workload": a.workload, "hv_calls": st["calls_all"], "wall_s": wall, "m_sb": int(v.size),
+           "nsect": int(last["nsect"]), "nterms": int(last["nterms"]),
+           "k_hv_y_f64": {"avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
+                          "tflops": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0},
+           "k_hv_w_f64": {"avg_launch_us": 1e6 * st["t_w"] / calls, "flops_per_launch": st["flops_w"] / calls}}
+    d.close()
+    ctx.close()
+    if a.calib_copy:
+        copy_probe(a.calib_copy, 1)
+        print(f"[bench] calibration copy: {a.calib_copy} MiB read + {a.calib_copy} MiB written", flush=True)
+    print(json.dumps(out), flush=True)
+
+
+def time_split(steps):
+    return {k: float(sum(s_[k] for s_ in steps)) for k in
+            ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed finite sweeps")
-    ap.add_argument("--warmup", type=int, default=0, help="untimed finite sweeps after the iDMRG growth")
-    ap.add_argument("--workload", default="hubbard_L64_m256", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20, help="timed finite DMRG steps")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed finite DMRG steps after the iDMRG growth")
+    ap.add_argument("--workload", default="spin1_L100_m1024", choices=sorted(WORKLOADS))
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-MFMA comparison run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-random-start", action="store_true",
-                    help="skip the FP64 run whose every Lanczos starts from the seeded random vector")
+    ap.add_argument("--random-start", action="store_true",
+                    help="also run the FP64 path with the seeded random Lanczos start in every step")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--sequential", action="store_true",
                     help="batch workloads: run the chains one after another instead of in one batch")
@@ -363,85 +454,102 @@ def main():
     if w.get("pbc"):
         main_pbc(N, ctx, w, a)
         return
-    print(f"[bench] workload {a.workload}: FP64 path, {a.warmup} warmup + {a.steps} timed sweeps", flush=True)
+    if w.get("replay"):
+        main_replay(N, ctx, w, a)
+        return
+    print(f"[bench] workload {a.workload}: FP64 path, iDMRG growth, {a.warmup} warmup + {a.steps} timed finite "
+          f"steps, then the rest of {w['sweeps']} finite sweeps", flush=True)
+    copy_gbs = copy_probe(1024, 10)
     d64, r64 = run_path(N, ctx, w, a.warmup, a.steps, "fp64", a.verbose)
     st = r64["stats"]
     hv_time = st["t_w"] + st["t_y"]
     value = st["flops"] / hv_time / 1e9 if hv_time > 0 else 0.0
-    ms_per_step = 1e3 * float(np.mean(r64["walls"]))
-    e64 = r64["steps"][-1]["e_per_site"][0]
-    print(f"[bench] fp64 done: {ms_per_step:.1f} ms/sweep, H.v {value:.1f} GFLOP/s, E0/L {e64:.12f}", flush=True)
+    ms_per_step = 1e3 * r64["timed"] / a.steps
+    fin = r64["finite"]
+    e64 = fin[-1]["e_per_site"][0]
+    print(f"[bench] fp64 done: {ms_per_step:.1f} ms/step, H.v {value:.1f} GFLOP/s, sweeps "
+          f"{[round(x, 2) for x in r64['sweep_walls']]} s, E0/L {e64:.12f}", flush=True)
     calls = max(1, st["calls"])
     y_avg_s = st["t_y"] / calls
     y_flops = st["flops_y"] / calls
     roofline = {"bound": "mfma", "kernel": "k_hv_y_f64",
                 "kernel_variants": "k_hv_y_f64 (32x32 split-K) or k_hv_y_f64_t64 (64x64 LDS-staged), "
                                    "chosen per superblock by its tile count; the figures cover both",
-                "achieved": y_flops / y_avg_s / 1e12,
+                "achieved": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0,
                 "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
-                "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA,
+                "frac": (y_flops / y_avg_s) / PEAK_FP64_MFMA if y_avg_s > 0 else 0.0,
                 "traffic": profiled_traffic(a.workload, "k_hv_y_f64"),
                 "avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
                 "w_kernel": {"name": "k_hv_w_f64", "avg_launch_us": 1e6 * st["t_w"] / calls,
                              "flops_per_launch": st["flops_w"] / calls}}
-    steps = r64["steps"]
+    tsteps = r64["timed_steps"]
     out = {
         "metric": METRIC, "value": value, "unit": "GFLOP/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": a.workload, "model": "hubbard" if w["model"] == "electron" else w["model"],
-                   "L": w["L"], "ldmrg": (w["L"] - 2) // 2, "m": w["m"], "U": w["U"], "bc": "obc",
-                   "hfmode": True, "neigen": 1, "lanc_tol": 1e-12, "seed": w["seed"],
+                   "L": w["L"], "ldmrg": (w["L"] - 2) // 2, "m": w["m"], "finite_sweeps": w["sweeps"],
+                   "U": w["U"], "bc": "obc", "hfmode": True, "neigen": 1, "lanc_tol": 1e-12, "seed": w["seed"],
+                   "step": "one finite DMRG step (step_dmrg: enlargement, Lanczos ground state, rho eigensolve, "
+                           "truncation), timed after the iDMRG growth and the warmup steps",
                    "lanczos_start": "predicted wavefunction after the first step of each sweep direction"},
         "roofline": roofline,
-        "sweep_wall_s": r64["walls"], "idmrg_wall_s": r64["t_idmrg"],
+        "timed_steps": {"positions": [f"{x['type']}{x['label']} {x['left_len']}+{x['right_len']}" for x in tsteps],
+                        "m_sb": [int(x["m_sb"]) for x in tsteps], "hv_calls": [int(x["numop"]) for x in tsteps]},
+        "finite_sweep_wall_s": r64["sweep_walls"], "idmrg_wall_s": r64["t_idmrg"],
         **hv_fields(st, hv_time),
-        "e0_per_site": e64, "max_m_sb": int(max(s["m_sb"] for s in steps)),
-        "max_nsect": int(max(s["nsect"] for s in steps)),
-        "time_split_s": {k: float(sum(s[k] for s in steps)) for k in
-                         ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
+        "finite_hv_gflops_all_sweeps": (st["flops"] + r64["stats_rest"]["flops"]) /
+        max(1e-30, hv_time + r64["stats_rest"]["t_w"] + r64["stats_rest"]["t_y"]) / 1e9,
+        "e0_per_site": e64, "max_m_sb": int(max(x["m_sb"] for x in fin)),
+        "max_nsect": int(max(x["nsect"] for x in fin)),
+        "time_split_finite_s": time_split(fin),
+        "hbm_copy_probe_gbs": copy_gbs,
     }
-    out["hbm_copy_probe_gbs"] = copy_probe(1024, 10)
     if os.environ.get("NSS_PROFILE") == "1":
         out["phase_timers_s"] = ctx.timers()
-    if not a.no_random_start:
+    if a.random_start:
         print("[bench] FP64 path, random Lanczos start in every step (no wavefunction prediction)", flush=True)
         dr, rr = run_path(N, ctx, w, a.warmup, a.steps, "fp64", a.verbose, predict=False)
-        sr = rr["stats"]
-        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(steps, rr["steps"]))
+        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(fin, rr["finite"]))
         out["fp64_random_start"] = {
-            "ms_per_step": 1e3 * float(np.mean(rr["walls"])), "hv_calls": sr["calls_all"],
-            "hv_gflops": sr["flops"] / max(1e-30, sr["t_w"] + sr["t_y"]) / 1e9,
-            "max_abs_dE0_per_site_vs_predicted": de,
-            "time_split_s": {k: float(sum(s_[k] for s_ in rr["steps"])) for k in
-                             ("t_enlarge", "t_diag", "t_hv", "t_rdm", "t_renorm", "t_step")},
-        }
-        print(f"[bench] random start done: {out['fp64_random_start']['ms_per_step']:.1f} ms/sweep, "
-              f"{sr['calls_all']} H.v, max|dE0/L| vs predicted {de:.2e}", flush=True)
+            "finite_sweep_wall_s": rr["sweep_walls"], "hv_calls_finite": int(sum(x["numop"] for x in rr["finite"])),
+            "max_abs_dE0_per_site_vs_predicted": de}
         dr.close()
+    cpu = None
+    if not a.no_cpu:
+        print("[bench] CPU baseline (1 core, scalar C restatement)", flush=True)
+        cpu = cpu_baseline(d64, fin)
+    d64.close()
     if not a.no_mixed:
         print("[bench] FP32-MFMA path", flush=True)
         d32, r32 = run_path(N, ctx, w, a.warmup, a.steps, "mixed", a.verbose)
         s32 = r32["stats"]
-        e32 = r32["steps"][-1]["e_per_site"][0]
-        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(steps, r32["steps"]))
+        f32 = r32["finite"]
+        allx = r64["growth"] + fin
+        ally = r32["growth"] + f32
+        de = max(abs(x["e_per_site"][0] - y["e_per_site"][0]) for x, y in zip(allx, ally))
         t32 = s32["t_w32"] + s32["t_y32"]
+        n32 = int(sum(x["numop_fp32"] for x in f32))
+        n64 = int(sum(x["numop"] - x["numop_fp32"] for x in f32))
         out["fp32_mixed"] = {
-            "ms_per_step": 1e3 * float(np.mean(r32["walls"])),
+            "ms_per_step": 1e3 * r32["timed"] / a.steps,
+            "finite_sweep_wall_s": r32["sweep_walls"],
+            "speedup_vs_fp64_sweeps": sum(r64["sweep_walls"]) / max(1e-30, sum(r32["sweep_walls"])),
             "hv_fp32_gflops": s32["flops32"] / t32 / 1e9 if t32 > 0 else 0.0,
-            "hv_fp32_calls": s32["calls_all32"], "hv_fp64_refine_calls": s32["calls_all"],
             "roofline_y": {"kernel": "k_hv_y_f32", "frac": (s32["flops_y32"] / max(1e-30, s32["t_y32"])) /
                            PEAK_FP32_MFMA, "peak_tflops": PEAK_FP32_MFMA / 1e12},
-            "e0_per_site": e32, "max_abs_dE0_per_site_vs_fp64": de,
-            "speedup_vs_fp64_sweep": ms_per_step / (1e3 * float(np.mean(r32["walls"]))),
+            "finite_hv_calls": {"fp32": n32, "fp64": n64, "fp64_fraction": n64 / max(1, n32 + n64)},
+            "e0_per_site": f32[-1]["e_per_site"][0],
+            "max_abs_dE0_per_site_vs_fp64": de,
+            "max_abs_dS_vs_fp64": max(abs(x["entropy_left"] - y["entropy_left"]) for x, y in zip(allx, ally)),
+            "time_split_finite_s": time_split(f32),
         }
-        print(f"[bench] mixed done: {out['fp32_mixed']['ms_per_step']:.1f} ms/sweep, max|dE0/L| {de:.2e}",
-              flush=True)
+        print(f"[bench] mixed done: {out['fp32_mixed']['ms_per_step']:.1f} ms/step, sweeps "
+              f"{[round(x, 2) for x in r32['sweep_walls']]} s, max|dE0/L| {de:.2e}, FP64 H.v "
+              f"{100 * out['fp32_mixed']['finite_hv_calls']['fp64_fraction']:.1f}%", flush=True)
         d32.close()
-    if not a.no_cpu:
-        print("[bench] CPU baseline (1 core, scalar C restatement)", flush=True)
-        out["cpu_baseline"] = cpu_baseline(d64)
-    d64.close()
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     ctx.close()
     if a.calib_copy:
         copy_probe(a.calib_copy, 1)

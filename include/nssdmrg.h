@@ -260,6 +260,12 @@ int nss_dmrg_run(nss_dmrg* d, int type);
  * msweep[im], :214-239) one call at a time. */
 int nss_dmrg_finite_begin(nss_dmrg* d);
 int nss_dmrg_sweep(nss_dmrg* d, int im);
+/* One step of the finite sweeps after nss_dmrg_finite_begin: the next
+ * step_dmrg of the three loops of sweep after sweep (src/DMRG_MAIN.f90:
+ * 214-239), returning on the device-synchronised end of that step; *done = 1
+ * (and no step run) once every MSWEEP sweep is complete.  Sweeps and single
+ * steps may be mixed as long as nss_dmrg_sweep starts at a sweep boundary. */
+int nss_dmrg_finite_step(nss_dmrg* d, int* done);
 int nss_dmrg_nsteps(nss_dmrg* d);
 /* Independent chains solved together (no counterpart in the reference,
  * whose runs are one chain per process): the n engines share one context

@@ -379,6 +379,12 @@ class DMRG:
         """Finite sweep im (0-based) with Mstates = msweep[im]."""
         F.check(F.lib().nss_dmrg_sweep(self._h, im))
 
+    def finite_step(self) -> bool:
+        """The next step of the finite sweeps; False once they are complete."""
+        done = C.c_int32()
+        F.check(F.lib().nss_dmrg_finite_step(self._h, C.byref(done)))
+        return not done.value
+
     def steps(self) -> list:
         n = F.lib().nss_dmrg_nsteps(self._h)
         out = []

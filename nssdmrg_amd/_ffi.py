@@ -98,6 +98,7 @@ SIGNATURES = {
     "nss_dmrg_run": (C.c_int, [_P, C.c_int]),
     "nss_dmrg_finite_begin": (C.c_int, [_P]),
     "nss_dmrg_sweep": (C.c_int, [_P, C.c_int]),
+    "nss_dmrg_finite_step": (C.c_int, [_P, _I32]),
     "nss_dmrg_batch_run": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_int]),
     "nss_dmrg_batch_finite_begin": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "nss_dmrg_batch_sweep": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.c_int]),

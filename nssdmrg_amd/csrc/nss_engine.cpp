@@ -1601,28 +1601,53 @@ struct Engine {
       bl2[right.length] = right;
     }
     finite_ready = true;
+    fs_im = fs_loop = 0;
+  }
+
+  // Position in the finite algorithm: sweep fs_im, loop fs_loop of its three
+  // (left->right to the right end, right->left to the left end, left->right
+  // back to the middle).
+  int fs_im = 0, fs_loop = 0;
+
+  // The next step of the finite sweeps (src/DMRG_MAIN.f90:214-239); false
+  // once sweep `last` (or the last of MSWEEP) is complete.  The far block of a
+  // step is needed only by the last step of each loop (the next loop
+  // enlarges it; after the last it is the final state).
+  bool finite_step(int last) {
+    NSS_REQUIRE(finite_ready, NSS_E_STATE, "finite sweep before nss_dmrg_finite_begin");
+    const int fn = 2 * par.ldmrg;
+    for (;;) {
+      if (fs_im > last || fs_im >= (int)msweep.size()) return false;
+      if (fs_loop == 0 && left.length < fn) {
+        right = bl2.at(fn - left.length);
+        step('f', 1, fs_im, left.length + 1 >= fn);
+        bl1[left.length] = left;
+        return true;
+      }
+      if (fs_loop == 0) fs_loop = 1;
+      if (fs_loop == 1 && right.length < fn) {
+        left = bl1.at(fn - right.length);
+        step('f', 2, fs_im, right.length + 1 >= fn);
+        bl2[right.length] = right;
+        return true;
+      }
+      if (fs_loop == 1) fs_loop = 2;
+      if (left.length <= par.ldmrg) {
+        right = bl2.at(fn - left.length);
+        step('f', 1, fs_im, left.length + 1 > par.ldmrg);
+        bl1[left.length] = left;
+        return true;
+      }
+      fs_loop = 0;
+      ++fs_im;
+    }
   }
 
   void sweep(int im) {
     NSS_REQUIRE(finite_ready, NSS_E_STATE, "finite sweep before nss_dmrg_finite_begin");
     NSS_REQUIRE(im >= 0 && im < (int)msweep.size(), NSS_E_ARG, "sweep index outside MSWEEP");
-    const int fn = 2 * par.ldmrg;
-    // the far block of a step is needed only by the last step of each loop
-    // (the next loop enlarges it; after the last it is the final state)
-    while (left.length < fn) {
-      right = bl2.at(fn - left.length);
-      step('f', 1, im, left.length + 1 >= fn);
-      bl1[left.length] = left;
-    }
-    while (right.length < fn) {
-      left = bl1.at(fn - right.length);
-      step('f', 2, im, right.length + 1 >= fn);
-      bl2[right.length] = right;
-    }
-    while (left.length <= par.ldmrg) {
-      right = bl2.at(fn - left.length);
-      step('f', 1, im, left.length + 1 > par.ldmrg);
-      bl1[left.length] = left;
+    NSS_REQUIRE(im == fs_im && fs_loop == 0, NSS_E_STATE, "finite sweeps run in order, each one whole");
+    while (finite_step(im)) {
     }
   }
 
@@ -1841,6 +1866,14 @@ extern "C" int nss_dmrg_finite_begin(nss_dmrg* d) {
   NSS_REQUIRE(d && d->e, NSS_E_ARG, "nss_dmrg_finite_begin: NULL handle");
   NSS_HIP(hipSetDevice(d->e->ctx->device));
   d->e->finite_begin();
+  NSS_API_END
+}
+
+extern "C" int nss_dmrg_finite_step(nss_dmrg* d, int* done) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(d && d->e && done, NSS_E_ARG, "nss_dmrg_finite_step: NULL arguments");
+  NSS_HIP(hipSetDevice(d->e->ctx->device));
+  *done = d->e->finite_step(INT32_MAX) ? 0 : 1;
   NSS_API_END
 }
 
