@@ -124,7 +124,7 @@ struct Bases {
   double* dot = nullptr;
 };
 
-// Forced tile edge (NSS_GEMM_TILE = 32 or 64), 0 for the size-based choice.
+// Forced tile edge (NSS_GEMM_TILE = 32, 64 or 128), 0 for the size-based choice.
 int gemm_default_tile();
 
 // One grouped launch: tiles over outputs, segments over K-contributions.
@@ -140,6 +140,14 @@ struct GemmPlan {
   // waves share LDS-staged operands; for large launches).  NSS_GEMM_TILE
   // forces either.
   int tile = 32;
+  // 128 -> k_*_t128 (128x128 tile, two-stage LDS ring; for launches with
+  // large operands): each tile's K chain cut into nsplit parts, one
+  // workgroup each, summed in part order by the tile's last workgroup
+  // through part (partial tiles) and cnt (arrival counters, zero between
+  // launches)
+  int nsplit = 1;
+  DevArr<double> part;
+  DevArr<unsigned> cnt;
   std::vector<GemmTile> outs;  // one per add_output (row0 = col0 = 0)
   void clear() { htiles.clear(); hsegs.clear(); outs.clear(); flops = 0.0; }
   // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
@@ -255,6 +263,22 @@ int64_t tred_ws_size(int n);
 // Column j of the Householder tridiagonalisation of every block (launch j = 0
 // .. nmax-1 in order; nmax = the largest n of the batch).
 void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t s);
+// The whole tridiagonalisation in one cooperative launch with the trailing
+// matrices in LDS (k_tred_lds): the plan gives each block that fits a
+// number of workgroups (gcount) and maps workgroup ids to (block, index)
+// (wgmap, XCD-aware); covered[b] = 0 leaves block b to launch_tred_col.
+struct TredLdsPlan {
+  std::vector<int2> wgmap;
+  std::vector<int> gcount, covered;
+  int grid = 0;
+  size_t lds = 0;
+};
+bool tred_lds_plan(int device, const std::vector<int>& dims, TredLdsPlan* plan);
+// bar: one zeroed counter per block; abort: zeroed, set when a barrier wait
+// times out (the results are then invalid).  Returns 0 when the cooperative
+// launch is refused (nothing ran).
+int launch_tred_lds(const TredBlk* blks, const int2* wgmap, const int* gcnt, unsigned* bar, int* abort, int grid,
+                    size_t lds, hipStream_t s);
 // Every eigenvalue of every T (d, e in ws): position p to w[eoff[b] + p],
 // ascending within each unreduced block of T.
 void launch_tri_eigvals(const TredBlk* blks, const int64_t* eoff, double* w, int nblk, int nmax, hipStream_t s);
@@ -292,7 +316,8 @@ void launch_unpad_eig(const double* vecs, const double* vals, int ncls, const in
                       const int64_t* eoff, int nb, double* dst, double* w, hipStream_t s);
 
 // out[b] = max_{i<j} |a_ij - a_ji| of the d[b] x d[b] row-major block at a + off[b]
-void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, double* out,
+// (dmax >= every d[b]; NaN differences report +inf)
+void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, int dmax, double* out,
                      hipStream_t s);
 
 // Kronecker-scatter for block enlargement: dst sub-block (rows x cols, ld)
@@ -361,6 +386,11 @@ struct nss_ctx {
   double* lz_pin = nullptr;
   size_t lz_pin_n = 0;
   hipEvent_t lz_ev = nullptr;
+  // rho eigensolve: pinned host copy of the symmetry check's maxima, and the
+  // event after which rho is complete (the auxiliary streams wait on it)
+  double* rdm_pin = nullptr;
+  size_t rdm_pin_n = 0;
+  hipEvent_t rho_ev = nullptr;
   // engines created on this context: destroyed (their device memory
   // returned) before the context's pool goes away
   std::vector<struct nss_dmrg*> engines;

@@ -17,6 +17,8 @@
 
 #include "nss_internal.h"
 
+#include <type_traits>
+
 namespace nss {
 
 template <typename T>
@@ -411,6 +413,305 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// 128x128 output tile per workgroup of eight waves (a 32x64 block each, 2x4
+// MFMA 16x16 blocks).  Operands move from global memory to LDS by the
+// LDS-DMA loads (global_load_lds_dword: lane l's dword lands at the
+// wave-uniform LDS base + 4 l), 16-deep K slices through a ring of G128_NST
+// stages: slice c+2 is in flight while slice c computes, and no register
+// holds operand data on the way.  Per slice every wave issues the same
+// number of DMA instructions, so "slice c has landed" is a vmcnt bound
+// (all but the later slices' instructions done) followed by a workgroup
+// barrier.  Rows/columns past M/N and k past K load clamped (valid)
+// addresses; the A fragments carry the segment's alpha and are zeroed for
+// k past K at use.
+// A tile's K chain (its segments' slices in order) can be cut into nsplit
+// parts, one workgroup each: every part writes its partial tile to part[],
+// and the workgroup that completes the tile (a per-tile arrival counter)
+// sums the partials in part order, so the result does not depend on timing,
+// applies beta and the dot epilogue and resets the counter for the next
+// launch.
+// LDS layouts of a 128x16 slice (2048 elements per operand), chosen per
+// segment by where the operand is contiguous:
+//   RK (contiguous along k: A untransposed, B transposed): element (r, k)
+//      at r*16 + (k ^ sw(r)), sw(r) = ((r >> 1) & 7) << 1;
+//   KR (contiguous along the row index: A transposed, B untransposed):
+//      element (r, k) at k*128 + (r ^ ((k & 1) << 4)).
+// Both keep the 16 lanes of a fragment read (16 rows, one k) and the two
+// 16-lane groups of a half-wave on distinct banks.
+constexpr int G128 = 128, GBK = 16, G128_NW = 8, G128_THREADS = 64 * G128_NW, G128_NST = 3;
+
+template <typename T>
+struct Dma128 {  // DMA instructions per slice: EPI elements each, IPW per wave and operand
+  static constexpr int EPI = 256 / (int)sizeof(T);
+  static constexpr int IPW = G128 * GBK / EPI / G128_NW;
+};
+
+struct View128 {
+  const char* A;
+  const char* B;
+  int64_t ars, aks, brs, bks;  // element (row, k) at row * rs + k * ks (elements)
+  int K;
+  bool ark, brk;  // operand contiguous along k (layout RK)
+  double alpha;
+};
+
+struct SplitK {
+  int nsplit;
+  void* part;     // ntiles * nsplit * 128 * 128 elements (nsplit > 1)
+  unsigned* cnt;  // ntiles arrival counters, zero between launches
+};
+
+__device__ __forceinline__ int sw_rk(int r) { return ((r >> 1) & 7) << 1; }
+
+// offset of element (row, k) of a stored slice
+__device__ __forceinline__ int off128(int r, int k, bool rk) {
+  return rk ? r * 16 + (k ^ sw_rk(r)) : k * 128 + (r ^ ((k & 1) << 4));
+}
+
+// element (row, k) stored at offset o
+__device__ __forceinline__ void inv128(int o, bool rk, int& r, int& k) {
+  const int r_rk = o >> 4, k_kr = o >> 7;
+  r = rk ? r_rk : (o & 127) ^ ((k_kr & 1) << 4);
+  k = rk ? (o & 15) ^ sw_rk(r_rk) : k_kr;
+}
+
+template <typename T>
+__device__ __forceinline__ void view128(const GemmSeg& g, const Bases& bs, View128& v) {
+  v.A = reinterpret_cast<const char*>(resolve_c<T>(g.abase, g.a, g.aoff, bs));
+  v.B = reinterpret_cast<const char*>(resolve_c<T>(g.bbase, g.b, g.boff, bs));
+  const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
+  v.ars = ta ? 1 : g.lda;
+  v.aks = ta ? g.lda : 1;
+  v.brs = tb ? g.ldb : 1;  // B's "row" is the output column j
+  v.bks = tb ? 1 : g.ldb;
+  v.K = g.K;
+  v.ark = !ta;
+  v.brk = tb;
+  v.alpha = g.alpha;
+}
+
+typedef const __attribute__((address_space(1))) void* gvoid_p;
+typedef __attribute__((address_space(3))) void* lvoid_p;
+
+// This wave's DMA instructions of slice [k0, k0+16) into stage (As, Bs).
+template <typename T>
+__device__ __forceinline__ void dma128(const View128& v, const GemmTile& tl, int k0, int wave, int lane, T* As,
+                                       T* Bs) {
+  using D = Dma128<T>;
+  const int sub = sizeof(T) == 8 ? lane >> 1 : lane;        // element slot within the instruction
+  const int half = sizeof(T) == 8 ? (lane & 1) * 4 : 0;     // byte within the element
+#pragma unroll
+  for (int i = 0; i < D::IPW; ++i) {
+    const int q = wave * D::IPW + i;
+    int r, k;
+    inv128(q * D::EPI + sub, v.ark, r, k);
+    const int64_t gi = min(tl.row0 + r, tl.M - 1);
+    const int64_t gk = min(k0 + k, v.K - 1);
+    const char* src = v.A + (gi * v.ars + gk * v.aks) * (int64_t)sizeof(T) + half;
+    __builtin_amdgcn_global_load_lds((gvoid_p)src, (lvoid_p)(As + q * D::EPI), 4, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < D::IPW; ++i) {
+    const int q = wave * D::IPW + i;
+    int j, k;
+    inv128(q * D::EPI + sub, v.brk, j, k);
+    const int64_t gj = min(tl.col0 + j, tl.N - 1);
+    const int64_t gk = min(k0 + k, v.K - 1);
+    const char* src = v.B + (gj * v.brs + gk * v.bks) * (int64_t)sizeof(T) + half;
+    __builtin_amdgcn_global_load_lds((gvoid_p)src, (lvoid_p)(Bs + q * D::EPI), 4, 0, 0);
+  }
+}
+
+// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// s_waitcnt vmcnt(N) lgkmcnt(0): this wave's DMA up to the last N
+// instructions and all of its LDS reads are done
+template <int N>
+__device__ __forceinline__ void wait_vm_lds() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+
+// the wave's 32x64 block of one stage: rows wm + [0, 32), columns wn + [0, 64)
+template <typename T>
+__device__ __forceinline__ void mma128(const T* As, const T* Bs, bool ark, bool brk, T alpha, int kval, int wm,
+                                       int wn, int fr, int fk, typename Mfma<T>::acc_t (&acc)[2][4]) {
+#pragma unroll
+  for (int kk = 0; kk < GBK; kk += 4) {
+    const int k = kk + fk;
+    const T sc = k < kval ? alpha : T(0);
+    T a[2], b[4];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) a[m] = sc * As[off128(wm + 16 * m + fr, k, ark)];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) b[n] = Bs[off128(wn + 16 * n + fr, k, brk)];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) acc[m][n] = Mfma<T>::mma(a[m], b[n], acc[m][n]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void gemm_tile128(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
+                                             int ntiles, const Bases& bs, const SplitK& sk) {
+  using acc_t = typename Mfma<T>::acc_t;
+  using D = Dma128<T>;
+  constexpr int NDMA = 2 * D::IPW;  // DMA instructions per wave and slice
+  // one LDS object per stage and operand (the compiler tells their DMA apart)
+  __shared__ T As0[G128 * GBK], As1[G128 * GBK], As2[G128 * GBK];
+  __shared__ T Bs0[G128 * GBK], Bs1[G128 * GBK], Bs2[G128 * GBK];
+  __shared__ double dred[G128_NW];
+  __shared__ int last_sh;
+  const int item = xcd_tile(ntiles * sk.nsplit);
+  if (item < 0) return;
+  const int t = item / sk.nsplit, part = item - t * sk.nsplit;
+  const GemmTile tl = tiles[t];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  const int fr = lane & 15, fk = lane >> 4;
+  acc_t acc[2][4];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n) acc[m][n] = acc_t{0, 0, 0, 0};
+  // this part's slices [c0, c1) of the chain
+  const int send = tl.seg0 + tl.nseg;
+  int total = 0;
+  for (int q = tl.seg0; q < send; ++q) total += (segs[q].K + GBK - 1) / GBK;
+  const int c0 = (int)((int64_t)total * part / sk.nsplit);
+  const int c1 = (int)((int64_t)total * (part + 1) / sk.nsplit);
+  if (c0 < c1) {
+    // DMA cursor (segment s, slice in within it) at slice c0
+    int s = tl.seg0, in = c0;
+    for (;;) {
+      const int ns = (segs[s].K + GBK - 1) / GBK;
+      if (in < ns) break;
+      in -= ns;
+      ++s;
+    }
+    View128 v;
+    view128<T>(segs[s], bs, v);
+    // per stage: layout, alpha and valid k count of the slice it holds
+    bool sark[G128_NST], sbrk[G128_NST];
+    T salpha[G128_NST];
+    int skval[G128_NST];
+    auto issue = [&](auto stc) {  // the slice at the cursor into stage ST, then advance the cursor
+      constexpr int ST = decltype(stc)::value;
+      T* As = ST == 0 ? As0 : ST == 1 ? As1 : As2;
+      T* Bs = ST == 0 ? Bs0 : ST == 1 ? Bs1 : Bs2;
+      const int k0 = in * GBK;
+      dma128<T>(v, tl, k0, wave, lane, As, Bs);
+      sark[ST] = v.ark;
+      sbrk[ST] = v.brk;
+      salpha[ST] = static_cast<T>(v.alpha);
+      skval[ST] = v.K - k0;
+      ++in;
+      if (in * GBK >= v.K && s + 1 < send) {
+        ++s;
+        in = 0;
+        view128<T>(segs[s], bs, v);
+      }
+    };
+    issue(std::integral_constant<int, 0>{});
+    if (c0 + 1 < c1) issue(std::integral_constant<int, 1>{});
+    // slice c in stage ST: wait for its DMA (slice c+1's may stay in
+    // flight) and for this wave's reads of slice c-1, meet (a bare barrier:
+    // a fenced one would drain slice c+1's DMA too), put slice c+2 in
+    // flight into the stage slice c-1 used, compute
+    auto step = [&](int c, auto stc) {
+      constexpr int ST = decltype(stc)::value;
+      constexpr int NX = (ST + 2) % G128_NST;
+      if (c + 1 < c1) wait_vm_lds<NDMA>();
+      else wait_vm_lds<0>();
+      __builtin_amdgcn_s_barrier();
+      if (c + 2 < c1) issue(std::integral_constant<int, NX>{});
+      const T* As = ST == 0 ? As0 : ST == 1 ? As1 : As2;
+      const T* Bs = ST == 0 ? Bs0 : ST == 1 ? Bs1 : Bs2;
+      mma128<T>(As, Bs, sark[ST], sbrk[ST], salpha[ST], skval[ST], wm, wn, fr, fk, acc);
+    };
+    for (int c = c0; c < c1; c += G128_NST) {
+      step(c, std::integral_constant<int, 0>{});
+      if (c + 1 < c1) step(c + 1, std::integral_constant<int, 1>{});
+      if (c + 2 < c1) step(c + 2, std::integral_constant<int, 2>{});
+    }
+  }
+  if (sk.nsplit > 1) {
+    // partial tile, lane-major per register: part[((t * nsplit + p) * NW + wave) * 2048 + r * 64 + lane]
+    T* P = static_cast<T*>(sk.part);
+    T* mine = P + ((int64_t)(t * sk.nsplit + part) * G128_NW + wave) * 2048;
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mine[((m * 4 + n) * 4 + q) * 64 + lane] = acc[m][n][q];
+    // every wave's stores done, the workgroup meets, one thread releases them
+    // at agent scope with its arrival; the last arrival acquires
+    wait_vm<0>();
+    __syncthreads();
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const bool last = __hip_atomic_fetch_add(sk.cnt + t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                        (unsigned)(sk.nsplit - 1);
+      if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      last_sh = last;
+    }
+    __syncthreads();
+    if (!last_sh) return;
+    // the sum in part order
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        acc_t sum = acc_t{0, 0, 0, 0};
+        for (int p = 0; p < sk.nsplit; ++p) {
+          const T* src = P + ((int64_t)(t * sk.nsplit + p) * G128_NW + wave) * 2048 + (m * 4 + n) * 256 + lane;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const T x = p == part ? acc[m][n][q] : src[q * 64];
+            sum[q] = p == 0 ? x : sum[q] + x;
+          }
+        }
+        acc[m][n] = sum;
+      }
+    if (tid == 0) sk.cnt[t] = 0u;
+  }
+  T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
+  const T* X = static_cast<const T*>(bs.x) + tl.coff;  // read only with bs.dot (Y and x share offsets)
+  double dsum = 0.0;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // accumulator item q of lane l holds D[row][l%16] with
+        // row = l/16 + 4q (f64 16x16x4) or 4*(l/16) + q (f32 16x16x4)
+        const int row = tl.row0 + wm + m * 16 + (sizeof(T) == 8 ? fk + 4 * q : fk * 4 + q);
+        const int col = tl.col0 + wn + n * 16 + fr;
+        if (row < tl.M && col < tl.N) {
+          T* p = C + (size_t)row * tl.ldc + col;
+          const T out = tl.beta ? *p + acc[m][n][q] : acc[m][n][q];
+          *p = out;
+          if (bs.dot) dsum += (double)out * (double)X[(size_t)row * tl.ldc + col];
+        }
+      }
+  if (bs.dot) {  // four partials of x^T y per tile (waves 2q and 2q+1 to slot q), fixed order
+    const double rr = wave_sum(dsum);
+    if (lane == 0) dred[wave] = rr;
+    __syncthreads();
+    if (tid < 4) bs.dot[4 * t + tid] = dred[2 * tid] + dred[2 * tid + 1];
+  }
+}
+
 // One kernel symbol per use so that traces separate them: the two H*v
 // phases (W: Psi.B^T per term, Y: per-sector accumulation) and the generic
 // grouped GEMM (rho, rotations); the _t64 symbols are the 64x64 LDS-staged
@@ -419,6 +720,11 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,                       \
                                               const GemmSeg* __restrict__ segs, int ntiles, Bases bs) { \
     gemm_tile32<T, NSS_NBUF, SKIPLOAD>(tiles, segs, ntiles, bs);                                        \
+  }                                                                                                     \
+  __global__ __launch_bounds__(G128_THREADS) void name##_t128(const GemmTile* __restrict__ tiles,       \
+                                                        const GemmSeg* __restrict__ segs, int ntiles,   \
+                                                        Bases bs, SplitK sk) {                          \
+    gemm_tile128<T>(tiles, segs, ntiles, bs, sk);                                                       \
   }                                                                                                     \
   __global__ __launch_bounds__(256) void name##_t64(const GemmTile* __restrict__ tiles,                 \
                                                     const GemmSeg* __restrict__ segs, int ntiles,       \
@@ -436,13 +742,23 @@ NSS_GEMM_KERNEL(k_hv_w_f32, float, true)
 NSS_GEMM_KERNEL(k_hv_y_f32, float, false)
 
 typedef void (*gemm_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases);
+typedef void (*gemm128_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases, SplitK);
 
 // With start/stop events the launch records them itself (the dispatch's own
 // begin and end timestamps, hipExtLaunchKernelGGL): no marker packets
 // between the kernels of an iteration.
 static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t k32, gemm_kernel_t k64,
-                        hipEvent_t ev0, hipEvent_t ev1) {
+                        gemm128_kernel_t k128, hipEvent_t ev0, hipEvent_t ev1) {
   const int n = (int)p.htiles.size();
+  if (p.tile == 128) {
+    const SplitK sk{p.nsplit, p.nsplit > 1 ? p.part.p : nullptr, p.nsplit > 1 ? p.cnt.p : nullptr};
+    const unsigned grid = 8u * (unsigned)((n * p.nsplit + 7) / 8);
+    if (ev0)
+      hipExtLaunchKernelGGL(k128, dim3(grid), dim3(G128_THREADS), 0, s, ev0, ev1, 0, p.tiles.p, p.segs.p, n, b, sk);
+    else hipLaunchKernelGGL(k128, dim3(grid), dim3(G128_THREADS), 0, s, p.tiles.p, p.segs.p, n, b, sk);
+    NSS_HIP(hipGetLastError());
+    return;
+  }
   const unsigned grid = 8u * (unsigned)((n + 7) / 8);
   gemm_kernel_t k = p.tile == 64 ? k64 : k32;
   if (ev0) hipExtLaunchKernelGGL(k, dim3(grid), dim3(256), 0, s, ev0, ev1, 0, p.tiles.p, p.segs.p, n, b);
@@ -452,15 +768,15 @@ static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t
 
 void launch_gemm_f64(const GemmPlan& p, Bases b, hipStream_t s, int kind, hipEvent_t ev0, hipEvent_t ev1) {
   if (p.htiles.empty()) return;
-  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f64, k_hv_w_f64_t64, ev0, ev1);
-  else if (kind == GEMM_HV_Y) launch_plan(p, b, s, k_hv_y_f64, k_hv_y_f64_t64, ev0, ev1);
-  else launch_plan(p, b, s, k_gemm_f64, k_gemm_f64_t64, ev0, ev1);
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f64, k_hv_w_f64_t64, k_hv_w_f64_t128, ev0, ev1);
+  else if (kind == GEMM_HV_Y) launch_plan(p, b, s, k_hv_y_f64, k_hv_y_f64_t64, k_hv_y_f64_t128, ev0, ev1);
+  else launch_plan(p, b, s, k_gemm_f64, k_gemm_f64_t64, k_gemm_f64_t128, ev0, ev1);
 }
 
 void launch_gemm_f32(const GemmPlan& p, Bases b, hipStream_t s, int kind, hipEvent_t ev0, hipEvent_t ev1) {
   if (p.htiles.empty()) return;
-  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f32, k_hv_w_f32_t64, ev0, ev1);
-  else launch_plan(p, b, s, k_hv_y_f32, k_hv_y_f32_t64, ev0, ev1);
+  if (kind == GEMM_HV_W) launch_plan(p, b, s, k_hv_w_f32, k_hv_w_f32_t64, k_hv_w_f32_t128, ev0, ev1);
+  else launch_plan(p, b, s, k_hv_y_f32, k_hv_y_f32_t64, k_hv_y_f32_t128, ev0, ev1);
 }
 
 // ------------------------------------------------------------ vectors
@@ -1865,30 +2181,60 @@ void launch_unpad_eig(const double* vecs, const double* vals, int ncls, const in
 }
 
 // ------------------------------------------------------------ symmetry check
+// max |a_ij - a_ji| of every block: a 32x32 tile pair (I, J), I <= J, per
+// workgroup (grid x: tile pairs of the largest block, y: block), both tiles
+// read row-wise through LDS; the block's maximum by an unsigned max on the
+// bit pattern (non-negative doubles order like their bits; NaN as +inf).
+// out[] must enter zeroed.
+constexpr int ASYM_T = 32;
+
 __global__ __launch_bounds__(256) void k_asym_max(const double* __restrict__ a, const int64_t* __restrict__ off,
-                                                  const int32_t* __restrict__ dim, double* __restrict__ out) {
-  __shared__ double sh[256];
-  const int b = blockIdx.x;
+                                                  const int32_t* __restrict__ dim,
+                                                  unsigned long long* __restrict__ out) {
+  __shared__ double tj[ASYM_T][ASYM_T + 1];
+  __shared__ double red[4];
+  const int b = blockIdx.y;
   const int d = dim[b];
+  const int nt = (d + ASYM_T - 1) / ASYM_T;
+  // pair index p -> (I, J), I <= J, row-major over the upper triangle
+  int p = blockIdx.x, I = 0;
+  while (I < nt && p >= nt - I) {
+    p -= nt - I;
+    ++I;
+  }
+  if (I >= nt) return;  // workgroup-uniform
+  const int J = I + p;
   const double* m = a + off[b];
-  double mx = 0.0;
-  for (int64_t e = threadIdx.x; e < (int64_t)d * d; e += 256) {
-    const int i = (int)(e / d), j = (int)(e % d);
-    if (j > i) mx = fmax(mx, fabs(m[(size_t)i * d + j] - m[(size_t)j * d + i]));
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+  for (int r = ty; r < ASYM_T; r += 8) {  // tile (J, I): row J*32 + r, column I*32 + tx
+    const int gi = J * ASYM_T + r, gj = I * ASYM_T + tx;
+    tj[r][tx] = (gi < d && gj < d) ? m[(size_t)gi * d + gj] : 0.0;
   }
-  sh[threadIdx.x] = mx;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
-    __syncthreads();
+  double mx = 0.0;
+  for (int r = ty; r < ASYM_T; r += 8) {  // a(I*32 + r, J*32 + tx) against a(J*32 + tx, I*32 + r)
+    const int gi = I * ASYM_T + r, gj = J * ASYM_T + tx;
+    if (gi < d && gj < d && gj > gi) {
+      const double x = fabs(m[(size_t)gi * d + gj] - tj[tx][r]);
+      mx = (x == x) ? fmax(mx, x) : HUGE_VAL;
+    }
   }
-  if (threadIdx.x == 0) out[b] = sh[0];
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (r > 0.0) atomicMax(out + b, (unsigned long long)__double_as_longlong(r));
+  }
 }
 
-void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, double* out,
+void launch_asym_max(const double* a, const int64_t* off, const int32_t* d, int nblk, int dmax, double* out,
                      hipStream_t s) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_asym_max, dim3(nblk), dim3(256), 0, s, a, off, d, out);
+  NSS_HIP(hipMemsetAsync(out, 0, (size_t)nblk * sizeof(double), s));
+  const int nt = (dmax + ASYM_T - 1) / ASYM_T;
+  hipLaunchKernelGGL(k_asym_max, dim3((unsigned)(nt * (nt + 1) / 2), (unsigned)nblk), dim3(256), 0, s, a, off, d,
+                     reinterpret_cast<unsigned long long*>(out));
   NSS_HIP(hipGetLastError());
 }
 

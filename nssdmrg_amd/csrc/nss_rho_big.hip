@@ -219,6 +219,276 @@ void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t
   NSS_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------ 1'. tridiagonal, LDS-resident
+// The same reduction with the trailing matrix held on chip: the G workgroups
+// of a block own its rows cyclically (row i in workgroup i mod G, slot
+// i / G of its LDS), one launch covers every column, and a per-block
+// counter barrier between columns replaces the kernel boundary.  Per column
+// j a workgroup reads only p_{j-1} (rows j..n-1) and row j (published by its
+// owner in the previous column, after the update of column j-2), recomputes
+// w_{j-1}, row j after the update of column j-1 and v_j redundantly (every
+// workgroup runs the same arithmetic in the same order, so their copies agree
+// bit for bit), updates its own rows in LDS and publishes its share of p_j.
+// Reflector j-1 goes to row j-1 of the block in global memory, as in the
+// per-column kernel.  The launch is cooperative (every workgroup resident);
+// a barrier wait longer than TL_SPIN_MAX polls raises the abort flag, after
+// which every workgroup leaves.
+constexpr int TL_THREADS = 512;
+constexpr int TL_WAVES = TL_THREADS / 64;
+constexpr unsigned TL_SPIN_MAX = 1u << 22;
+
+// Every wave waits for its own stores, the workgroup meets, and one thread
+// releases them at agent scope with the counter increment; it polls the
+// counter with relaxed loads (an acquiring poll would invalidate the caches
+// on every round) and acquires once when the count is reached.
+__device__ __forceinline__ bool tl_barrier(unsigned* cnt, unsigned target, int* abort, int* flag) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    unsigned spins = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        ok = 0;
+        break;
+      }
+      if (++spins > TL_SPIN_MAX) {
+        __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *flag = ok;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+__global__ __launch_bounds__(TL_THREADS) void k_tred_lds(const TredBlk* __restrict__ blks,
+                                                         const int2* __restrict__ wgmap, const int* __restrict__ gcnt,
+                                                         unsigned* __restrict__ bar, int* __restrict__ abort) {
+  extern __shared__ double sm[];  // rows[R][n], then vbuf[2][n], wp[n]
+  __shared__ double red[TL_WAVES];
+  __shared__ int flag;
+  const int2 mp = wgmap[blockIdx.x];
+  if (mp.x < 0) return;
+  const int b = mp.x, g = mp.y, G = gcnt[b];
+  const TredBlk B = blks[b];
+  const int n = B.n;
+  const int R = (n - g + G - 1) / G;  // rows g, g + G, ...
+  double* rows = sm;
+  double* vb0 = sm + (size_t)R * n;
+  double* vb1 = vb0 + n;
+  double* wp = vb1 + n;
+  double* A = B.a;
+  double* ws = B.ws;
+  double* rowbuf = ws;        // [2][n]: row j+1 after the update of column j-1
+  double* pbuf = ws + 2 * n;  // [2][n]
+  double* dd = ws + 4 * n;
+  double* ee = ws + 5 * n;
+  double* tau = ws + 6 * n;
+  unsigned* cnt = bar + b;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int s = 0; s < R; ++s) {
+    const double* src = A + (int64_t)(g + s * G) * n;
+    for (int c = tid; c < n; c += TL_THREADS) rows[(size_t)s * n + c] = src[c];
+  }
+  for (int c = tid; c < n; c += TL_THREADS) vb0[c] = vb1[c] = wp[c] = 0.0;
+  __syncthreads();
+  double taup = 0.0;
+  for (int j = 0; j < n; ++j) {
+    double* vp = (j & 1) ? vb0 : vb1;  // v_{j-1} at its columns j..n-1 (zero for j = 0)
+    double* vj = (j & 1) ? vb1 : vb0;
+    // w_{j-1} = p_{j-1} - (tau/2)(p.v) v over rows j..n-1
+    if (j > 0) {
+      const double* pq = pbuf + ((j - 1) & 1) * n;
+      double pv = 0.0;
+      for (int c = j + tid; c < n; c += TL_THREADS) {
+        const double p = pq[c];
+        wp[c] = p;
+        pv += p * vp[c];
+      }
+      pv = bsum<TL_WAVES>(pv, red);
+      const double K = 0.5 * taup * pv;
+      for (int c = j + tid; c < n; c += TL_THREADS) wp[c] -= K * vp[c];
+      __syncthreads();
+    }
+    // row j after the update of column j-1, into vj[j..n)
+    const double v0 = vp[j], w0 = wp[j];
+    const double* rj = j == 0 ? A : rowbuf + (j & 1) * n;
+    double s2 = 0.0;
+    for (int c = j + tid; c < n; c += TL_THREADS) {
+      const double r = rj[c] - v0 * wp[c] - w0 * vp[c];
+      vj[c] = r;
+      if (c >= j + 2) s2 += r * r;
+    }
+    s2 = bsum<TL_WAVES>(s2, red);
+    const double dj = vj[j];
+    const double alpha = j + 1 < n ? vj[j + 1] : 0.0;
+    double tj = 0.0, beta = alpha, scale = 0.0;
+    if (s2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    __syncthreads();  // every thread has read vj[j], vj[j + 1]
+    for (int c = j + 1 + tid; c < n; c += TL_THREADS) vj[c] = c == j + 1 ? 1.0 : vj[c] * scale;
+    __syncthreads();
+    if (g == 0) {
+      if (tid == 0) {
+        dd[j] = dj;
+        ee[j] = j + 1 < n ? beta : 0.0;
+        tau[j] = tj;
+      }
+      if (j > 0) {  // reflector j-1 to its row (columns j..n-1)
+        double* keep = A + (int64_t)(j - 1) * n;
+        for (int c = j + tid; c < n; c += TL_THREADS) keep[c] = vp[c];
+      }
+    }
+    // owned rows i > j: the update of column j-1, then p_j[i] = tau_j A_i v_j;
+    // row j+1 also to rowbuf for the next column
+    double* po = pbuf + (j & 1) * n;
+    double* rn = rowbuf + ((j + 1) & 1) * n;
+    const int s0 = j + 1 > g ? (j + 1 - g + G - 1) / G : 0;  // first slot with row > j
+    for (int s = s0 + wv; s < R; s += TL_WAVES) {
+      const int i = g + s * G;
+      const double vi = vp[i], wi = wp[i];
+      double* row = rows + (size_t)s * n;
+      double acc = 0.0;
+      const bool pub = i == j + 1;
+#pragma unroll 4
+      for (int c = j + 1 + lane; c < n; c += 64) {
+        const double a = row[c] - vi * wp[c] - wi * vp[c];
+        row[c] = a;
+        acc += a * vj[c];
+        if (pub) rn[c] = a;
+      }
+      acc = wsum(acc);
+      if (lane == 0) po[i] = tj * acc;
+    }
+    taup = tj;
+    if (!tl_barrier(cnt, (unsigned)(j + 1) * (unsigned)G, abort, &flag)) return;
+  }
+}
+
+namespace {
+int tl_lds_bytes(int device) {
+  static int cached = -1;
+  if (cached >= 0) return cached;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
+  const int dyn = lds - 1024;  // the kernel's static LDS is under 100 bytes
+  if (hipFuncSetAttribute((const void*)k_tred_lds, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) {
+    (void)hipGetLastError();
+    cached = 0;
+  } else {
+    cached = dyn;
+  }
+  return cached;
+}
+}  // namespace
+
+bool tred_lds_plan(int device, const std::vector<int>& dims, TredLdsPlan* plan) {
+  plan->wgmap.clear();
+  plan->gcount.assign(dims.size(), 0);
+  plan->covered.assign(dims.size(), 0);
+  plan->grid = 0;
+  const int lds = tl_lds_bytes(device);
+  if (lds <= 0) return false;
+  int ncu = 0, occ = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+    return false;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_tred_lds, TL_THREADS, (size_t)lds) !=
+          hipSuccess ||
+      occ <= 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  constexpr int NX = 8;  // workgroup k runs on XCD k mod 8: a block's workgroups share one L2 where they fit
+  const int per_x = ncu * occ / NX;
+  if (per_x <= 0) return false;
+  const int cap = per_x * NX;
+  // workgroups per block: the fewest whose rows fit in LDS
+  std::vector<int> need(dims.size(), 0);
+  for (size_t b = 0; b < dims.size(); ++b) {
+    const int64_t n = dims[b];
+    const int64_t room = lds / (int64_t)sizeof(double) - 3 * n;  // doubles left for rows
+    const int64_t rmax = room / n;
+    if (rmax < 1) continue;
+    need[b] = (int)((n + rmax - 1) / rmax);
+  }
+  // largest blocks first, while the launch stays co-resident
+  std::vector<int> ord(dims.size());
+  for (size_t b = 0; b < ord.size(); ++b) ord[b] = (int)b;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return dims[x] > dims[y]; });
+  int used = 0;
+  for (int b : ord)
+    if (need[b] > 0 && used + need[b] <= cap) {
+      plan->gcount[b] = need[b];
+      plan->covered[b] = 1;
+      used += need[b];
+    }
+  if (used == 0) return false;
+  // spare workgroups to the blocks with the most rows per workgroup
+  for (;;) {
+    int best = -1;
+    int64_t bw = 0;
+    for (int b : ord)
+      if (plan->covered[b]) {
+        const int64_t R = (dims[b] + plan->gcount[b] - 1) / plan->gcount[b];
+        if (R > 8 && R * dims[b] > bw) {
+          bw = R * dims[b];
+          best = b;
+        }
+      }
+    if (best < 0 || used >= cap) break;
+    ++plan->gcount[best];
+    ++used;
+  }
+  // workgroup ids: XCD x holds ids x, x + 8, ...; a block goes to the XCD
+  // with the most free slots, spilling over the next ones if it needs more
+  std::vector<std::vector<int2>> bins(NX);
+  std::vector<int> ordg(ord);
+  std::stable_sort(ordg.begin(), ordg.end(), [&](int x, int y) { return plan->gcount[x] > plan->gcount[y]; });
+  for (int b : ordg) {
+    if (!plan->covered[b]) continue;
+    int left = plan->gcount[b], gi = 0;
+    while (left > 0) {
+      int x = 0;
+      for (int k = 1; k < NX; ++k)
+        if (bins[k].size() < bins[x].size()) x = k;
+      const int take = std::min(left, per_x - (int)bins[x].size());
+      if (take <= 0) return false;  // cannot happen: used <= cap
+      for (int t = 0; t < take; ++t) bins[x].push_back(make_int2(b, gi++));
+      left -= take;
+    }
+  }
+  size_t slots = 0;
+  for (auto& v : bins) slots = std::max(slots, v.size());
+  plan->grid = (int)(slots * NX);
+  plan->wgmap.assign(plan->grid, make_int2(-1, 0));
+  for (int x = 0; x < NX; ++x)
+    for (size_t t = 0; t < bins[x].size(); ++t) plan->wgmap[t * NX + x] = bins[x][t];
+  plan->lds = (size_t)lds;
+  return true;
+}
+
+int launch_tred_lds(const TredBlk* blks, const int2* wgmap, const int* gcnt, unsigned* bar, int* abort, int grid,
+                    size_t lds, hipStream_t s) {
+  void* args[] = {(void*)&blks, (void*)&wgmap, (void*)&gcnt, (void*)&bar, (void*)&abort};
+  const hipError_t e = hipLaunchCooperativeKernel((const void*)k_tred_lds, dim3(grid), dim3(TL_THREADS), args,
+                                                  (unsigned)lds, s);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return 1;
+}
+
 // ------------------------------------------------------------ 2. eigenvalues
 // Position p of block b belongs to the unreduced block [s, e) of T that holds
 // row p and takes eigenvalue p - s of that block (ascending within it).

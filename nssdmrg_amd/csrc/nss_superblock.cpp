@@ -5,6 +5,7 @@
 //   Y phase: Y_q = H_L,q Psi_q + Psi_q H_R,q^T + sum_{t->q} A_t W_t
 // which is the reference's evaluation order (B first, then A).
 #include <algorithm>
+#include <functional>
 #include <cfloat>
 #include <chrono>
 #include <cstdlib>
@@ -91,13 +92,18 @@ DevicePool::~DevicePool() {
 }
 
 // ---------------------------------------------------------------- plans
+// Read at every plan upload (tests switch them per case).
 int gemm_default_tile() {
-  static const int t = [] {
-    const char* e = std::getenv("NSS_GEMM_TILE");
-    const int v = e ? std::atoi(e) : 0;
-    return (v == 32 || v == 64) ? v : 0;
-  }();
-  return t;
+  const char* e = std::getenv("NSS_GEMM_TILE");
+  const int v = e ? std::atoi(e) : 0;
+  return (v == 32 || v == 64 || v == 128) ? v : 0;
+}
+
+// NSS_GEMM_SPLIT = 1..16 forces the K split of 128x128 launches.
+static int gemm_forced_split() {
+  const char* e = std::getenv("NSS_GEMM_SPLIT");
+  const int v = e ? std::atoi(e) : 0;
+  return (v >= 1 && v <= 16) ? v : 0;
 }
 
 void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
@@ -130,10 +136,40 @@ void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int 
 // ones the split K chain of the 32x32 kernel does.
 static constexpr int64_t GEMM_T64_MIN_TILES = 2048;
 
+// The 128x128 kernel runs only when NSS_GEMM_TILE=128 asks for it: on the
+// FP64 H*v launches of the spin-1 m=1024 workload it measured slower than
+// the 32x32 / 64x64 pair (Y 493 vs 347 us, W 281 vs 187 us), and on plain
+// FP64 GEMMs no faster than 64x64 (37.5 vs 41.5 TFLOP/s at 4096x4096x2048);
+// in FP32 it leads on large products (72 vs 62 TFLOP/s).  Its K chains are
+// split until about two workgroups per CU are busy.
+static constexpr int GEMM_T128_TARGET_WG = 512;
+
 void GemmPlan::upload(DevicePool* pool, hipStream_t s) {
   int64_t n32 = 0;
   for (const GemmTile& o : outs) n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
   tile = gemm_default_tile() ? gemm_default_tile() : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
+  nsplit = 1;
+  if (tile == 128) {
+    int64_t n128 = 0, slices = 0;
+    for (const GemmTile& o : outs) {
+      const int64_t nt = (int64_t)((o.M + 127) / 128) * ((o.N + 127) / 128);
+      int64_t sl = 0;
+      for (int q = o.seg0; q < o.seg0 + o.nseg; ++q) sl += (hsegs[q].K + 15) / 16;
+      n128 += nt;
+      slices += nt * sl;
+    }
+    if (n128 > 0) {
+      const int64_t want = (GEMM_T128_TARGET_WG + n128 - 1) / n128;
+      const int64_t cap = std::max<int64_t>(1, slices / n128 / 4);  // at least 4 slices per part on average
+      nsplit = gemm_forced_split() ? gemm_forced_split()
+                                   : (int)std::max<int64_t>(1, std::min<int64_t>({want, cap, 8}));
+      if (nsplit > 1) {
+        part.resize(pool, (size_t)n128 * nsplit * 16384);
+        cnt.resize(pool, (size_t)n128);
+        NSS_HIP(hipMemsetAsync(cnt.p, 0, (size_t)n128 * sizeof(unsigned), s));
+      }
+    }
+  }
   htiles.clear();
   for (const GemmTile& o : outs)
     for (int r = 0; r < o.M; r += tile)
@@ -474,6 +510,11 @@ void nss_release_rocblas(nss_ctx* ctx) {
   if (ctx->lz_pin) (void)hipHostFree(ctx->lz_pin);
   ctx->lz_pin = nullptr;
   ctx->lz_pin_n = 0;
+  if (ctx->rho_ev) (void)hipEventDestroy(ctx->rho_ev);
+  ctx->rho_ev = nullptr;
+  if (ctx->rdm_pin) (void)hipHostFree(ctx->rdm_pin);
+  ctx->rdm_pin = nullptr;
+  ctx->rdm_pin_n = 0;
   if (ctx->rocblas) {
     (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->rocblas));
     ctx->rocblas = nullptr;
@@ -547,6 +588,16 @@ static int rho_class(int d, int dlarge) {
 constexpr int64_t RHO_SLAB_BYTES = int64_t(1) << 30;
 
 // Auxiliary stream k with its rocblas handle bound to it (created on first use).
+// NSS_TRED_LDS=1 sends the large blocks that fit the chip's LDS to the
+// one-launch LDS-resident tridiagonalisation; by default every large block
+// takes the per-column kernel, which measured faster on the spin-1 m=1024
+// workload (about 8 us per column each; the LDS path's cross-XCD barrier
+// costs more than a kernel boundary).  Read on every call.
+static bool tred_lds_enabled() {
+  const char* e = std::getenv("NSS_TRED_LDS");
+  return e && std::atoi(e) == 1;
+}
+
 static rocblas_handle aux_handle(nss_ctx* ctx, int k, hipStream_t* st) {
   if (!ctx->aux[k]) NSS_HIP(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
   if (!ctx->aux_rocblas[k]) {
@@ -743,32 +794,47 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   R.upload(ctx->pool, ctx->stream);
   Bases b{nullptr, nullptr, rho.p};
   launch_gemm_f64(R, b, ctx->stream, GEMM_GENERIC);
-  // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the device
+  // build_density_matrix's symmetry invariant (DMRG_RDM.f90:159-160), on the
+  // device; its maxima reach pinned host memory behind the eigensolves and
+  // are checked before any result of this call is used
   pt.next("rdm.symcheck");
   std::vector<int64_t> hoff;
   std::vector<int32_t> hdim;
+  int dchk = 0;
   for (const Item& it : items)
     if (it.d > 1) {
       hoff.push_back(it.roff);
       hdim.push_back(it.d);
+      dchk = std::max(dchk, it.d);
     }
   const int nchk = (int)hoff.size();
-  std::vector<double> asym(std::max(1, nchk), 0.0);
   DevArr<int64_t> doff;
   DevArr<int32_t> ddim;
   DevArr<double> dasym;
+  if (ctx->rdm_pin_n < (size_t)std::max(1, nchk)) {
+    if (ctx->rdm_pin) NSS_HIP(hipHostFree(ctx->rdm_pin));
+    ctx->rdm_pin = nullptr;
+    ctx->rdm_pin_n = 0;
+    const size_t n = (size_t)std::max(64, 2 * nchk);
+    NSS_HIP(hipHostMalloc((void**)&ctx->rdm_pin, n * sizeof(double), hipHostMallocDefault));
+    ctx->rdm_pin_n = n;
+  }
+  if (!ctx->rho_ev) NSS_HIP(hipEventCreateWithFlags(&ctx->rho_ev, hipEventDisableTiming));
   if (nchk) {
     doff.resize(ctx->pool, nchk);
     ddim.resize(ctx->pool, nchk);
     dasym.resize(ctx->pool, nchk);
     NSS_HIP(hipMemcpyAsync(doff.p, hoff.data(), nchk * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     NSS_HIP(hipMemcpyAsync(ddim.p, hdim.data(), nchk * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    launch_asym_max(rho.p, doff.p, ddim.p, nchk, dasym.p, ctx->stream);
-    NSS_HIP(hipMemcpyAsync(asym.data(), dasym.p, nchk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    launch_asym_max(rho.p, doff.p, ddim.p, nchk, dchk, dasym.p, ctx->stream);
+    NSS_HIP(hipMemcpyAsync(ctx->rdm_pin, dasym.p, nchk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   }
-  NSS_HIP(hipStreamSynchronize(ctx->stream));
-  for (int c = 0; c < nchk; ++c)
-    if (!(asym[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
+  NSS_HIP(hipEventRecord(ctx->rho_ev, ctx->stream));
+  // the check itself, once the context stream has passed the copy
+  auto check_symmetric = [&]() {
+    for (int c = 0; c < nchk; ++c)
+      if (!(ctx->rdm_pin[c] <= 1e-12)) throw Error(NSS_E_NUMERIC, "build_density_matrix error: rho not Symmetric");
+  };
   // eigen-decompositions: blocks up to the LDS limit (the 1x1 and tiny
   // ones included) with the batched GPU Jacobi kernel, larger ones with
   // rocSOLVER dsyevd; device results overwrite rho in place
@@ -820,8 +886,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   }
   // blocks above the Jacobi limit up to tmax: the tridiagonal-path kernel on
   // the last auxiliary stream, concurrently with the Jacobi launch and the
-  // rocSOLVER classes (rho is complete: the symmetry check above
-  // synchronised the context stream)
+  // rocSOLVER classes (after rho_ev)
   // every auxiliary stream used here is drained before this function
   // returns or throws: the pool does not track streams, so buffers freed on
   // the way out must not be in use there
@@ -841,6 +906,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     const int nb = (int)tdim.size();
     hipStream_t st = nullptr;
     (void)aux_handle(ctx, nss_ctx::AUX_TRI, &st);
+    NSS_HIP(hipStreamWaitEvent(st, ctx->rho_ev, 0));
     used_aux[nss_ctx::AUX_TRI] = true;
     dtoff.resize(ctx->pool, nb);
     dteoff.resize(ctx->pool, nb);
@@ -854,9 +920,9 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   // blocks above the Jacobi limit: rocSOLVER dsyevd, strided-batched per
   // size class (dims rounded up to a multiple of 64, padded slabs).  The
   // classes run on the auxiliary streams, concurrently with the Jacobi
-  // launch on the context stream (rho is complete: the symmetry check above
-  // synchronised the context stream; every batch writes disjoint rows of
-  // rho and of the eigenvalue array).
+  // launch on the context stream (each auxiliary stream waits on rho_ev,
+  // after which rho is complete; every batch writes disjoint rows of rho and
+  // of the eigenvalue array).
   // blocks above tmax up to bmax: the large-block path on its own auxiliary
   // stream (Householder tridiagonalisation, one launch per column over all of
   // them, then every eigenvalue); their eigenvectors come later
@@ -864,13 +930,17 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   for (int i = 0; i < ni; ++i)
     if (items[i].kind == 4) bigs.push_back(i);
   auto def = std::make_shared<RhoDeferred>();
-  DevArr<TredBlk> dtb;
+  DevArr<TredBlk> dtb, dtbr;
+  DevArr<int2> dtlm;
+  DevArr<int> dtlg;
+  std::function<bool()> big_abort;
   DevArr<int64_t> dbeo;
   std::vector<int64_t> hbeo;
   if (!bigs.empty()) {
     pt.next("rdm.big");
     hipStream_t st = nullptr;
     (void)aux_handle(ctx, nss_ctx::AUX_BIG, &st);
+    NSS_HIP(hipStreamWaitEvent(st, ctx->rho_ev, 0));
     used_aux[nss_ctx::AUX_BIG] = true;
     const int nb = (int)bigs.size();
     int64_t wtot = 0;
@@ -880,7 +950,12 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
       wtot += tred_ws_size(items[i].d);
       nbmax = std::max(nbmax, items[i].d);
     }
-    def->ws.resize(ctx->pool, wtot);
+    // after the blocks' workspaces: the LDS path's barrier counters and
+    // abort flag (unsigned, zeroed), inside ws so that hws carries the flag
+    const int64_t nbar_d = (nb + 2) / 2;
+    def->ws.resize(ctx->pool, wtot + nbar_d);
+    unsigned* dbar = reinterpret_cast<unsigned*>(def->ws.p + wtot);
+    NSS_HIP(hipMemsetAsync(dbar, 0, nbar_d * sizeof(double), st));
     for (int k = 0; k < nb; ++k) {
       const Item& it = items[bigs[k]];
       TredBlk t{};
@@ -894,10 +969,40 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
     dbeo.resize(ctx->pool, nb);
     NSS_HIP(hipMemcpyAsync(dtb.p, def->blk.data(), nb * sizeof(TredBlk), hipMemcpyHostToDevice, st));
     NSS_HIP(hipMemcpyAsync(dbeo.p, hbeo.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    for (int j = 0; j < nbmax; ++j) launch_tred_col(dtb.p, nb, j, nbmax, st);
+    // tridiagonalisation: one cooperative launch for the blocks whose
+    // trailing matrices fit the chip's LDS, one launch per column for the rest
+    std::vector<int> bdims(nb);
+    for (int k = 0; k < nb; ++k) bdims[k] = def->blk[k].n;
+    TredLdsPlan plan;
+    bool lds_ok = tred_lds_enabled() && tred_lds_plan(ctx->device, bdims, &plan);
+    if (lds_ok) {
+      dtlm.resize(ctx->pool, plan.grid);
+      dtlg.resize(ctx->pool, nb);
+      NSS_HIP(hipMemcpyAsync(dtlm.p, plan.wgmap.data(), plan.grid * sizeof(int2), hipMemcpyHostToDevice, st));
+      NSS_HIP(hipMemcpyAsync(dtlg.p, plan.gcount.data(), nb * sizeof(int), hipMemcpyHostToDevice, st));
+      lds_ok = launch_tred_lds(dtb.p, dtlm.p, dtlg.p, dbar, reinterpret_cast<int*>(dbar + nb), plan.grid, plan.lds,
+                               st) != 0;
+      if (ctx->prof) ctx->timers["rdm.count_tred_lds"] += lds_ok;
+    }
+    std::vector<TredBlk> rest;
+    int rmax = 0;
+    for (int k = 0; k < nb; ++k)
+      if (!lds_ok || !plan.covered[k]) {
+        rest.push_back(def->blk[k]);
+        rmax = std::max(rmax, def->blk[k].n);
+      }
+    if (!rest.empty()) {
+      dtbr.resize(ctx->pool, rest.size());
+      NSS_HIP(hipMemcpyAsync(dtbr.p, rest.data(), rest.size() * sizeof(TredBlk), hipMemcpyHostToDevice, st));
+      for (int j = 0; j < rmax; ++j) launch_tred_col(dtbr.p, (int)rest.size(), j, rmax, st);
+    }
     launch_tri_eigvals(dtb.p, dbeo.p, dD.p, nb, nbmax, st);
-    def->hws.resize(wtot);
-    NSS_HIP(hipMemcpyAsync(def->hws.data(), def->ws.p, wtot * sizeof(double), hipMemcpyDeviceToHost, st));
+    def->hws.resize(wtot + nbar_d);
+    NSS_HIP(hipMemcpyAsync(def->hws.data(), def->ws.p, (wtot + nbar_d) * sizeof(double), hipMemcpyDeviceToHost,
+                           st));
+    big_abort = [def, wtot, nb]() {
+      return reinterpret_cast<const unsigned*>(def->hws.data() + wtot)[nb] != 0;
+    };
   }
   pt.next("rdm.rocsolver");
   std::map<int, std::vector<int>> classes;
@@ -937,6 +1042,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
       ClassBatch& B = batches[c];
       const int k = (int)(c % nss_ctx::AUX_TRI);
       rocblas_handle hnd = aux_handle(ctx, k, &B.st);
+      NSS_HIP(hipStreamWaitEvent(B.st, ctx->rho_ev, 0));
       used_aux[k] = true;
       ++c;
       B.ncls = kv.first;
@@ -979,6 +1085,10 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   for (int k = 0; k < nss_ctx::NAUX; ++k)
     if (used_aux[k]) NSS_HIP(hipStreamSynchronize(ctx->aux[k]));
   if (ctx->prof) ctx->timers["rdm.rocsolver_wait"] += now_s() - tw0;
+  NSS_HIP(hipEventSynchronize(ctx->rho_ev));
+  check_symmetric();
+  if (big_abort && big_abort())
+    throw Error(NSS_E_HIP, "rho tridiagonalisation: a workgroup barrier of the LDS-resident path timed out");
   for (ClassBatch& B : batches)
     NSS_HIP(hipMemcpy(B.ci.data(), B.dci.p, B.nb * sizeof(int32_t), hipMemcpyDeviceToHost));
   for (const ClassBatch& B : batches)

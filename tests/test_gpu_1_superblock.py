@@ -98,6 +98,31 @@ def test_hv_uneven_sectors(ctx, seed):
     assert rel(ctx.hv(v, fp32=True), ref) < 1e-5
 
 
+@pytest.mark.parametrize("split", ["auto", "1", "3", "8"])
+@pytest.mark.parametrize("layout", ["C", "F"])
+def test_hv_128_tile_kernel(ctx, split, layout, monkeypatch):
+    # the 128x128 kernel forced on sectors around and across its tile edge,
+    # long and short K chains, both operand layouts, K chains split into
+    # 1, 3, 8 parts or the size-based count; FP64 against the oracle,
+    # bitwise repeatable, and the FP32-MFMA plan at its bar
+    monkeypatch.setenv("NSS_GEMM_TILE", "128")
+    if split == "auto":
+        monkeypatch.delenv("NSS_GEMM_SPLIT", raising=False)
+    else:
+        monkeypatch.setenv("NSS_GEMM_SPLIT", split)
+    dims = [(600, 40), (40, 600), (300, 300), (130, 129), (1, 1), (128, 128), (17, 255), (256, 3)]
+    sb = _random_sb(7, dims, 16)
+    print(f"case t128 split={split} layout={layout} n={sb.n}", flush=True)
+    upload(ctx, sb, layout)
+    v = O.splitmix64_uniform_fast(sb.n, 21)
+    ref = O.hv_direct(sb, v)
+    y1 = ctx.hv(v)
+    assert rel(y1, ref) < 1e-12
+    assert np.array_equal(y1, ctx.hv(v))
+    upload(ctx, sb, layout, precision="fp32")
+    assert rel(ctx.hv(v, fp32=True), ref) < 1e-5
+
+
 def test_hv_linearity_and_symmetry(ctx):
     sb = oracle_sb("hubbard", 3, 24)
     upload(ctx, sb)
@@ -265,11 +290,15 @@ def test_rdm_eigh_rank_deficient_blocks(ctx):
             assert np.abs(vec.T @ vec - np.eye(len(ev))).max() < 1e-13, (side, k)
 
 
-def test_rdm_eigh_large_block_path(ctx):
+@pytest.mark.parametrize("tred", ["column", "lds"])
+def test_rdm_eigh_large_block_path(ctx, tred, monkeypatch):
     # blocks of the large-block path (129..2048 states): graded spectra like a
     # DMRG rho (Schmidt values 10^(-k/8)), a flat random one, low rank, and
-    # the 2048 end; every eigenpair against numpy, scaled by the block's norm
-    print("case large-block rho path", flush=True)
+    # the 2048 end; every eigenpair against numpy, scaled by the block's norm.
+    # tred="lds": the blocks that fit the chip's LDS (all but the 2048 one)
+    # go through the one-launch LDS-resident tridiagonalisation
+    monkeypatch.setenv("NSS_TRED_LDS", "1" if tred == "lds" else "0")
+    print(f"case large-block rho path, {tred} tridiagonalisation", flush=True)
     dims = [(900, 900), (1500, 40), (2048, 3), (129, 600), (700, 700), (300, 1)]
     sb = _random_sb(13, dims, 0)
     upload(ctx, sb)
