@@ -5,8 +5,8 @@
                                 sweeps, LANC_NEIGEN=1, tol 1e-12, seed 0
   config2_hubbard_L64_m<M>.json Hubbard t=1, U=4, HFMODE=T, half filling,
                                 OBC, L=64 (LDMRG=31), 3 finite sweeps at
-                                m=M (the full m=256 is beyond the numpy
-                                oracle's reach; M is the largest it finishes)
+                                m=M (128, and the full 256: about 77 minutes
+                                of the numpy oracle on one core)
 
 Each record: step type, left length, L, E0, entropy of rho_L and its
 number / configurational split, kept count and truncation error of the left

@@ -3,9 +3,8 @@
   1  spin-1/2 Heisenberg, L=40, m=64, 2 finite sweeps: step by step against
      the oracle's records (tests/golden/config1_spin12_L40_m64.json).
   2  Hubbard U=4, half filling, L=64, 3 finite sweeps: step by step against
-     the oracle at the largest m the numpy oracle finishes
-     (config2_hubbard_L64_m*.json), and at the full m=256 FP64 against the
-     FP32-MFMA path and through invariants.
+     the oracle at m=128 and at the full m=256 (config2_hubbard_L64_m*.json),
+     and at m=256 FP64 against the FP32-MFMA path and through invariants.
   3  spin-1 Heisenberg, L=100, m=1024, 4 finite sweeps: FP64 against the
      FP32-MFMA path and invariants.
   4  64 Hubbard chains of L=32 (m, U, filling per chain from the seed), one
