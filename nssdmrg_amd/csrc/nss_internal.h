@@ -190,6 +190,8 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
 // With apart, the pass works on w - alpha v_a (- beta v_b with vb and bprev:
 // beta = bprev[0], written to bout by block 0), alpha the sum of the na
 // partials in apart (na < 0: lz_nch(n) of them), written to aout.
+// m <= LZ_MAX_BASIS
+constexpr int LZ_MAX_BASIS = 512;
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
                       double* hout, double* sq, hipStream_t s, const double* apart = nullptr,
                       const double* va = nullptr, double* aout = nullptr, int64_t na = -1,

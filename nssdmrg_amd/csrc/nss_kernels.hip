@@ -940,83 +940,87 @@ __global__ __launch_bounds__(256) void k_lz_alpha(double* __restrict__ w, const 
   }
 }
 
-// partial[j*nch + c] = <V_j, w - alpha v_a - beta v_b> over chunk c, alpha =
-// the sum of apart[0..na) (every block reduces it in the same fixed order),
-// beta = bprev[0] (v_b and bprev may be null); with apart == null, <V_j, w>
-__global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv,
-                                                  const double* __restrict__ w, int64_t n,
-                                                  double* __restrict__ partial, int64_t nch,
-                                                  const double* __restrict__ apart, int64_t na,
-                                                  const double* __restrict__ va, const double* __restrict__ vb,
-                                                  const double* __restrict__ bprev) {
+// One classical Gram-Schmidt pass w -= V h, h = V^T w', in four launches
+// that each read what they need once: the recurrence scalars (alpha from its
+// partials, beta), the chunk products (every block keeps its chunk of w' in
+// registers and runs it against all m basis vectors), the coefficient sums
+// (one block per vector) and the update.  Every sum has a fixed order.
+
+// aout[0] = sum of apart[0..na) (fixed order); bout[0] = bprev[0] or 0
+__global__ __launch_bounds__(256) void k_lz_scalars(const double* __restrict__ apart, int64_t na,
+                                                    double* __restrict__ aout, const double* __restrict__ bprev,
+                                                    double* __restrict__ bout) {
   __shared__ double sh[4];
-  __shared__ double ash;
-  const int64_t c = blockIdx.x;
-  const int j = blockIdx.y;
-  double a = 0.0;
-  if (apart) {
-    const double a0 = coop_sum(apart, na, sh);
-    if (threadIdx.x == 0) ash = a0;
-    __syncthreads();
-    a = ash;
+  const double a = coop_sum(apart, na, sh);
+  if (threadIdx.x == 0) {
+    aout[0] = a;
+    if (bout) bout[0] = bprev ? bprev[0] : 0.0;
   }
-  const double* v = V + (size_t)j * ldv;
-  const int64_t i0 = c * LZ_CHUNK, i1 = min<int64_t>(n, i0 + LZ_CHUNK);
-  double acc = 0.0;
-  if (apart && vb) {
-    const double b = bprev[0];
-#pragma unroll 8
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i] - b * vb[i]);
-  } else if (apart) {
-#pragma unroll 8
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * (w[i] - a * va[i]);
-  } else {
-#pragma unroll 8
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc += v[i] * w[i];
-  }
-  const double r = block_sum256(acc, sh);
-  if (threadIdx.x == 0) partial[(size_t)j * nch + c] = r;
 }
 
-// h[j] = sum_c partial[j*nch + c] (every block, one thread per column, a
-// fixed order; block 0 publishes h); w -= V[0..m) h; sq[block] = sum of
-// squares of the block's new rows
-__global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ V, int64_t ldv, int m,
-                                                     const double* __restrict__ partial, int64_t nch,
-                                                     double* __restrict__ h, double* __restrict__ w,
-                                                     int64_t n, double* __restrict__ sq,
-                                                     const double* __restrict__ apart, int64_t na,
-                                                     const double* __restrict__ va, double* __restrict__ aout,
-                                                     const double* __restrict__ vb,
-                                                     const double* __restrict__ bprev, double* __restrict__ bout) {
-  extern __shared__ double hs[];
-  __shared__ double red[4];
-  __shared__ double ash;
-  double a = 0.0;
-  if (apart) {  // alpha as k_lz_gemvt reduces it; w' = w - alpha v_a - beta v_b on the fly
-    const double a0 = coop_sum(apart, na, red);
-    if (threadIdx.x == 0) {
-      ash = a0;
-      if (blockIdx.x == 0 && aout) aout[0] = a0;
-      if (blockIdx.x == 0 && bout) bout[0] = vb ? bprev[0] : 0.0;
-    }
-    __syncthreads();
-    a = ash;
-  }
+// partial[j*nch + c] = <V_j, w'> over chunk c, w' = w - alpha v_a - beta v_b
+// (alpha = ascal[0] when ascal, beta = bprev[0] when vb)
+__global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv, int m,
+                                                  const double* __restrict__ w, int64_t n,
+                                                  double* __restrict__ partial, int64_t nch,
+                                                  const double* __restrict__ ascal, const double* __restrict__ va,
+                                                  const double* __restrict__ vb, const double* __restrict__ bprev) {
+  constexpr int PER = LZ_CHUNK / 256;
+  __shared__ double red[LZ_MAX_BASIS][4];
+  const int64_t c = blockIdx.x;
+  const int64_t i0 = c * LZ_CHUNK;
+  const double a = ascal ? ascal[0] : 0.0;
   const double b = vb ? bprev[0] : 0.0;
-  for (int j = threadIdx.x; j < m; j += 256) {
-    const double* p = partial + (size_t)j * nch;
-    double a0 = 0.0, a1 = 0.0;
-    int64_t c = 0;
-    for (; c + 2 <= nch; c += 2) {
-      a0 += p[c];
-      a1 += p[c + 1];
+  double x[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int64_t i = i0 + threadIdx.x + 256 * u;
+    double wi = 0.0;
+    if (i < n) {
+      wi = w[i];
+      if (ascal) wi -= a * va[i];
+      if (vb) wi -= b * vb[i];
     }
-    if (c < nch) a0 += p[c];
-    hs[j] = a0 + a1;
-    if (blockIdx.x == 0) h[j] = a0 + a1;
+    x[u] = wi;
+  }
+  const int wave = threadIdx.x >> 6;
+  for (int j = 0; j < m; ++j) {
+    const double* v = V + (size_t)j * ldv;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int64_t i = i0 + threadIdx.x + 256 * u;
+      if (i < n) acc += v[i] * x[u];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[j][wave] = acc;
   }
   __syncthreads();
+  for (int j = threadIdx.x; j < m; j += 256) partial[(size_t)j * nch + c] = (red[j][0] + red[j][1]) + (red[j][2] + red[j][3]);
+}
+
+// h[j] = sum_c partial[j*nch + c], one block per j
+__global__ __launch_bounds__(256) void k_lz_hsum(const double* __restrict__ partial, int64_t nch,
+                                                 double* __restrict__ h) {
+  __shared__ double sh[4];
+  const double r = coop_sum(partial + (size_t)blockIdx.x * nch, nch, sh);
+  if (threadIdx.x == 0) h[blockIdx.x] = r;
+}
+
+// w = w' - V[0..m) h (w' as in k_lz_gemvt); sq[block] = sum of squares of
+// the block's new rows
+__global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ V, int64_t ldv, int m,
+                                                     const double* __restrict__ h, double* __restrict__ w,
+                                                     int64_t n, double* __restrict__ sq,
+                                                     const double* __restrict__ ascal, const double* __restrict__ va,
+                                                     const double* __restrict__ vb,
+                                                     const double* __restrict__ bprev) {
+  extern __shared__ double hs[];
+  __shared__ double red[4];
+  for (int j = threadIdx.x; j < m; j += 256) hs[j] = h[j];
+  __syncthreads();
+  const double a = ascal ? ascal[0] : 0.0;
+  const double b = vb ? bprev[0] : 0.0;
   double ss = 0.0;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     // eight basis vectors per step, their loads issued together
@@ -1032,7 +1036,9 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
       a3 += x[3] * hs[j + 3] + x[7] * hs[j + 7];
     }
     for (; j < m; ++j) a0 += V[(size_t)j * ldv + i] * hs[j];
-    const double wi = apart ? (vb ? w[i] - a * va[i] - b * vb[i] : w[i] - a * va[i]) : w[i];
+    double wi = w[i];
+    if (ascal) wi -= a * va[i];
+    if (vb) wi -= b * vb[i];
     const double x = wi - ((a0 + a1) + (a2 + a3));
     w[i] = x;
     ss += x * x;
@@ -1070,12 +1076,21 @@ void launch_lz_recurrence(double* w, const double* vprev, const double* vj, int6
 void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial,
                       double* hout, double* sq, hipStream_t s, const double* apart, const double* va,
                       double* aout, int64_t na, const double* vb, const double* bprev, double* bout) {
+  NSS_REQUIRE(m >= 1 && m <= LZ_MAX_BASIS, NSS_E_ARG, "Gram-Schmidt pass: basis size outside 1.." +
+                                                        std::to_string(LZ_MAX_BASIS));
+  NSS_REQUIRE(!apart || aout, NSS_E_ARG, "Gram-Schmidt pass: alpha partials without an alpha slot");
   const int64_t nch = lz_nch(n);
   if (na < 0) na = nch;
-  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)m), dim3(256), 0, s, V, ldv, w, n, partial, nch,
-                     apart, na, va, vb, bprev);
+  const double* ascal = nullptr;
+  if (apart) {
+    hipLaunchKernelGGL(k_lz_scalars, dim3(1), dim3(256), 0, s, apart, na, aout, vb ? bprev : nullptr, bout);
+    ascal = aout;
+  }
+  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch), dim3(256), 0, s, V, ldv, m, w, n, partial, nch, ascal, va, vb,
+                     bprev);
+  hipLaunchKernelGGL(k_lz_hsum, dim3((unsigned)m), dim3(256), 0, s, partial, nch, hout);
   hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
-                     ldv, m, partial, nch, hout, w, n, sq, apart, na, va, aout, vb, bprev, bout);
+                     ldv, m, hout, w, n, sq, ascal, va, vb, bprev);
   NSS_HIP(hipGetLastError());
 }
 

@@ -512,6 +512,7 @@ LanczosOut lanczos_device(nss_ctx* ctx, SbDevice* sb, int neigen, int kmax, int 
   NSS_REQUIRE(n > 0, NSS_E_STATE, "lanczos: empty superblock");
   NSS_REQUIRE(neigen >= 1 && neigen <= 4 && neigen <= n, NSS_E_ARG, "lanczos: 1 <= neigen <= min(4,n)");
   if (kmax <= 0) kmax = 150;
+  kmax = std::min(kmax, LZ_MAX_BASIS - 1);  // the Gram-Schmidt pass takes kmax + 1 vectors
   kmax = (int)std::min<int64_t>(kmax, n);
   kmax = std::max(kmax, std::min<int>((int)n, neigen + 2));
   if (maxiter <= 0) maxiter = 512;

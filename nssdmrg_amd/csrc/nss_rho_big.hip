@@ -5,12 +5,12 @@
 //
 //  1. Householder tridiagonalisation T = Q^T A Q (dsytd2's lower form,
 //     Q = H_0 H_1 ... H_{n-3}, H_k = I - tau_k v_k v_k^T, v_k[k+1] = 1), one
-//     launch per column j for all blocks.  Each workgroup owns TRED_ROWS rows
-//     of the trailing matrix.  It recomputes, redundantly, the previous
-//     step's w = p - (tau/2)(p.v) v and the updated row j (the column j by
-//     symmetry), forms v_j, then in one pass over its rows applies the
-//     previous rank-2 update A -= v w^T + w v^T and accumulates its share of
-//     p_j = tau_j A v_j.  The kernel boundary is the only global
+//     launch per column j for all blocks.  Each wave of a workgroup owns one
+//     row of the trailing matrix.  The workgroup recomputes, redundantly,
+//     the previous step's w = p - (tau/2)(p.v) v and the updated row j (the
+//     column j by symmetry), forms v_j, then each wave applies the previous
+//     rank-2 update A -= v w^T + w v^T to its row and accumulates its share
+//     of p_j = tau_j A v_j.  The kernel boundary is the only global
 //     synchronisation.  Reflector k is kept in row k of the block
 //     (columns k+1..n-1), written by the launch after the one that formed it
 //     (nothing reads row k then).
@@ -32,9 +32,6 @@ namespace nss {
 
 namespace {
 
-constexpr int TRED_THREADS = 1024;
-constexpr int TRED_WAVES = TRED_THREADS / 64;
-constexpr int TRED_ROWS = 16;  // trailing-matrix rows per workgroup: one per wave
 constexpr int EIG_THREADS = 256;
 constexpr int EIG_BIS = 4;     // lanes (interior points) per eigenvalue
 constexpr int EIG_PER_WG = EIG_THREADS / EIG_BIS;
@@ -111,17 +108,23 @@ __device__ __forceinline__ double rcp_d(double q) {
 int64_t tred_ws_size(int n) { return (int64_t)7 * n; }
 
 // ------------------------------------------------------------ 1. tridiagonal
-__global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __restrict__ blks, int nblk, int j,
-                                                           int nmax) {
+// Every global load of a column is issued at the start, in one round: the
+// previous v and p and row j (into registers, then LDS) and this wave's own
+// row (into registers, RV doubles per lane: n <= 64 RV), so the launch
+// waits for memory once rather than once per dependent stage.  NT threads,
+// one trailing-matrix row per wave.
+template <int NT, int RV>
+__global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blks, int nblk, int j, int nmax) {
+  constexpr int NW = NT / 64, ROWS = NW, NU = (RHO_BIG_MAX + NT - 1) / NT;
   extern __shared__ double sm[];  // vp, wp over [j, n); vj over [j, n)
-  __shared__ double red[TRED_WAVES];
+  __shared__ double red[NW];
   const int b = blockIdx.x % nblk, t = blockIdx.x / nblk;
   const TredBlk B = blks[b];
   const int n = B.n;
   if (j >= n) return;
-  const int r0 = j + 1 + t * TRED_ROWS;
+  const int r0 = j + 1 + t * ROWS;
   if (t > 0 && r0 >= n) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int m = n - j;  // the trailing range [j, n)
   double* vp = sm;
   double* wp = sm + nmax;
@@ -133,35 +136,66 @@ __global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __rest
   double* dd = ws + 4 * n;
   double* ee = ws + 5 * n;
   double* tau = ws + 6 * n;
-  // previous step: v_{j-1}, p_{j-1} over [j, n); w = p - (tau/2)(p.v) v
+  // loads: v_{j-1}, p_{j-1} and row j over [j, n) (at most NU entries per
+  // thread), this wave's row at columns j+1.. n-1
+  const double* vq = vbuf + ((j - 1) & 1) * n;
+  const double* pq = pbuf + ((j - 1) & 1) * n;
+  const double* rowj = A + (int64_t)j * n + j;
+  double lv[NU], lp[NU], lr[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int i = tid + u * NT;
+    lv[u] = lp[u] = lr[u] = 0.0;
+    if (i < m) {
+      if (j > 0) {
+        lv[u] = vq[j + i];
+        lp[u] = pq[j + i];
+      }
+      lr[u] = rowj[i];
+    }
+  }
+  const int io = r0 + wv;  // this wave's row
+  const bool own = io < n;
+  double* row = A + (int64_t)(own ? io : j) * n;
+  double rv[RV];
+#pragma unroll
+  for (int u = 0; u < RV; ++u) {
+    const int c = j + 1 + lane + 64 * u;
+    rv[u] = (own && c < n) ? row[c] : 0.0;
+  }
+  // previous step: w = p - (tau/2)(p.v) v
   const double taup = j > 0 ? tau[j - 1] : 0.0;
   double pv = 0.0;
-  if (j > 0) {
-    const double* vq = vbuf + ((j - 1) & 1) * n;
-    const double* pq = pbuf + ((j - 1) & 1) * n;
-    for (int i = tid; i < m; i += TRED_THREADS) {
-      const double v = vq[j + i], p = pq[j + i];
-      vp[i] = v;
-      wp[i] = p;
-      pv += p * v;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int i = tid + u * NT;
+    if (i < m) {
+      vp[i] = lv[u];
+      wp[i] = lp[u];
+      pv += lp[u] * lv[u];
     }
-  } else {
-    for (int i = tid; i < m; i += TRED_THREADS) vp[i] = wp[i] = 0.0;
   }
-  pv = bsum<TRED_WAVES>(pv, red);  // (its barriers also publish vp, wp)
+  pv = bsum<NW>(pv, red);  // (its barriers also publish vp, wp)
   const double K = 0.5 * taup * pv;
-  for (int i = tid; i < m; i += TRED_THREADS) wp[i] -= K * vp[i];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int i = tid + u * NT;
+    if (i < m) wp[i] = lp[u] - K * lv[u];
+  }
   __syncthreads();
   // row j after the previous update (the column j by symmetry), into vj
   const double v0 = vp[0], w0 = wp[0];
   double s2 = 0.0;
-  const double* rowj = A + (int64_t)j * n + j;
-  for (int i = tid; i < m; i += TRED_THREADS) {
-    const double r = rowj[i] - v0 * wp[i] - w0 * vp[i];
-    vj[i] = r;
-    if (i >= 2) s2 += r * r;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int i = tid + u * NT;
+    if (i < m) {
+      const double r = lr[u] - v0 * wp[i] - w0 * lv[u];
+      vj[i] = r;
+      if (i >= 2) s2 += r * r;
+    }
   }
-  s2 = bsum<TRED_WAVES>(s2, red);
+  s2 = bsum<NW>(s2, red);
   const double dj = vj[0];
   const double alpha = m > 1 ? vj[1] : 0.0;
   double tj = 0.0, beta = alpha, scale = 0.0;
@@ -171,7 +205,7 @@ __global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __rest
     scale = 1.0 / (alpha - beta);
   }
   __syncthreads();  // every thread has read vj[0], vj[1]
-  for (int i = tid + 1; i < m; i += TRED_THREADS) vj[i] = i == 1 ? 1.0 : vj[i] * scale;
+  for (int i = tid + 1; i < m; i += NT) vj[i] = i == 1 ? 1.0 : vj[i] * scale;
   __syncthreads();
   if (t == 0) {
     if (tid == 0) {
@@ -180,42 +214,45 @@ __global__ __launch_bounds__(TRED_THREADS) void k_tred_col(const TredBlk* __rest
       tau[j] = tj;
     }
     double* vo = vbuf + (j & 1) * n;
-    for (int i = tid + 1; i < m; i += TRED_THREADS) vo[j + i] = vj[i];
+    for (int i = tid + 1; i < m; i += NT) vo[j + i] = vj[i];
     if (j > 0) {  // reflector j-1 to its row (columns j..n-1)
       double* keep = A + (int64_t)(j - 1) * n;
-      for (int i = tid; i < m; i += TRED_THREADS) keep[j + i] = vp[i];
+      for (int i = tid; i < m; i += NT) keep[j + i] = vp[i];
     }
   }
-  // owned rows: the previous update, then this row's share of p_j = tau_j A v_j
-  const int wv = tid >> 6, lane = tid & 63;
-  double* po = pbuf + (j & 1) * n;
-  const int r1 = min(r0 + TRED_ROWS, n);
-  for (int i = r0 + wv; i < r1; i += TRED_WAVES) {
-    const int li = i - j;
+  // this wave's row: the previous update, then its share of p_j = tau_j A v_j
+  if (own) {
+    const int li = io - j;
     const double vi = vp[li], wi = wp[li];
-    double* row = A + (int64_t)i * n;
     double acc = 0.0;
-    if (j > 0) {
-#pragma unroll 4
-      for (int c = j + 1 + lane; c < n; c += 64) {
+#pragma unroll
+    for (int u = 0; u < RV; ++u) {
+      const int c = j + 1 + lane + 64 * u;
+      if (c < n) {
         const int lc = c - j;
-        const double a = row[c] - vi * wp[lc] - wi * vp[lc];
-        row[c] = a;
+        double a = rv[u];
+        if (j > 0) {
+          a -= vi * wp[lc] + wi * vp[lc];
+          row[c] = a;
+        }
         acc += a * vj[lc];
       }
-    } else {
-#pragma unroll 4
-      for (int c = 1 + lane; c < n; c += 64) acc += row[c] * vj[c];
     }
     acc = wsum(acc);
-    if (lane == 0) po[i] = tj * acc;
+    if (lane == 0) (pbuf + (j & 1) * n)[io] = tj * acc;
   }
 }
 
 void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t s) {
-  const int tiles = std::max(1, (nmax - j - 1 + TRED_ROWS - 1) / TRED_ROWS);
-  hipLaunchKernelGGL(k_tred_col, dim3(nblk * tiles), dim3(TRED_THREADS), (size_t)3 * nmax * sizeof(double), s,
-                     blks, nblk, j, nmax);
+  NSS_REQUIRE(nmax <= RHO_BIG_MAX, NSS_E_ARG, "tridiagonalisation: block above the large-block limit");
+  const size_t lds = (size_t)3 * nmax * sizeof(double);
+  if (nmax <= 1024) {
+    const int tiles = std::max(1, (nmax - j - 1 + 15) / 16);
+    hipLaunchKernelGGL((k_tred_col<1024, 16>), dim3(nblk * tiles), dim3(1024), lds, s, blks, nblk, j, nmax);
+  } else {
+    const int tiles = std::max(1, (nmax - j - 1 + 7) / 8);
+    hipLaunchKernelGGL((k_tred_col<512, 32>), dim3(nblk * tiles), dim3(512), lds, s, blks, nblk, j, nmax);
+  }
   NSS_HIP(hipGetLastError());
 }
 

@@ -734,6 +734,14 @@ void rho_materialize(nss_ctx* ctx, const std::vector<RhoVec*>& vecs, const std::
   NSS_HIP(hipMemcpyAsync(B->dreq.p, B->hreq.data(), nreq * sizeof(InvReq), hipMemcpyHostToDevice, s));
   if (ncl) NSS_HIP(hipMemcpyAsync(B->dcl.p, B->hcl.data(), ncl * sizeof(GsCluster), hipMemcpyHostToDevice, s));
   NSS_HIP(hipMemcpyAsync(B->dbt.p, B->hbt.data(), nbt * sizeof(BtTask), hipMemcpyHostToDevice, s));
+  if (ctx->prof)
+    for (const GsCluster& c : B->hcl) {
+      const double nv = c.s1 - c.s0;
+      ctx->timers["rdm.gs_clusters"] += 1;
+      ctx->timers["rdm.gs_vectors"] += nv;
+      ctx->timers["rdm.gs_work_c2n"] += nv * nv * (c.re - c.rs);
+      ctx->timers["rdm.gs_cmax"] = std::max(ctx->timers["rdm.gs_cmax"], nv);
+    }
   launch_tri_invit(B->dreq.p, nreq, B->fac.p, nmax, s);
   launch_tri_cluster_gs(B->dcl.p, ncl, s);
   if (!lo.empty()) launch_tri_backtr(B->dbt.p, (int)lo.size(), 1024, s);
