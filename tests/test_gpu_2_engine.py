@@ -513,6 +513,7 @@ def test_entanglement_split_and_lambdas_vs_oracle(ctx, model):
     d, st = run(ctx, "f", site=site, hij=np.array(hij), ldmrg=L, mdmrg=M, msweep=[M], target_qn=tq, neigen=1,
                 lanc_dim_threshold=64, uloc=U, hfmode=True)
     assert len(st) == len(od.records)
+    clean = True  # no kept state so far at rounding level (rho null space)
     for i, (s, r) in enumerate(zip(st, od.records)):
         assert abs(s["sn_left"] - r.sn_left) < 1e-9 and abs(s["sc_left"] - r.sc_left) < 1e-9, i
         assert abs(s["sn_right"] - r.sn_right) < 1e-9 and abs(s["sc_right"] - r.sc_right) < 1e-9, i
@@ -522,8 +523,15 @@ def test_entanglement_split_and_lambdas_vs_oracle(ctx, model):
         assert set(got) == set(ref), i
         for q, rv in ref.items():
             gv = got[q]
-            big = rv > W_MIN
-            assert len(gv) == len(rv) and np.allclose(gv[big], rv[big], rtol=1e-8, atol=1e-14), (i, q)
+            # once a cut has kept null-space states, which ones it kept (and
+            # so the later blocks' dimensions) is arbitrary, in the reference
+            # too: from then on only the spectrum above W_MIN is compared
+            if clean:
+                assert len(gv) == len(rv), (i, q)
+            a, b = np.sort(gv[gv > W_MIN]), np.sort(rv[rv > W_MIN])
+            assert len(a) == len(b) and np.allclose(a, b, rtol=1e-8, atol=1e-14), (i, q)
+        if s["renormalized"] and min(s["cut_left"], s["cut_right"]) <= W_MIN:
+            clean = False
 
 
 def test_qntype_global_and_qn_truncation_vs_oracle(ctx):
