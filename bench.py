@@ -161,6 +161,7 @@ def run_path(N, ctx, w, warmup, steps, precision, verbose, predict=True):
     for _ in range(warmup):
         if not d.finite_step():
             raise SystemExit("bench.py: --warmup exceeds the workload's finite steps")
+    trace_marker()
     ctx.hv_stats(reset=True)
     t = time.perf_counter()
     for _ in range(steps):  # each finite step returns with the device synchronised
@@ -168,6 +169,7 @@ def run_path(N, ctx, w, warmup, steps, precision, verbose, predict=True):
             raise SystemExit("bench.py: --warmup + --steps exceed the workload's finite steps")
     timed = time.perf_counter() - t
     st = ctx.hv_stats(reset=True)
+    trace_marker()
     while d.finite_step():
         pass
     st_rest = ctx.hv_stats(reset=True)
@@ -266,6 +268,16 @@ def copy_probe(mib, reps):
     sec = C.c_double(0.0)
     F.check(F.lib().nss_probe_copy(n, reps, C.byref(sec)))
     return 2.0 * n * 8 / sec.value / 1e9
+
+
+def trace_marker():
+    """Two one-workgroup k_copy_probe dispatches (grid of 256 work items),
+    outside every timed interval: in a rocprofv3 kernel trace they bracket a
+    timed region, so its dispatches can be told from the rest of the run."""
+    import ctypes as C
+    from nssdmrg_amd import _ffi as F
+    sec = C.c_double(0.0)
+    F.check(F.lib().nss_probe_copy(512, 1, C.byref(sec)))
 
 
 def hv_fields(st, hv_time):
@@ -407,8 +419,11 @@ def main_replay(N, ctx, w, a):
     st = ctx.hv_stats(reset=True)
     calls = max(1, st["calls"])
     y_avg_s, y_flops = st["t_y"] / calls, st["flops_y"] / calls
+    sb = d.last_superblock()
     out = {"workload": a.workload, "hv_calls": st["calls_all"], "wall_s": wall, "m_sb": int(v.size),
            "nsect": int(last["nsect"]), "nterms": int(last["nterms"]),
+           "sectors": {"Dl": [int(x) for x in sb["Dl"]], "Dr": [int(x) for x in sb["Dr"]],
+                       "term_dst": [int(x) for x in sb["dst"]], "term_src": [int(x) for x in sb["src"]]},
            "k_hv_y_f64": {"avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
                           "tflops": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0},
            "k_hv_w_f64": {"avg_launch_us": 1e6 * st["t_w"] / calls, "flops_per_launch": st["flops_w"] / calls}}

@@ -115,6 +115,20 @@ struct GemmTile {
   int32_t cbase, beta;
 };
 
+// One work item of the large-output path: slices [c0, c1) of tile `tile`'s
+// K chain (part `part` of nsplit; partial-tile slot pslot when nsplit > 1,
+// the parts of a tile in consecutive slots).
+struct BigItem {
+  int32_t tile, part, nsplit, c0, c1, pslot;
+};
+
+struct BigArgs {
+  const BigItem* items;
+  int nitems, nbig;  // work items; large tiles at the front of the tile list
+  void* part;
+  unsigned* cnt;
+};
+
 struct Bases {
   const void* x;
   void* w;
@@ -134,27 +148,32 @@ struct GemmPlan {
   DevArr<GemmTile> tiles;
   DevArr<GemmSeg> segs;
   double flops = 0.0;
-  // Output tile edge, chosen in upload(): 32 -> k_* (32x32 tile, K chain
-  // split over the four waves of a workgroup, reduced through LDS; for
-  // launches too small to fill the chip); 64 -> k_*_t64 (64x64 tile, four
-  // waves share LDS-staged operands; for large launches).  NSS_GEMM_TILE
-  // forces either.
+  // Output tile edge of the outputs not on the large-output path, chosen
+  // in upload(): 32 -> k_* (32x32 tile, K chain split over the four waves
+  // of a workgroup, reduced through LDS; for launches too small to fill the
+  // chip); 64 -> k_*_t64 (64x64 tile, four waves share LDS-staged operands;
+  // for large launches of small outputs).  NSS_GEMM_TILE forces either.
   int tile = 32;
-  // 128 -> k_*_t128 (128x128 tile, two-stage LDS ring; for launches with
-  // large operands): each tile's K chain cut into nsplit parts, one
-  // workgroup each, summed in part order by the tile's last workgroup
-  // through part (partial tiles) and cnt (arrival counters, zero between
-  // launches)
-  int nsplit = 1;
+  // Large-output path (k_*_t128): the outputs of at least 128x128 whose
+  // edges pad little are cut into 128x128 tiles, the first nbig entries of
+  // htiles, and their slice chains into work items (hbig) of about equal
+  // length; the plan's other tiles are 32x32 and run in the same launch.
+  // A tile cut into several parts sums them in part order through part
+  // (partial tiles) and cnt (arrival counters, zero between launches).
+  int nbig = 0;
+  std::vector<BigItem> hbig;
+  DevArr<BigItem> big;
   DevArr<double> part;
   DevArr<unsigned> cnt;
   std::vector<GemmTile> outs;  // one per add_output (row0 = col0 = 0)
-  void clear() { htiles.clear(); hsegs.clear(); outs.clear(); flops = 0.0; }
+  void clear() { htiles.clear(); hsegs.clear(); outs.clear(); hbig.clear(); nbig = 0; flops = 0.0; }
   // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
   // the same list of segments.
   void add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
                   const std::vector<GemmSeg>& segs_for_c, int beta = 0);
-  void upload(DevicePool* pool, hipStream_t s);
+  // esize: operand bytes of the launches (8 FP64, 4 FP32), which set the K
+  // slice of the large-output path's work items
+  void upload(DevicePool* pool, hipStream_t s, int esize = 8);
 };
 
 constexpr int GEMM_BM = 64, GEMM_BN = 64;

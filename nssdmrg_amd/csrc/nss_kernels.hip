@@ -281,14 +281,15 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
 
 template <typename T, int NBUF, bool SKIPLOAD>
 __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
-                                            int ntiles, const Bases& bs) {
+                                            int t, const Bases& bs, T (*red)[32][33], int tix = -1,
+                                            bool store = true) {
+  // tix: this thread's index within the tile's 256 threads (threadIdx.x when
+  // a workgroup runs one tile); store = false computes without writing
+  if (tix < 0) tix = threadIdx.x;
   using acc_t = typename Mfma<T>::acc_t;
-  __shared__ T red[4][32][33];
-  const int t = xcd_tile(ntiles);
-  if (t < 0) return;
   const GemmTile tl = tiles[t];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int lane = tix & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tix >> 6);  // wave-uniform
   const int fr = lane & 15, fk = lane >> 4;
 
   acc_t acc[2][2];
@@ -396,10 +397,10 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   double dsum = 0.0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int e = threadIdx.x + 256 * u;
+    const int e = tix + 256 * u;
     const int row = e >> 5, col = e & 31;
     const int gi = tl.row0 + row, gj = tl.col0 + col;
-    if (gi < tl.M && gj < tl.N) {
+    if (store && gi < tl.M && gj < tl.N) {
       const T v = (red[0][row][col] + red[1][row][col]) + (red[2][row][col] + red[3][row][col]);
       T* p = C + (size_t)gi * tl.ldc + gj;
       const T out = tl.beta ? *p + v : v;
@@ -409,288 +410,324 @@ __device__ __forceinline__ void gemm_tile32(const GemmTile* __restrict__ tiles, 
   }
   if (bs.dot) {  // each wave's share of x^T y (no block barrier)
     const double r = wave_sum(dsum);
-    if ((threadIdx.x & 63) == 0) bs.dot[4 * t + (threadIdx.x >> 6)] = r;
+    if (lane == 0 && store) bs.dot[4 * t + wave] = r;
   }
 }
 
 // ---------------------------------------------------------------------------
-// 128x128 output tile per workgroup of eight waves (a 32x64 block each, 2x4
-// MFMA 16x16 blocks).  Operands move from global memory to LDS by the
-// LDS-DMA loads (global_load_lds_dword: lane l's dword lands at the
-// wave-uniform LDS base + 4 l), 16-deep K slices through a ring of G128_NST
-// stages: slice c+2 is in flight while slice c computes, and no register
-// holds operand data on the way.  Per slice every wave issues the same
-// number of DMA instructions, so "slice c has landed" is a vmcnt bound
-// (all but the later slices' instructions done) followed by a workgroup
-// barrier.  Rows/columns past M/N and k past K load clamped (valid)
-// addresses; the A fragments carry the segment's alpha and are zeroed for
-// k past K at use.
-// A tile's K chain (its segments' slices in order) can be cut into nsplit
-// parts, one workgroup each: every part writes its partial tile to part[],
-// and the workgroup that completes the tile (a per-tile arrival counter)
-// sums the partials in part order, so the result does not depend on timing,
+// Large-output path: 128x128 output tile per workgroup of GB_NW = 4 waves in
+// a 2x2 grid, each wave a 64x64 block (4x4 MFMA 16x16 blocks held in
+// registers for the whole K chain).  Operands go global -> registers -> LDS
+// in K slices of BK (16 for FP64, 32 for FP32), two LDS stages: while slice
+// c computes, slice c+1 goes from registers to the free stage and slice
+// c+2's loads are in flight (L2 hits, a few hundred cycles, against a
+// slice's thousands of MFMA cycles); one barrier per slice, two workgroups
+// per CU.  Every load address is a precomputed per-thread offset plus a
+// per-slice scalar, so a slice issues its loads and stores with next to no
+// vector arithmetic.
+// Every operand layout is stored the same way in LDS: element (k, r) of a
+// stage at k*P + (r ^ bsw(k)) (row pitch P, below).  Rows past M/N and k past
+// K load clamped (valid) addresses; k past K is zeroed and alpha applied as
+// the slice is written to LDS.
+// A work item is one part [c0, c1) of a tile's slice chain (the slices of
+// its segments in order).  A tile cut into several parts writes every
+// part's partial tile to part[] (lane-major per register, coalesced); the
+// workgroup that completes the tile (a per-tile arrival counter) sums the
+// partials in part order, so the result does not depend on timing, then
 // applies beta and the dot epilogue and resets the counter for the next
 // launch.
-// LDS layouts of a 128x16 slice (2048 elements per operand), chosen per
-// segment by where the operand is contiguous:
-//   RK (contiguous along k: A untransposed, B transposed): element (r, k)
-//      at r*16 + (k ^ sw(r)), sw(r) = ((r >> 1) & 7) << 1;
-//   KR (contiguous along the row index: A transposed, B untransposed):
-//      element (r, k) at k*128 + (r ^ ((k & 1) << 4)).
-// Both keep the 16 lanes of a fragment read (16 rows, one k) and the two
-// 16-lane groups of a half-wave on distinct banks.
-constexpr int G128 = 128, GBK = 16, G128_NW = 8, G128_THREADS = 64 * G128_NW, G128_NST = 3;
+constexpr int GB = 128, GB_NW = 4, GB_THREADS = 64 * GB_NW;
+constexpr int GB_WC = GB_NW / 2;          // wave grid 2 x GB_WC
+constexpr int GB_WN = GB / GB_WC;         // wave block 64 x GB_WN
+constexpr int GB_NB = GB_WN / 16;         // MFMA blocks per wave along N
+constexpr int GB_REGS = 4 * GB_NB * 4;    // accumulator elements per lane
+
+// LDS stage layout: row k (one k, 128 rows of the output) starts at
+// k*P + H*(k&1), element r of it at that start + r (rows do not overlap:
+// H <= P - 128).  With P = 138 and H = 6 (FP64) or 8 (FP32) an MFMA fragment
+// read (16 rows at each of 4 consecutive k) meets no bank twice in either
+// half-wave, and the stores of both global layouts take the minimum number
+// of passes (checked exhaustively over the lane patterns of the three access
+// kinds).  Rows are not permuted, so every address of a slice is a
+// per-thread base plus a compile-time offset.
+template <typename T>
+struct BigCfg {
+  static constexpr int BK = sizeof(T) == 8 ? 16 : 32;  // K slice
+  static constexpr int EPT = GB * BK / GB_THREADS;     // elements per thread and operand per slice
+  static constexpr int RPI = 64 / BK;                  // rows per wave load when contiguous along k
+  static constexpr int P = 138;                        // LDS row pitch (elements)
+  static constexpr int H = sizeof(T) == 8 ? 6 : 8;     // extra start offset of odd rows
+  static_assert(H >= 0 && H <= P - GB, "LDS rows overlap");
+  static constexpr int STAGE = BK * P + H;             // elements of one operand in one stage
+  static constexpr int KC_STEP = RPI * GB_NW;          // row step between a thread's elements (k-contiguous)
+  static constexpr int RC_KSTEP = GB_THREADS / GB;     // k step between a thread's elements (row-contiguous)
+};
+
+// LDS of the t128 launches: two stages of both operands, or the reduction
+// buffers of the GB_NW/4 tiles of 32x32 a workgroup runs on the small path
+constexpr int GB_LDS_BYTES = std::max<int>({(GB_NW / 4) * 4 * 32 * 33 * 8, 4 * BigCfg<double>::STAGE * 8,
+                                            4 * BigCfg<float>::STAGE * 4});
+
+// LDS pointers typed as such: a generic pointer to LDS compiles to flat
+// accesses, which share lgkmcnt with the global loads in flight.
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
 
 template <typename T>
-struct Dma128 {  // DMA instructions per slice: EPI elements each, IPW per wave and operand
-  static constexpr int EPI = 256 / (int)sizeof(T);
-  static constexpr int IPW = G128 * GBK / EPI / G128_NW;
-};
-
-struct View128 {
-  const char* A;
-  const char* B;
-  int64_t ars, aks, brs, bks;  // element (row, k) at row * rs + k * ks (elements)
-  int K;
-  bool ark, brk;  // operand contiguous along k (layout RK)
-  double alpha;
-};
-
-struct SplitK {
-  int nsplit;
-  void* part;     // ntiles * nsplit * 128 * 128 elements (nsplit > 1)
-  unsigned* cnt;  // ntiles arrival counters, zero between launches
-};
-
-__device__ __forceinline__ int sw_rk(int r) { return ((r >> 1) & 7) << 1; }
-
-// offset of element (row, k) of a stored slice
-__device__ __forceinline__ int off128(int r, int k, bool rk) {
-  return rk ? r * 16 + (k ^ sw_rk(r)) : k * 128 + (r ^ ((k & 1) << 4));
+__device__ __forceinline__ int big_rowstart(int k) {
+  return k * BigCfg<T>::P + BigCfg<T>::H * (k & 1);
 }
 
-// element (row, k) stored at offset o
-__device__ __forceinline__ void inv128(int o, bool rk, int& r, int& k) {
-  const int r_rk = o >> 4, k_kr = o >> 7;
-  r = rk ? r_rk : (o & 127) ^ ((k_kr & 1) << 4);
-  k = rk ? (o & 15) ^ sw_rk(r_rk) : k_kr;
+// Element u of this thread in a slice, by the operand's layout.  Contiguous
+// along k: lane l of wave w takes k = l / RPI of row l % RPI + RPI w + KC_STEP
+// u (each load instruction reads RPI rows of BK consecutive k).  Contiguous
+// along the row index: row tid % 128, k tid/128 + RC_KSTEP u.
+template <typename T>
+__device__ __forceinline__ int big_row(bool kc, int tid, int u) {
+  constexpr int RPI = BigCfg<T>::RPI;
+  return kc ? (tid & 63) % RPI + RPI * (tid >> 6) + BigCfg<T>::KC_STEP * u : tid % GB;
+}
+template <typename T>
+__device__ __forceinline__ int big_k(bool kc, int tid, int u) {
+  return kc ? (tid & 63) / BigCfg<T>::RPI : tid / GB + BigCfg<T>::RC_KSTEP * u;
+}
+
+// One operand of the current segment: element (r, k) at p + (r*rs + k*ks)
+// elements.  In an interior tile (all 128 rows valid) element u of this
+// thread sits at byte offset off0 + u*ustep from p + k0*ks*sizeof(T); edge
+// tiles and the segment's last, partial slice clamp rows and k instead.
+// (Element offsets stay below 2^31 and byte offsets below 2^32: the host
+// sends larger operands elsewhere.)
+template <typename T>
+struct BigOp {
+  gptr<char> p;
+  int rs, ks;
+  int r0, rmax;  // tile's first row; last valid row (relative)
+  bool kc;       // contiguous along k
+  uint32_t off0, ustep;
+};
+
+template <typename T>
+__device__ __forceinline__ void big_op(const T* base, int rs, int ks, bool kc, int r0, int rmax, int tid,
+                                       BigOp<T>& o) {
+  o.p = (gptr<char>)base;
+  o.rs = rs;
+  o.ks = ks;
+  o.kc = kc;
+  o.r0 = r0;
+  o.rmax = rmax;
+  o.off0 = (uint32_t)((big_row<T>(kc, tid, 0) + r0) * rs + big_k<T>(kc, tid, 0) * ks) * (uint32_t)sizeof(T);
+  o.ustep = (uint32_t)(kc ? BigCfg<T>::KC_STEP * rs : BigCfg<T>::RC_KSTEP * ks) * (uint32_t)sizeof(T);
 }
 
 template <typename T>
-__device__ __forceinline__ void view128(const GemmSeg& g, const Bases& bs, View128& v) {
-  v.A = reinterpret_cast<const char*>(resolve_c<T>(g.abase, g.a, g.aoff, bs));
-  v.B = reinterpret_cast<const char*>(resolve_c<T>(g.bbase, g.b, g.boff, bs));
+__device__ __forceinline__ void big_view(const GemmSeg& g, const Bases& bs, const GemmTile& tl, int tid,
+                                         BigOp<T>& A, BigOp<T>& B) {
   const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
-  v.ars = ta ? 1 : g.lda;
-  v.aks = ta ? g.lda : 1;
-  v.brs = tb ? g.ldb : 1;  // B's "row" is the output column j
-  v.bks = tb ? 1 : g.ldb;
-  v.K = g.K;
-  v.ark = !ta;
-  v.brk = tb;
-  v.alpha = g.alpha;
+  big_op<T>(resolve_c<T>(g.abase, g.a, g.aoff, bs), ta ? 1 : g.lda, ta ? g.lda : 1, !ta, tl.row0,
+            tl.M - 1 - tl.row0, tid, A);
+  // B's "row" is the output column j
+  big_op<T>(resolve_c<T>(g.bbase, g.b, g.boff, bs), tb ? g.ldb : 1, tb ? 1 : g.ldb, tb, tl.col0,
+            tl.N - 1 - tl.col0, tid, B);
 }
 
-typedef const __attribute__((address_space(1))) void* gvoid_p;
-typedef __attribute__((address_space(3))) void* lvoid_p;
-
-// This wave's DMA instructions of slice [k0, k0+16) into stage (As, Bs).
 template <typename T>
-__device__ __forceinline__ void dma128(const View128& v, const GemmTile& tl, int k0, int wave, int lane, T* As,
-                                       T* Bs) {
-  using D = Dma128<T>;
-  const int sub = sizeof(T) == 8 ? lane >> 1 : lane;        // element slot within the instruction
-  const int half = sizeof(T) == 8 ? (lane & 1) * 4 : 0;     // byte within the element
+struct BigSlice {  // one slice of both operands in registers, with what its LDS store needs
+  T a[BigCfg<T>::EPT], b[BigCfg<T>::EPT];
+  bool akc, bkc;
+  int kval;  // valid k of the slice
+  T alpha;
+};
+
+// Slice [k0, k0 + BK) of one operand into registers.
+template <typename T>
+__device__ __forceinline__ void big_load(const BigOp<T>& o, int k0, int K, int tid, T (&v)[BigCfg<T>::EPT]) {
+  if (k0 + BigCfg<T>::BK <= K && o.rmax >= GB - 1) {
+    const gptr<char> base = o.p + (uint32_t)(k0 * o.ks) * (uint32_t)sizeof(T);
 #pragma unroll
-  for (int i = 0; i < D::IPW; ++i) {
-    const int q = wave * D::IPW + i;
-    int r, k;
-    inv128(q * D::EPI + sub, v.ark, r, k);
-    const int64_t gi = min(tl.row0 + r, tl.M - 1);
-    const int64_t gk = min(k0 + k, v.K - 1);
-    const char* src = v.A + (gi * v.ars + gk * v.aks) * (int64_t)sizeof(T) + half;
-    __builtin_amdgcn_global_load_lds((gvoid_p)src, (lvoid_p)(As + q * D::EPI), 4, 0, 0);
-  }
+    for (int u = 0; u < BigCfg<T>::EPT; ++u) v[u] = *(gptr<T>)(base + (o.off0 + u * o.ustep));
+  } else {
+    // clamped rows and k: valid addresses; the store zeroes k past K, and
+    // rows past the tile's edge only reach outputs that are never stored
 #pragma unroll
-  for (int i = 0; i < D::IPW; ++i) {
-    const int q = wave * D::IPW + i;
-    int j, k;
-    inv128(q * D::EPI + sub, v.brk, j, k);
-    const int64_t gj = min(tl.col0 + j, tl.N - 1);
-    const int64_t gk = min(k0 + k, v.K - 1);
-    const char* src = v.B + (gj * v.brs + gk * v.bks) * (int64_t)sizeof(T) + half;
-    __builtin_amdgcn_global_load_lds((gvoid_p)src, (lvoid_p)(Bs + q * D::EPI), 4, 0, 0);
+    for (int u = 0; u < BigCfg<T>::EPT; ++u) {
+      const int rr = min(big_row<T>(o.kc, tid, u), o.rmax) + o.r0;
+      const int kk = min(k0 + big_k<T>(o.kc, tid, u), K - 1);
+      v[u] = *(gptr<T>)(o.p + (uint32_t)(rr * o.rs + kk * o.ks) * (uint32_t)sizeof(T));
+    }
   }
 }
 
-// s_waitcnt vmcnt(N) with the other counters left alone (gfx9 encoding)
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-
-// s_waitcnt vmcnt(N) lgkmcnt(0): this wave's DMA up to the last N
-// instructions and all of its LDS reads are done
-template <int N>
-__device__ __forceinline__ void wait_vm_lds() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
-}
-
-// the wave's 32x64 block of one stage: rows wm + [0, 32), columns wn + [0, 64)
+// Registers to an LDS stage operand S; alpha and the K bound applied only
+// where the slice needs them.
 template <typename T>
-__device__ __forceinline__ void mma128(const T* As, const T* Bs, bool ark, bool brk, T alpha, int kval, int wm,
-                                       int wn, int fr, int fk, typename Mfma<T>::acc_t (&acc)[2][4]) {
+__device__ __forceinline__ void big_store(lptr<T> S, const T (&v)[BigCfg<T>::EPT], bool kc, int kval, T scale,
+                                          int tid) {
+  constexpr int EPT = BigCfg<T>::EPT, KS = BigCfg<T>::KC_STEP, RS = BigCfg<T>::RC_KSTEP * BigCfg<T>::P;
+  const int k0t = big_k<T>(kc, tid, 0);
+  lptr<T> d = S + big_rowstart<T>(k0t) + big_row<T>(kc, tid, 0);
+  if (kval >= BigCfg<T>::BK && scale == T(1)) {
+    if (kc) {
 #pragma unroll
-  for (int kk = 0; kk < GBK; kk += 4) {
-    const int k = kk + fk;
-    const T sc = k < kval ? alpha : T(0);
-    T a[2], b[4];
+      for (int u = 0; u < EPT; ++u) d[u * KS] = v[u];
+    } else {
 #pragma unroll
-    for (int m = 0; m < 2; ++m) a[m] = sc * As[off128(wm + 16 * m + fr, k, ark)];
+      for (int u = 0; u < EPT; ++u) d[u * RS] = v[u];
+    }
+  } else if (kc) {
 #pragma unroll
-    for (int n = 0; n < 4; ++n) b[n] = Bs[off128(wn + 16 * n + fr, k, brk)];
+    for (int u = 0; u < EPT; ++u) d[u * KS] = k0t < kval ? scale * v[u] : T(0);
+  } else {
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
-#pragma unroll
-      for (int n = 0; n < 4; ++n) acc[m][n] = Mfma<T>::mma(a[m], b[n], acc[m][n]);
+    for (int u = 0; u < EPT; ++u) d[u * RS] = k0t + BigCfg<T>::RC_KSTEP * u < kval ? scale * v[u] : T(0);
   }
 }
 
 template <typename T>
-__device__ __forceinline__ void gemm_tile128(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
-                                             int ntiles, const Bases& bs, const SplitK& sk) {
+__device__ __forceinline__ void big_mma(lptr<T> As, lptr<T> Bs, int wm, int wn, int fr, int fk,
+                                        typename Mfma<T>::acc_t (&acc)[4][GB_NB]) {
+  constexpr int P = BigCfg<T>::P;
+  // k = s + fk with s a multiple of 4: every k-step reads at this lane's
+  // bases plus s*P
+  lptr<T> Ak = As + big_rowstart<T>(fk) + wm + fr;
+  lptr<T> Bk = Bs + big_rowstart<T>(fk) + wn + fr;
+#pragma unroll
+  for (int s = 0; s < BigCfg<T>::BK; s += 4) {
+    T a[4], b[GB_NB];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) a[m] = Ak[s * P + 16 * m];
+#pragma unroll
+    for (int n = 0; n < GB_NB; ++n) b[n] = Bk[s * P + 16 * n];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = 0; n < GB_NB; ++n) acc[m][n] = Mfma<T>::mma(a[m], b[n], acc[m][n]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void gemm_big(const GemmTile* __restrict__ tiles, const GemmSeg* __restrict__ segs,
+                                         const BigItem& it, const Bases& bs, void* part, unsigned* cnt,
+                                         lptr<T> smem) {
   using acc_t = typename Mfma<T>::acc_t;
-  using D = Dma128<T>;
-  constexpr int NDMA = 2 * D::IPW;  // DMA instructions per wave and slice
-  // one LDS object per stage and operand (the compiler tells their DMA apart)
-  __shared__ T As0[G128 * GBK], As1[G128 * GBK], As2[G128 * GBK];
-  __shared__ T Bs0[G128 * GBK], Bs1[G128 * GBK], Bs2[G128 * GBK];
-  __shared__ double dred[G128_NW];
-  __shared__ int last_sh;
-  const int item = xcd_tile(ntiles * sk.nsplit);
-  if (item < 0) return;
-  const int t = item / sk.nsplit, part = item - t * sk.nsplit;
-  const GemmTile tl = tiles[t];
+  constexpr int BK = BigCfg<T>::BK, STG = BigCfg<T>::STAGE;
+  const GemmTile tl = tiles[it.tile];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  const int wm = (wave / GB_WC) * 64, wn = (wave % GB_WC) * GB_WN;
   const int fr = lane & 15, fk = lane >> 4;
-  acc_t acc[2][4];
+  // stage st: A at smem + 2 st STG, B after it
+  acc_t acc[4][GB_NB];
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n) acc[m][n] = acc_t{0, 0, 0, 0};
-  // this part's slices [c0, c1) of the chain
-  const int send = tl.seg0 + tl.nseg;
-  int total = 0;
-  for (int q = tl.seg0; q < send; ++q) total += (segs[q].K + GBK - 1) / GBK;
-  const int c0 = (int)((int64_t)total * part / sk.nsplit);
-  const int c1 = (int)((int64_t)total * (part + 1) / sk.nsplit);
+    for (int n = 0; n < GB_NB; ++n) acc[m][n] = acc_t{0, 0, 0, 0};
+  const int c0 = it.c0, c1 = it.c1;
   if (c0 < c1) {
-    // DMA cursor (segment s, slice in within it) at slice c0
+    // load cursor: segment s, slice `in` within it
+    const int send = tl.seg0 + tl.nseg;
     int s = tl.seg0, in = c0;
+    GemmSeg g = segs[s];
     for (;;) {
-      const int ns = (segs[s].K + GBK - 1) / GBK;
+      const int ns = (g.K + BK - 1) / BK;
       if (in < ns) break;
       in -= ns;
-      ++s;
+      g = segs[++s];
     }
-    View128 v;
-    view128<T>(segs[s], bs, v);
-    // per stage: layout, alpha and valid k count of the slice it holds
-    bool sark[G128_NST], sbrk[G128_NST];
-    T salpha[G128_NST];
-    int skval[G128_NST];
-    auto issue = [&](auto stc) {  // the slice at the cursor into stage ST, then advance the cursor
-      constexpr int ST = decltype(stc)::value;
-      T* As = ST == 0 ? As0 : ST == 1 ? As1 : As2;
-      T* Bs = ST == 0 ? Bs0 : ST == 1 ? Bs1 : Bs2;
-      const int k0 = in * GBK;
-      dma128<T>(v, tl, k0, wave, lane, As, Bs);
-      sark[ST] = v.ark;
-      sbrk[ST] = v.brk;
-      salpha[ST] = static_cast<T>(v.alpha);
-      skval[ST] = v.K - k0;
+    BigOp<T> oa, ob;
+    big_view<T>(g, bs, tl, tid, oa, ob);
+    BigSlice<T> rg;
+    // the slice at the cursor into registers (a cursor past the chain's end
+    // reads clamped, valid addresses: the loads never depend on a branch),
+    // then the cursor moves on
+    auto issue = [&]() {
+      const int k0 = in * BK;
+      big_load<T>(oa, k0, g.K, tid, rg.a);
+      big_load<T>(ob, k0, g.K, tid, rg.b);
+      rg.akc = oa.kc;
+      rg.bkc = ob.kc;
+      rg.kval = g.K - k0;
+      rg.alpha = static_cast<T>(g.alpha);
       ++in;
-      if (in * GBK >= v.K && s + 1 < send) {
-        ++s;
+      if (in * BK >= g.K && s + 1 < send) {
+        g = segs[++s];
         in = 0;
-        view128<T>(segs[s], bs, v);
+        big_view<T>(g, bs, tl, tid, oa, ob);
       }
     };
-    issue(std::integral_constant<int, 0>{});
-    if (c0 + 1 < c1) issue(std::integral_constant<int, 1>{});
-    // slice c in stage ST: wait for its DMA (slice c+1's may stay in
-    // flight) and for this wave's reads of slice c-1, meet (a bare barrier:
-    // a fenced one would drain slice c+1's DMA too), put slice c+2 in
-    // flight into the stage slice c-1 used, compute
-    auto step = [&](int c, auto stc) {
-      constexpr int ST = decltype(stc)::value;
-      constexpr int NX = (ST + 2) % G128_NST;
-      if (c + 1 < c1) wait_vm_lds<NDMA>();
-      else wait_vm_lds<0>();
-      __builtin_amdgcn_s_barrier();
-      if (c + 2 < c1) issue(std::integral_constant<int, NX>{});
-      const T* As = ST == 0 ? As0 : ST == 1 ? As1 : As2;
-      const T* Bs = ST == 0 ? Bs0 : ST == 1 ? Bs1 : Bs2;
-      mma128<T>(As, Bs, sark[ST], sbrk[ST], salpha[ST], skval[ST], wm, wn, fr, fk, acc);
+    auto store = [&](int st) {
+      big_store<T>(smem + 2 * st * STG, rg.a, rg.akc, rg.kval, rg.alpha, tid);
+      big_store<T>(smem + (2 * st + 1) * STG, rg.b, rg.bkc, rg.kval, T(1), tid);
     };
-    for (int c = c0; c < c1; c += G128_NST) {
-      step(c, std::integral_constant<int, 0>{});
-      if (c + 1 < c1) step(c + 1, std::integral_constant<int, 1>{});
-      if (c + 2 < c1) step(c + 2, std::integral_constant<int, 2>{});
+    issue();  // slice c0 to stage 0
+    store(0);
+    issue();  // slice c0 + 1
+    __syncthreads();
+    // iteration c: slice c+1 from registers to the free stage, slice c+2's
+    // loads in flight behind slice c's MFMAs, one barrier (unrolled by two,
+    // so that the stage offsets are constants)
+    auto step = [&](int c, int st) {
+      if (c + 1 < c1) store(st ^ 1);
+      issue();
+      big_mma<T>(smem + 2 * st * STG, smem + (2 * st + 1) * STG, wm, wn, fr, fk, acc);
+      __syncthreads();
+    };
+    for (int c = c0; c < c1; c += 2) {
+      step(c, 0);
+      if (c + 1 < c1) step(c + 1, 1);
     }
   }
-  if (sk.nsplit > 1) {
-    // partial tile, lane-major per register: part[((t * nsplit + p) * NW + wave) * 2048 + r * 64 + lane]
-    T* P = static_cast<T*>(sk.part);
-    T* mine = P + ((int64_t)(t * sk.nsplit + part) * G128_NW + wave) * 2048;
+  __shared__ double dred[GB_NW];
+  if (it.nsplit > 1) {
+    // partial tile: part[(pslot * NW + wave) * 64 * REGS + reg * 64 + lane]
+    T* P = static_cast<T*>(part);
+    T* mine = P + ((int64_t)it.pslot * GB_NW + wave) * (64 * GB_REGS);
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int n = 0; n < 4; ++n)
+      for (int n = 0; n < GB_NB; ++n)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) mine[((m * 4 + n) * 4 + q) * 64 + lane] = acc[m][n][q];
+        for (int q = 0; q < 4; ++q) mine[((m * GB_NB + n) * 4 + q) * 64 + lane] = acc[m][n][q];
     // every wave's stores done, the workgroup meets, one thread releases them
     // at agent scope with its arrival; the last arrival acquires
-    wait_vm<0>();
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    __shared__ int last_sh;
     if (tid == 0) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      const bool last = __hip_atomic_fetch_add(sk.cnt + t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                        (unsigned)(sk.nsplit - 1);
+      const bool last = __hip_atomic_fetch_add(cnt + it.tile, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                        (unsigned)(it.nsplit - 1);
       if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       last_sh = last;
     }
     __syncthreads();
     if (!last_sh) return;
-    // the sum in part order
+    // the sum in part order (the tile's parts hold slots pslot - part .. )
+    const int64_t slot0 = it.pslot - it.part;
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
+      for (int n = 0; n < GB_NB; ++n) {
         acc_t sum = acc_t{0, 0, 0, 0};
-        for (int p = 0; p < sk.nsplit; ++p) {
-          const T* src = P + ((int64_t)(t * sk.nsplit + p) * G128_NW + wave) * 2048 + (m * 4 + n) * 256 + lane;
+        for (int p = 0; p < it.nsplit; ++p) {
+          const T* src = P + ((slot0 + p) * GB_NW + wave) * (64 * GB_REGS) + (m * GB_NB + n) * 256 + lane;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const T x = p == part ? acc[m][n][q] : src[q * 64];
+            const T x = p == it.part ? acc[m][n][q] : src[q * 64];
             sum[q] = p == 0 ? x : sum[q] + x;
           }
         }
         acc[m][n] = sum;
       }
-    if (tid == 0) sk.cnt[t] = 0u;
+    if (tid == 0) cnt[it.tile] = 0u;
   }
   T* C = const_cast<T*>(resolve_c<T>(tl.cbase, tl.c, tl.coff, bs));
   const T* X = static_cast<const T*>(bs.x) + tl.coff;  // read only with bs.dot (Y and x share offsets)
   double dsum = 0.0;
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int n = 0; n < 4; ++n)
+    for (int n = 0; n < GB_NB; ++n)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         // accumulator item q of lane l holds D[row][l%16] with
@@ -704,27 +741,61 @@ __device__ __forceinline__ void gemm_tile128(const GemmTile* __restrict__ tiles,
           if (bs.dot) dsum += (double)out * (double)X[(size_t)row * tl.ldc + col];
         }
       }
-  if (bs.dot) {  // four partials of x^T y per tile (waves 2q and 2q+1 to slot q), fixed order
-    const double rr = wave_sum(dsum);
-    if (lane == 0) dred[wave] = rr;
+  if (bs.dot) {  // four partials of x^T y per tile (the waves' sums in fixed groups of GB_NW/4)
+    const double r = wave_sum(dsum);
+    if (lane == 0) dred[wave] = r;
     __syncthreads();
-    if (tid < 4) bs.dot[4 * t + tid] = dred[2 * tid] + dred[2 * tid + 1];
+    if (tid < 4) {
+      double v = 0.0;
+      for (int w = 0; w < GB_NW / 4; ++w) v += dred[tid * (GB_NW / 4) + w];
+      bs.dot[4 * it.tile + tid] = v;
+    }
   }
+}
+
+// Workgroup b of a launch with nb work items dealt so that each XCD
+// (workgroups go to the 8 XCDs round-robin) receives a contiguous run of
+// the item list; -1 for padding.
+__device__ __forceinline__ int xcd_item(int b, int nitems) {
+  const int per = (nitems + 7) >> 3;
+  const int t = (b & 7) * per + (b >> 3);
+  return t < nitems ? t : -1;
 }
 
 // One kernel symbol per use so that traces separate them: the two H*v
 // phases (W: Psi.B^T per term, Y: per-sector accumulation) and the generic
 // grouped GEMM (rho, rotations); the _t64 symbols are the 64x64 LDS-staged
-// variant of each.
+// variant of each, the _t128 symbols the launch with large outputs: its
+// first workgroups run the 128x128 work items, the rest the 32x32 tiles of
+// the plan's other outputs, all sharing one LDS buffer.
 #define NSS_GEMM_KERNEL(name, T, SKIPLOAD)                                                              \
   __global__ __launch_bounds__(256) void name(const GemmTile* __restrict__ tiles,                       \
                                               const GemmSeg* __restrict__ segs, int ntiles, Bases bs) { \
-    gemm_tile32<T, NSS_NBUF, SKIPLOAD>(tiles, segs, ntiles, bs);                                        \
+    __shared__ T red[4][32][33];                                                                        \
+    const int t = xcd_tile(ntiles);                                                                     \
+    if (t >= 0) gemm_tile32<T, NSS_NBUF, SKIPLOAD>(tiles, segs, t, bs, red);                            \
   }                                                                                                     \
-  __global__ __launch_bounds__(G128_THREADS) void name##_t128(const GemmTile* __restrict__ tiles,       \
-                                                        const GemmSeg* __restrict__ segs, int ntiles,   \
-                                                        Bases bs, SplitK sk) {                          \
-    gemm_tile128<T>(tiles, segs, ntiles, bs, sk);                                                       \
+  __global__ __launch_bounds__(GB_THREADS, 2) void name##_t128(const GemmTile* __restrict__ tiles,      \
+                                                               const GemmSeg* __restrict__ segs,        \
+                                                               int ntiles, Bases bs, BigArgs ba) {      \
+    __shared__ __attribute__((aligned(16))) char smem[GB_LDS_BYTES];                                    \
+    const int nb8 = 8 * ((ba.nitems + 7) / 8);                                                          \
+    if ((int)blockIdx.x < nb8) {                                                                        \
+      const int i = xcd_item(blockIdx.x, ba.nitems);                                                    \
+      if (i >= 0) gemm_big<T>(tiles, segs, ba.items[i], bs, ba.part, ba.cnt, (lptr<T>)(smem));          \
+    } else {                                                                                            \
+      /* GB_NW / 4 tiles of 32x32 per workgroup, four waves each; a missing */                          \
+      /* last tile repeats the one before without storing, so every wave */                            \
+      /* meets the same barriers */                                                                     \
+      const int nsm = ntiles - ba.nbig, tpw = GB_NW / 4;                                                \
+      const int pair = xcd_item(blockIdx.x - nb8, (nsm + tpw - 1) / tpw);                               \
+      if (pair >= 0) {                                                                                  \
+        const int h = threadIdx.x >> 8, t = pair * tpw + h;                                             \
+        gemm_tile32<T, 1, SKIPLOAD>(tiles, segs, ba.nbig + min(t, nsm - 1), bs,                         \
+                                           reinterpret_cast<T(*)[32][33]>(smem) + 4 * h,                \
+                                           threadIdx.x & 255, t < nsm);                                 \
+      }                                                                                                 \
+    }                                                                                                   \
   }                                                                                                     \
   __global__ __launch_bounds__(256) void name##_t64(const GemmTile* __restrict__ tiles,                 \
                                                     const GemmSeg* __restrict__ segs, int ntiles,       \
@@ -732,6 +803,11 @@ __device__ __forceinline__ void gemm_tile128(const GemmTile* __restrict__ tiles,
     const int t = xcd_tile(ntiles);                                                                     \
     if (t >= 0) gemm_tile<T>(tiles + t, segs, bs, t);                                                   \
   }
+static_assert((GB_NW / 4) * 4 * 32 * 33 * sizeof(double) <= GB_LDS_BYTES, "32x32 reduction buffers exceed the LDS");
+static_assert(4 * BigCfg<double>::STAGE * sizeof(double) <= GB_LDS_BYTES &&
+                  4 * BigCfg<float>::STAGE * sizeof(float) <= GB_LDS_BYTES,
+              "two stages of both operands exceed the LDS");
+static_assert(GB_NW % 4 == 0 && GB_NW <= 8, "t128 workgroups run GB_NW/4 small tiles of four waves");
 // W and the generic GEMM (rho, rotations) have short K chains; Y's chain
 // runs over every term of its sector.
 NSS_GEMM_KERNEL(k_hv_w_f64, double, true)
@@ -742,7 +818,7 @@ NSS_GEMM_KERNEL(k_hv_w_f32, float, true)
 NSS_GEMM_KERNEL(k_hv_y_f32, float, false)
 
 typedef void (*gemm_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases);
-typedef void (*gemm128_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases, SplitK);
+typedef void (*gemm128_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases, BigArgs);
 
 // With start/stop events the launch records them itself (the dispatch's own
 // begin and end timestamps, hipExtLaunchKernelGGL): no marker packets
@@ -750,12 +826,14 @@ typedef void (*gemm128_kernel_t)(const GemmTile*, const GemmSeg*, int, Bases, Sp
 static void launch_plan(const GemmPlan& p, Bases b, hipStream_t s, gemm_kernel_t k32, gemm_kernel_t k64,
                         gemm128_kernel_t k128, hipEvent_t ev0, hipEvent_t ev1) {
   const int n = (int)p.htiles.size();
-  if (p.tile == 128) {
-    const SplitK sk{p.nsplit, p.nsplit > 1 ? p.part.p : nullptr, p.nsplit > 1 ? p.cnt.p : nullptr};
-    const unsigned grid = 8u * (unsigned)((n * p.nsplit + 7) / 8);
+  if (p.nbig > 0) {
+    const int nitems = (int)p.hbig.size();
+    const BigArgs ba{p.big.p, nitems, p.nbig, p.part.p, p.cnt.p};
+    const int npairs = (n - p.nbig + GB_NW / 4 - 1) / (GB_NW / 4);
+    const unsigned grid = 8u * (unsigned)((nitems + 7) / 8) + 8u * (unsigned)((npairs + 7) / 8);
     if (ev0)
-      hipExtLaunchKernelGGL(k128, dim3(grid), dim3(G128_THREADS), 0, s, ev0, ev1, 0, p.tiles.p, p.segs.p, n, b, sk);
-    else hipLaunchKernelGGL(k128, dim3(grid), dim3(G128_THREADS), 0, s, p.tiles.p, p.segs.p, n, b, sk);
+      hipExtLaunchKernelGGL(k128, dim3(grid), dim3(GB_THREADS), 0, s, ev0, ev1, 0, p.tiles.p, p.segs.p, n, b, ba);
+    else hipLaunchKernelGGL(k128, dim3(grid), dim3(GB_THREADS), 0, s, p.tiles.p, p.segs.p, n, b, ba);
     NSS_HIP(hipGetLastError());
     return;
   }

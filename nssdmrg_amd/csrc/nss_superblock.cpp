@@ -99,7 +99,8 @@ int gemm_default_tile() {
   return (v == 32 || v == 64 || v == 128) ? v : 0;
 }
 
-// NSS_GEMM_SPLIT = 1..16 forces the K split of 128x128 launches.
+// NSS_GEMM_SPLIT = 1..16 forces the number of parts of every 128x128 tile
+// of the large-output path (capped at its slice count).
 static int gemm_forced_split() {
   const char* e = std::getenv("NSS_GEMM_SPLIT");
   const int v = e ? std::atoi(e) : 0;
@@ -136,42 +137,71 @@ void GemmPlan::add_output(int cbase, void* c, int64_t coff, int ldc, int M, int 
 // ones the split K chain of the 32x32 kernel does.
 static constexpr int64_t GEMM_T64_MIN_TILES = 2048;
 
-// The 128x128 kernel runs only when NSS_GEMM_TILE=128 asks for it: on the
-// FP64 H*v launches of the spin-1 m=1024 workload it measured slower than
-// the 32x32 / 64x64 pair (Y 493 vs 347 us, W 281 vs 187 us), and on plain
-// FP64 GEMMs no faster than 64x64 (37.5 vs 41.5 TFLOP/s at 4096x4096x2048);
-// in FP32 it leads on large products (72 vs 62 TFLOP/s).  Its K chains are
-// split until about two workgroups per CU are busy.
-static constexpr int GEMM_T128_TARGET_WG = 512;
-
-void GemmPlan::upload(DevicePool* pool, hipStream_t s) {
-  int64_t n32 = 0;
-  for (const GemmTile& o : outs) n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
-  tile = gemm_default_tile() ? gemm_default_tile() : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
-  nsplit = 1;
-  if (tile == 128) {
-    int64_t n128 = 0, slices = 0;
-    for (const GemmTile& o : outs) {
-      const int64_t nt = (int64_t)((o.M + 127) / 128) * ((o.N + 127) / 128);
-      int64_t sl = 0;
-      for (int q = o.seg0; q < o.seg0 + o.nseg; ++q) sl += (hsegs[q].K + 15) / 16;
-      n128 += nt;
-      slices += nt * sl;
-    }
-    if (n128 > 0) {
-      const int64_t want = (GEMM_T128_TARGET_WG + n128 - 1) / n128;
-      const int64_t cap = std::max<int64_t>(1, slices / n128 / 4);  // at least 4 slices per part on average
-      nsplit = gemm_forced_split() ? gemm_forced_split()
-                                   : (int)std::max<int64_t>(1, std::min<int64_t>({want, cap, 8}));
-      if (nsplit > 1) {
-        part.resize(pool, (size_t)n128 * nsplit * 16384);
-        cnt.resize(pool, (size_t)n128);
-        NSS_HIP(hipMemsetAsync(cnt.p, 0, (size_t)n128 * sizeof(unsigned), s));
-      }
-    }
+// Element offsets of a segment's operands within the output's tiles fit the
+// large-output path's 32-bit arithmetic.
+static bool big_offsets_fit(const GemmTile& o, const std::vector<GemmSeg>& segs) {
+  for (int q = o.seg0; q < o.seg0 + o.nseg; ++q) {
+    const GemmSeg& g = segs[q];
+    const bool ta = g.flags & SEG_TA, tb = g.flags & SEG_TB;
+    const int64_t a = (int64_t)(o.M - 1) * (ta ? 1 : g.lda) + (int64_t)(g.K - 1) * (ta ? g.lda : 1);
+    const int64_t b = (int64_t)(o.N - 1) * (tb ? g.ldb : 1) + (int64_t)(g.K - 1) * (tb ? 1 : g.ldb);
+    if (a >= INT32_MAX || b >= INT32_MAX) return false;
   }
+  return true;
+}
+
+// Outputs of at least 128x128 whose 128x128 tiling pads at most 30% of
+// their area take the large-output path (k_*_t128).
+static bool big_output(const GemmTile& o) {
+  if (o.M < 128 || o.N < 128) return false;
+  const double padded = (double)((o.M + 127) / 128 * 128) * (double)((o.N + 127) / 128 * 128);
+  return padded <= 1.3 * (double)o.M * (double)o.N;
+}
+
+// The slice chains of the large-output tiles are cut into work items of
+// about equal length, about this many per launch (two per CU of the chip's
+// 256; NSS_GEMM_BIG_ITEMS overrides), at most 16 parts per tile and at least
+// 8 slices per part.
+static int64_t gemm_big_items() {
+  const char* e = std::getenv("NSS_GEMM_BIG_ITEMS");
+  const int64_t v = e ? std::atoll(e) : 0;
+  return v >= 8 ? v : 512;
+}
+
+void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
+  const int forced = gemm_default_tile();
+  const int bk = esize == 8 ? 16 : 32;  // K slice of the large-output path
+  std::vector<char> isbig(outs.size(), 0);
+  int64_t n32 = 0;
+  bool anybig = false;
+  for (size_t i = 0; i < outs.size(); ++i) {
+    const GemmTile& o = outs[i];
+    isbig[i] = (forced == 128 || (forced == 0 && big_output(o))) && big_offsets_fit(o, hsegs);
+    anybig = anybig || isbig[i];
+    if (!isbig[i]) n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
+  }
+  tile = anybig ? 32 : (forced == 32 || forced == 64) ? forced : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
   htiles.clear();
-  for (const GemmTile& o : outs)
+  hbig.clear();
+  std::vector<int64_t> tsl;  // slices of each large tile's chain
+  for (size_t i = 0; i < outs.size(); ++i) {
+    if (!isbig[i]) continue;
+    const GemmTile& o = outs[i];
+    int64_t sl = 0;
+    for (int q = o.seg0; q < o.seg0 + o.nseg; ++q) sl += (hsegs[q].K + bk - 1) / bk;
+    for (int r = 0; r < o.M; r += 128)
+      for (int cc = 0; cc < o.N; cc += 128) {
+        GemmTile t = o;
+        t.row0 = r;
+        t.col0 = cc;
+        htiles.push_back(t);
+        tsl.push_back(sl);
+      }
+  }
+  nbig = (int)htiles.size();
+  for (size_t i = 0; i < outs.size(); ++i) {
+    if (isbig[i]) continue;
+    const GemmTile& o = outs[i];
     for (int r = 0; r < o.M; r += tile)
       for (int cc = 0; cc < o.N; cc += tile) {
         GemmTile t = o;
@@ -179,6 +209,28 @@ void GemmPlan::upload(DevicePool* pool, hipStream_t s) {
         t.col0 = cc;
         htiles.push_back(t);
       }
+  }
+  if (nbig > 0) {
+    int64_t total = 0;
+    for (int64_t v : tsl) total += v;
+    const int64_t nit = gemm_big_items();
+    const int64_t len = std::max<int64_t>(8, (total + nit - 1) / nit);
+    const int fs = gemm_forced_split();
+    int pslot = 0;
+    for (int t = 0; t < nbig; ++t) {
+      const int64_t sl = tsl[t];
+      int ns = fs ? fs : (int)std::clamp<int64_t>((sl + len / 2) / len, 1, 16);
+      ns = (int)std::max<int64_t>(1, std::min<int64_t>(ns, sl));
+      for (int p = 0; p < ns; ++p)
+        hbig.push_back(BigItem{t, p, ns, (int)(sl * p / ns), (int)(sl * (p + 1) / ns), ns > 1 ? pslot + p : 0});
+      if (ns > 1) pslot += ns;
+    }
+    big.resize(pool, hbig.size());
+    NSS_HIP(hipMemcpyAsync(big.p, hbig.data(), hbig.size() * sizeof(BigItem), hipMemcpyHostToDevice, s));
+    part.resize(pool, (size_t)std::max(1, pslot) * 16384);
+    cnt.resize(pool, nbig);
+    NSS_HIP(hipMemsetAsync(cnt.p, 0, (size_t)nbig * sizeof(unsigned), s));
+  }
   tiles.resize(pool, std::max<size_t>(1, htiles.size()));
   segs.resize(pool, std::max<size_t>(1, hsegs.size()));
   if (!htiles.empty())
@@ -339,8 +391,8 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, cons
     sb->Wf.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
     sb->Xf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
     sb->Yf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
-    sb->W32.upload(ctx->pool, ctx->stream);
-    sb->Y32.upload(ctx->pool, ctx->stream);
+    sb->W32.upload(ctx->pool, ctx->stream, 4);
+    sb->Y32.upload(ctx->pool, ctx->stream, 4);
     sb->has32 = true;
   }
 }
@@ -447,8 +499,8 @@ void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int6
     plan_append(W, f32 ? sbs[c]->W32 : sbs[c]->W64, xoff[c], woff[c]);
     plan_append(Y, f32 ? sbs[c]->Y32 : sbs[c]->Y64, xoff[c], woff[c]);
   }
-  W.upload(pool, s);
-  Y.upload(pool, s);
+  W.upload(pool, s, f32 ? 4 : 8);
+  Y.upload(pool, s, f32 ? 4 : 8);
 }
 
 // H*v of merged plans, timed like sb_hv.
