@@ -157,6 +157,16 @@ int nss_host_tridiag_lowest(int n, const double* d, const double* e, double* the
  * d[4*l + r] receives accumulator item r of lane l.  a, b: 64, d: 256. */
 int nss_probe_mfma(int prec, const double* a, const double* b, double* d);
 
+/* The H*v tiling modes the autotuner has measured in this process, one row
+ * of five ints per shape class: {phase (1 W, 2 Y), operand bytes (8 or 4),
+ * log2 of its 32x32 tile count, log2 of its largest output edge, mode (1:
+ * 32x32 tiles, 2: 64x64 tiles, 3: large-output path with 256 work items, 4:
+ * with 512)}.  Up to maxrows rows go to rows (may be NULL); *count gets the
+ * number of classes.  No counterpart in the reference: it reports how this
+ * library tiles its grouped H*v launches (NSS_GEMM_AUTOTUNE=0 disables the
+ * measurement, NSS_GEMM_TILE forces a tile). */
+int nss_gemm_tune_table(int* rows, int maxrows, int* count);
+
 /* Plain device copy of n doubles (n even), timed over reps launches on the
  * null stream: *seconds = average launch time.  A probe with no counterpart
  * in the reference: it calibrates HBM byte counters and measures the copy

@@ -163,6 +163,25 @@ class Context:
         return out
 
 
+GEMM_MODES = {1: "tile32", 2: "tile64", 3: "large_items256", 4: "large_items512"}
+
+
+def gemm_tune_table() -> list:
+    """The H*v tiling modes the autotuner measured in this process, one dict
+    per shape class (phase, operand bytes, log2 of the 32x32 tile count and of
+    the largest output edge, chosen mode)."""
+    cnt = C.c_int()
+    F.check(F.lib().nss_gemm_tune_table(None, 0, C.byref(cnt)))
+    rows = (C.c_int * max(1, 5 * cnt.value))()
+    F.check(F.lib().nss_gemm_tune_table(rows, cnt.value, C.byref(cnt)))
+    out = []
+    for r in range(cnt.value):
+        k, es, lt, lm, m = rows[5 * r:5 * r + 5]
+        out.append({"phase": "W" if k == 1 else "Y", "bytes": es, "log2_tiles32": lt, "log2_max_edge": lm,
+                    "mode": GEMM_MODES.get(m, str(m))})
+    return out
+
+
 def kept_count(evals_desc, mstates: int, mdim: int, deg_thr: float, estates: float = 0.0):
     """m_s rule of renormalize_block (src/DMRG_RDM.f90:204-220) -> (m_s, trunc_err)."""
     ev = np.ascontiguousarray(evals_desc, dtype=np.float64)

@@ -89,6 +89,7 @@ SIGNATURES = {
     "nss_host_sym_eig": (C.c_int, [C.c_int, _D, _D]),
     "nss_probe_mfma": (C.c_int, [C.c_int, _D, _D, _D]),
     "nss_probe_copy": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+    "nss_gemm_tune_table": (C.c_int, [C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     "nss_dmrg_measure": (C.c_int, [C.c_void_p, C.c_int, _D, C.c_int, C.POINTER(C.c_int32), _D]),
     "nss_dmrg_load_blocks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
     "nss_dmrg_save_blocks": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),

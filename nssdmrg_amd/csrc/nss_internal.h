@@ -166,15 +166,40 @@ struct GemmPlan {
   DevArr<double> part;
   DevArr<unsigned> cnt;
   std::vector<GemmTile> outs;  // one per add_output (row0 = col0 = 0)
-  void clear() { htiles.clear(); hsegs.clear(); outs.clear(); hbig.clear(); nbig = 0; flops = 0.0; }
+  void clear() {
+    htiles.clear();
+    hsegs.clear();
+    outs.clear();
+    hbig.clear();
+    nbig = 0;
+    flops = 0.0;
+    mode = 0;
+  }
   // Adds output C (M x N, ldc) as tiles of the kernel's tile size, each with
   // the same list of segments.
   void add_output(int cbase, void* c, int64_t coff, int ldc, int M, int N,
                   const std::vector<GemmSeg>& segs_for_c, int beta = 0);
+  // How upload() tiles the plan: PLAN_AUTO by sizes (outputs of 128x128 and
+  // more on the large-output path with about 512 work items, the rest 32x32,
+  // or 64x64 for launches of many small outputs), or one of the fixed modes
+  // the autotuner measures (plan_autotune).
+  int mode = 0;
   // esize: operand bytes of the launches (8 FP64, 4 FP32), which set the K
   // slice of the large-output path's work items
   void upload(DevicePool* pool, hipStream_t s, int esize = 8);
 };
+
+enum : int { PLAN_AUTO = 0, PLAN_T32 = 1, PLAN_T64 = 2, PLAN_BIG256 = 3, PLAN_BIG512 = 4, PLAN_NMODES = 5 };
+
+// Uploads an H*v plan (kind GEMM_HV_W or GEMM_HV_Y, operands of esize
+// bytes) in the mode measured fastest for its shape class: a class seen for
+// the first time is timed in every candidate mode on the buffers of b
+// (which the trial launches overwrite); later plans of the class reuse the
+// cached mode.  Without measure, an uncached class takes the size-based mode.
+void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure);
+// The cached modes: count, and the entries as (kind, esize, log2 tiles,
+// log2 edge, mode) rows.
+int plan_tune_entries(int* rows, int maxrows);
 
 constexpr int GEMM_BM = 64, GEMM_BN = 64;
 
@@ -535,7 +560,7 @@ void sb_hv_fp32(nss_ctx* ctx, SbDevice* sb, const double* x, double* y);     // 
 // offsets), launched by hv_plans32.
 int64_t sb_wsize(const SbDevice* sb);
 void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
-                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s,
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, ::nss_ctx* ctx, hipStream_t s,
                        bool f32 = false);
 void hv_plans(::nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b);
 void hv_plans32(::nss_ctx* ctx, const GemmPlan& W, const GemmPlan& Y, Bases b);

@@ -731,7 +731,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     NSS_HIP(hipMemcpyAsync(hc[c].x, v0, n[c] * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   GemmPlan Wp, Yp;
-  sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx->pool, s);
+  sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx, s);
   // host side of every chain: the projected tridiagonal
   std::vector<std::vector<double>> d(K, std::vector<double>(kb1, 0.0)), e(K, std::vector<double>(kb1, 0.0)),
       beta(K, std::vector<double>(kb1, 0.0));
@@ -808,7 +808,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
           ax.push_back(xoff[c]);
           aw.push_back(woff[c]);
         }
-      sb_merge_hv_plans(asb, ax, aw, Wp, Yp, ctx->pool, s);
+      sb_merge_hv_plans(asb, ax, aw, Wp, Yp, ctx, s);
     }
   }
   NSS_HIP(hipStreamSynchronize(s));

@@ -6,6 +6,7 @@
 // which is the reference's evaluation order (B first, then A).
 #include <algorithm>
 #include <functional>
+#include <map>
 #include <cfloat>
 #include <chrono>
 #include <cstdlib>
@@ -169,18 +170,27 @@ static int64_t gemm_big_items() {
 }
 
 void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
-  const int forced = gemm_default_tile();
+  // NSS_GEMM_TILE overrides the plan's mode (set by the autotuner), which
+  // overrides the size-based choice
+  int forced = gemm_default_tile();
+  int64_t nit = gemm_big_items();
+  if (!forced && mode != PLAN_AUTO) {
+    forced = mode == PLAN_T32 ? 32 : mode == PLAN_T64 ? 64 : 0;
+    if (mode == PLAN_BIG256) nit = 256;
+    if (mode == PLAN_BIG512) nit = 512;
+  }
+  const bool nobig = forced == 32 || forced == 64;
   const int bk = esize == 8 ? 16 : 32;  // K slice of the large-output path
   std::vector<char> isbig(outs.size(), 0);
   int64_t n32 = 0;
   bool anybig = false;
   for (size_t i = 0; i < outs.size(); ++i) {
     const GemmTile& o = outs[i];
-    isbig[i] = (forced == 128 || (forced == 0 && big_output(o))) && big_offsets_fit(o, hsegs);
+    isbig[i] = (forced == 128 || (!nobig && big_output(o))) && big_offsets_fit(o, hsegs);
     anybig = anybig || isbig[i];
     if (!isbig[i]) n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
   }
-  tile = anybig ? 32 : (forced == 32 || forced == 64) ? forced : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
+  tile = anybig ? 32 : nobig ? forced : (n32 >= GEMM_T64_MIN_TILES ? 64 : 32);
   htiles.clear();
   hbig.clear();
   std::vector<int64_t> tsl;  // slices of each large tile's chain
@@ -213,7 +223,6 @@ void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
   if (nbig > 0) {
     int64_t total = 0;
     for (int64_t v : tsl) total += v;
-    const int64_t nit = gemm_big_items();
     const int64_t len = std::max<int64_t>(8, (total + nit - 1) / nit);
     const int fs = gemm_forced_split();
     int pslot = 0;
@@ -242,6 +251,139 @@ void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
   // host vectors must outlive the async copies: callers keep the plan alive
   // and synchronize before mutating it.
   NSS_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- autotuner
+// The tiling mode of an H*v plan is measured, not guessed: the first plan of
+// a shape class is uploaded and launched in each candidate mode (each timed
+// twice with HIP events on the context stream, from flushed caches) and the
+// fastest mode is cached for the class; the size-based mode stays unless
+// another is more than 3% faster.  Shape class: phase (W or Y),
+// operand bytes, and the number of 32x32 tiles and the largest output edge,
+// both rounded up to powers of two.  Candidates: 32x32 tiles; 64x64 tiles
+// (launches of at least 256 tiles of 32x32); the large-output path with 256
+// or 512 work items (plans with an output it takes).  NSS_GEMM_AUTOTUNE=0
+// keeps the size-based mode; NSS_GEMM_TILE forces a tile for every plan.
+namespace {
+struct TuneKey {
+  int kind, esize, lt, lm;
+  bool operator<(const TuneKey& o) const {
+    return kind != o.kind ? kind < o.kind : esize != o.esize ? esize < o.esize : lt != o.lt ? lt < o.lt : lm < o.lm;
+  }
+};
+std::mutex tune_mu;
+std::map<TuneKey, int> tune_cache;
+
+int ceil_log2(int64_t v) {
+  int r = 0;
+  while (((int64_t)1 << r) < v) ++r;
+  return r;
+}
+
+bool autotune_enabled() {
+  const char* e = std::getenv("NSS_GEMM_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+}  // namespace
+
+void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure) {
+  hipStream_t s = ctx->stream;
+  auto launch = [&]() {
+    if (esize == 8) launch_gemm_f64(p, b, s, kind);
+    else launch_gemm_f32(p, b, s, kind);
+  };
+  if (gemm_default_tile() || !autotune_enabled() || p.outs.empty()) {
+    p.mode = PLAN_AUTO;
+    p.upload(ctx->pool, s, esize);
+    return;
+  }
+  int64_t n32 = 0;
+  int mx = 0;
+  bool anybig = false;
+  for (const GemmTile& o : p.outs) {
+    n32 += (int64_t)((o.M + 31) / 32) * ((o.N + 31) / 32);
+    mx = std::max(mx, std::max(o.M, o.N));
+    anybig = anybig || (big_output(o) && big_offsets_fit(o, p.hsegs));
+  }
+  const TuneKey key{kind, esize, ceil_log2(n32), ceil_log2(mx)};
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tune_cache.find(key);
+    if (it != tune_cache.end() || !measure) {
+      p.mode = it != tune_cache.end() ? it->second : PLAN_AUTO;
+      p.upload(ctx->pool, s, esize);
+      return;
+    }
+  }
+  std::vector<int> cand{PLAN_T32};
+  if (n32 >= 256) cand.push_back(PLAN_T64);
+  if (anybig) {
+    cand.push_back(PLAN_BIG256);
+    cand.push_back(PLAN_BIG512);
+  }
+  // the size-based mode, kept unless another is measurably faster
+  const int dflt = anybig ? PLAN_BIG512 : n32 >= GEMM_T64_MIN_TILES ? PLAN_T64 : PLAN_T32;
+  int best = dflt;
+  if (cand.size() > 1) {
+    // each candidate timed from cold caches, as an H*v inside a Lanczos
+    // iteration runs (the vector work between calls evicts its operands):
+    // a 512 MiB fill between the launches flushes L2 and the MALL
+    constexpr size_t FLUSH = (size_t)512 << 20;
+    DevArr<char> flush;
+    flush.resize(ctx->pool, FLUSH);
+    hipEvent_t e0, e1;
+    NSS_HIP(hipEventCreate(&e0));
+    NSS_HIP(hipEventCreate(&e1));
+    std::vector<float> t(cand.size(), 0.f);
+    for (int rep = 0; rep < 2; ++rep)
+      for (size_t c = 0; c < cand.size(); ++c) {
+        p.mode = cand[c];
+        p.upload(ctx->pool, s, esize);
+        NSS_HIP(hipMemsetAsync(flush.p, rep, FLUSH, s));
+        NSS_HIP(hipEventRecord(e0, s));
+        launch();
+        NSS_HIP(hipEventRecord(e1, s));
+        NSS_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        NSS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        t[c] += ms;
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    float td = -1.f, tb = 0.f;
+    int bi = 0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+      if (cand[c] == dflt) td = t[c];
+      if (c == 0 || t[c] < tb) {
+        tb = t[c];
+        bi = (int)c;
+      }
+    }
+    if (!(td >= 0.f && td <= 1.03f * tb)) best = cand[bi];
+  }
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache[key] = best;
+  }
+  p.mode = best;
+  p.upload(ctx->pool, s, esize);
+}
+
+int plan_tune_entries(int* rows, int maxrows) {
+  std::lock_guard<std::mutex> lk(tune_mu);
+  int r = 0;
+  for (const auto& kv : tune_cache) {
+    if (rows && r < maxrows) {
+      int* o = rows + 5 * r;
+      o[0] = kv.first.kind;
+      o[1] = kv.first.esize;
+      o[2] = kv.first.lt;
+      o[3] = kv.first.lm;
+      o[4] = kv.second;
+    }
+    ++r;
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------- superblock
@@ -383,16 +525,28 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, cons
   build_hv_plans(sb, HL, HR, terms, false, sb->W64, sb->Y64);
   sb->flops = sb->W64.flops + sb->Y64.flops;
   sb->W.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
-  sb->W64.upload(ctx->pool, ctx->stream);
-  sb->Y64.upload(ctx->pool, ctx->stream);
+  {
+    // the autotuner's trial launches read x and W and write W and y: scratch
+    // vectors, zeroed (the operator blocks are the superblock's own)
+    DevArr<double> tx, ty;
+    tx.resize(ctx->pool, std::max<int64_t>(1, sb->n));
+    ty.resize(ctx->pool, std::max<int64_t>(1, sb->n));
+    NSS_HIP(hipMemsetAsync(tx.p, 0, std::max<int64_t>(1, sb->n) * sizeof(double), ctx->stream));
+    const Bases tb{tx.p, sb->W.p, ty.p};
+    plan_autotune(ctx, sb->W64, GEMM_HV_W, 8, tb, true);
+    plan_autotune(ctx, sb->Y64, GEMM_HV_Y, 8, tb, true);
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
+  }
   sb->has32 = false;
   if (want_fp32) {
     build_hv_plans(sb, HL, HR, terms, true, sb->W32, sb->Y32);
     sb->Wf.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
     sb->Xf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
     sb->Yf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
-    sb->W32.upload(ctx->pool, ctx->stream, 4);
-    sb->Y32.upload(ctx->pool, ctx->stream, 4);
+    NSS_HIP(hipMemsetAsync(sb->Xf.p, 0, std::max<int64_t>(1, sb->n) * sizeof(float), ctx->stream));
+    const Bases tb{sb->Xf.p, sb->Wf.p, sb->Yf.p};
+    plan_autotune(ctx, sb->W32, GEMM_HV_W, 4, tb, true);
+    plan_autotune(ctx, sb->Y32, GEMM_HV_Y, 4, tb, true);
     sb->has32 = true;
   }
 }
@@ -490,7 +644,7 @@ static void plan_append(GemmPlan& dst, const GemmPlan& src, int64_t xoff, int64_
 }
 
 void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int64_t>& xoff,
-                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, DevicePool* pool, hipStream_t s,
+                       const std::vector<int64_t>& woff, GemmPlan& W, GemmPlan& Y, nss_ctx* ctx, hipStream_t s,
                        bool f32) {
   W.clear();
   Y.clear();
@@ -499,8 +653,11 @@ void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int6
     plan_append(W, f32 ? sbs[c]->W32 : sbs[c]->W64, xoff[c], woff[c]);
     plan_append(Y, f32 ? sbs[c]->Y32 : sbs[c]->Y64, xoff[c], woff[c]);
   }
-  W.upload(pool, s, f32 ? 4 : 8);
-  Y.upload(pool, s, f32 ? 4 : 8);
+  // tiling modes from the autotuner's cache (a merged plan of one chain has
+  // that chain's shape class), size-based for classes not measured yet
+  (void)s;
+  plan_autotune(ctx, W, GEMM_HV_W, f32 ? 4 : 8, Bases{}, false);
+  plan_autotune(ctx, Y, GEMM_HV_Y, f32 ? 4 : 8, Bases{}, false);
 }
 
 // H*v of merged plans, timed like sb_hv.
@@ -1550,6 +1707,13 @@ extern "C" int nss_sb_hv_device_noalloc(nss_ctx* ctx, const double* d_v, double*
   const Bases b{d_v, ctx->noalloc_scratch, d_hv};
   launch_gemm_f64(sb->W64, b, ctx->stream, GEMM_HV_W);
   launch_gemm_f64(sb->Y64, b, ctx->stream, GEMM_HV_Y);
+  NSS_API_END
+}
+
+extern "C" int nss_gemm_tune_table(int* rows, int maxrows, int* count) {
+  NSS_API_BEGIN
+  NSS_REQUIRE(count && maxrows >= 0, NSS_E_ARG, "nss_gemm_tune_table: NULL count or negative maxrows");
+  *count = nss::plan_tune_entries(rows, maxrows);
   NSS_API_END
 }
 
