@@ -159,9 +159,8 @@ int nss_probe_mfma(int prec, const double* a, const double* b, double* d);
 
 /* The H*v tiling modes the autotuner has measured in this process, one row
  * of five ints per shape class: {phase (1 W, 2 Y), operand bytes (8 or 4),
- * log2 of its 32x32 tile count, log2 of its largest output edge, mode (1:
- * 32x32 tiles, 2: 64x64 tiles, 3: large-output path with 256 work items, 4:
- * with 512)}.  Up to maxrows rows go to rows (may be NULL); *count gets the
+ * ceil(4 log2) of its 32x32 tile count and of its largest output edge, mode (1:
+ * 32x32 tiles, 2: 64x64 tiles, 3: large-output path of 128x128 tiles)}.  Up to maxrows rows go to rows (may be NULL); *count gets the
  * number of classes.  No counterpart in the reference: it reports how this
  * library tiles its grouped H*v launches (NSS_GEMM_AUTOTUNE=0 disables the
  * measurement, NSS_GEMM_TILE forces a tile). */

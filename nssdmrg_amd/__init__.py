@@ -163,13 +163,13 @@ class Context:
         return out
 
 
-GEMM_MODES = {1: "tile32", 2: "tile64", 3: "large_items256", 4: "large_items512"}
+GEMM_MODES = {1: "tile32", 2: "tile64", 3: "large_output"}
 
 
 def gemm_tune_table() -> list:
     """The H*v tiling modes the autotuner measured in this process, one dict
-    per shape class (phase, operand bytes, log2 of the 32x32 tile count and of
-    the largest output edge, chosen mode)."""
+    per shape class (phase, operand bytes, 32x32 tile count and largest output
+    edge in quarter octaves, i.e. ceil(4 log2), chosen mode)."""
     cnt = C.c_int()
     F.check(F.lib().nss_gemm_tune_table(None, 0, C.byref(cnt)))
     rows = (C.c_int * max(1, 5 * cnt.value))()
@@ -177,7 +177,7 @@ def gemm_tune_table() -> list:
     out = []
     for r in range(cnt.value):
         k, es, lt, lm, m = rows[5 * r:5 * r + 5]
-        out.append({"phase": "W" if k == 1 else "Y", "bytes": es, "log2_tiles32": lt, "log2_max_edge": lm,
+        out.append({"phase": "W" if k == 1 else "Y", "bytes": es, "tiles32_4log2": lt, "max_edge_4log2": lm,
                     "mode": GEMM_MODES.get(m, str(m))})
     return out
 

@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/nssdmrg.h"
@@ -189,14 +190,16 @@ struct GemmPlan {
   void upload(DevicePool* pool, hipStream_t s, int esize = 8);
 };
 
-enum : int { PLAN_AUTO = 0, PLAN_T32 = 1, PLAN_T64 = 2, PLAN_BIG256 = 3, PLAN_BIG512 = 4, PLAN_NMODES = 5 };
+enum : int { PLAN_AUTO = 0, PLAN_T32 = 1, PLAN_T64 = 2, PLAN_BIG = 3, PLAN_NMODES = 4 };
 
 // Uploads an H*v plan (kind GEMM_HV_W or GEMM_HV_Y, operands of esize
 // bytes) in the mode measured fastest for its shape class: a class seen for
 // the first time is timed in every candidate mode on the buffers of b
-// (which the trial launches overwrite); later plans of the class reuse the
+// (which the trial launches overwrite), each timed launch after prelude
+// (enqueued on the context stream); later plans of the class reuse the
 // cached mode.  Without measure, an uncached class takes the size-based mode.
-void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure);
+void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure,
+                   const std::function<void()>& prelude);
 // The cached modes: count, and the entries as (kind, esize, log2 tiles,
 // log2 edge, mode) rows.
 int plan_tune_entries(int* rows, int maxrows);
@@ -343,7 +346,7 @@ struct GsCluster {     // slots s0..s1-1 of a slab, nonzero in rows rs..re-1
   int64_t ldz;
   int32_t rs, re, s0, s1;
 };
-void launch_tri_cluster_gs(const GsCluster* cl, int ncl, hipStream_t s);
+void launch_tri_cluster_gs(const GsCluster* cl, int ncl, int nrmax, hipStream_t s);
 struct BtTask {        // back-transformation of slots s0..s0+ns-1 of a slab
   const double* a;     // the reduced block (reflectors in its rows)
   const double* tau;

@@ -20,8 +20,9 @@
 //  3. On request, for chosen positions only (the kept states of a
 //     truncation): inverse iteration with T - lambda I (dstein: two solves
 //     with the dgttrf factors from a pseudo-random start), one lane per
-//     vector; modified Gram-Schmidt within clusters of eigenvalues closer
-//     than 1e-3 ||T|| (dstein's criterion), one workgroup per cluster; the
+//     vector; Gram-Schmidt within clusters of eigenvalues closer than
+//     1e-3 ||T|| (dstein's criterion), one workgroup per cluster, in panels
+//     (modified within a panel, the panel then projected out of the rest); the
 //     back-transformation x = Q z, one wave per group of vectors held in
 //     registers while the reflectors stream past.
 #include <cfloat>
@@ -37,7 +38,6 @@ constexpr int EIG_BIS = 4;     // lanes (interior points) per eigenvalue
 constexpr int EIG_PER_WG = EIG_THREADS / EIG_BIS;
 constexpr int GS_THREADS = 1024;
 constexpr int GS_WAVES = GS_THREADS / 64;
-constexpr int GS_TILE = 512;   // dot products per LDS round in the cluster Gram-Schmidt
 
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
@@ -761,14 +761,22 @@ void launch_tri_invit(const InvReq* req, int nreq, double* fac, int nmax, hipStr
   NSS_HIP(hipGetLastError());
 }
 
-// Right-looking modified Gram-Schmidt over the vectors s0..s1-1 of a cluster
-// (nonzero in rows rs..re-1 only), up to two sweeps.  Step j: h_k = z_j . z_k
-// for k >= j (waves over rows, lanes over k: the slots of a row are
-// contiguous), then z_k -= (h_k / h_j) z_j; z_j is scaled by 1/sqrt(h_j) at
-// the start of step j+1, which reads neither z_j nor its scale.
+// Block Gram-Schmidt over the vectors s0..s1-1 of a cluster (nonzero in rows
+// rs..re-1 only), up to two sweeps, one workgroup per cluster.  The vectors
+// are taken in panels of PW: a panel is copied to LDS and orthonormalised
+// there by modified Gram-Schmidt (step j: h_k = z_j . z_k for the panel's
+// k >= j, then z_k -= (h_k / h_j) z_j; z_j is scaled by 1/sqrt(h_j) in step
+// j+1, which reads neither z_j nor its scale), written back, and projected
+// out of every later vector of the cluster at once (C = P^T Z, Z -= P C,
+// waves over rows and lanes over vectors, the wave partials of C summed in
+// a fixed order), so each later vector is read twice per panel rather than
+// twice per vector.  A second sweep runs where the first removed overlaps
+// above 1e-6.
+template <int PW>
 __global__ __launch_bounds__(GS_THREADS) void k_tri_cluster_gs(const GsCluster* __restrict__ cls) {
-  __shared__ double part[GS_WAVES][64];
-  __shared__ double hb[GS_TILE];
+  extern __shared__ double gsm[];
+  __shared__ double dr[GS_WAVES][PW];
+  __shared__ double hh[PW];
   __shared__ double red[GS_WAVES];
   const GsCluster C = cls[blockIdx.x];
   const int c = C.s1 - C.s0;
@@ -776,56 +784,118 @@ __global__ __launch_bounds__(GS_THREADS) void k_tri_cluster_gs(const GsCluster* 
   const int nr = C.re - C.rs;
   const int64_t ldz = C.ldz;
   double* Z = C.z + (int64_t)C.rs * ldz + C.s0;  // row i, vector k at Z[i * ldz + k]
+  double* Pl = gsm;                                // panel: row i, column q at Pl[i * PW + q]
+  double* part = gsm + (size_t)nr * PW;           // wave partials of C: [wave][q][lane]
+  double* Cc = part + (size_t)GS_WAVES * PW * 64;  // C: [q][lane]
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  constexpr int RG = GS_THREADS / PW;  // row groups of the in-panel dot products
   for (int sweep = 0; sweep < 2; ++sweep) {
-    double inv = 1.0;
     double ovl = 0.0;
-    for (int j = 0; j < c; ++j) {
-      if (j > 0)
-        for (int i = tid; i < nr; i += GS_THREADS) Z[i * ldz + (j - 1)] *= inv;
-      // the coefficients of k = j + 1 .. c-1 in tiles of GS_TILE, h_j first
-      double hj = 0.0;
-      for (int k0 = j; k0 < c; k0 += GS_TILE) {
-        const int kt = min(GS_TILE, c - k0);
-        for (int q = 0; q < kt; q += 64) {
-          const int k = k0 + q + lane;
-          double acc = 0.0;
-          if (k < c)
-            for (int i = wv; i < nr; i += GS_WAVES) acc += Z[i * ldz + j] * Z[i * ldz + k];
-          part[wv][lane] = acc;
-          __syncthreads();
-          if (wv == 0) {
-            double s = 0.0;
+    for (int p0 = 0; p0 < c; p0 += PW) {
+      const int pw = min(PW, c - p0), p1 = p0 + pw;
+      for (int e = tid; e < nr * PW; e += GS_THREADS) {
+        const int i = e / PW, q = e - i * PW;
+        Pl[e] = q < pw ? Z[(int64_t)i * ldz + p0 + q] : 0.0;
+      }
+      __syncthreads();
+      // (a) modified Gram-Schmidt in LDS
+      double inv = 1.0;
+      for (int j = 0; j < pw; ++j) {
+        // phase 1: column j-1 scaled; partial dots of column j with k >= j
+        const int q = tid & (PW - 1), g = tid / PW;
+        if (j > 0)
+          for (int i = tid; i < nr; i += GS_THREADS) Pl[i * PW + j - 1] *= inv;
+        double acc = 0.0;
+        if (q >= j && q < pw)
+          for (int i = g; i < nr; i += RG) acc += Pl[i * PW + j] * Pl[i * PW + q];
+        // lanes with equal q (lane % PW) hold one sum each: reduce them
 #pragma unroll
-            for (int w = 0; w < GS_WAVES; ++w) s += part[w][lane];
-            if (q + lane < kt) hb[q + lane] = s;
-          }
-          __syncthreads();
+        for (int o = PW; o < 64; o <<= 1) acc += __shfl_xor(acc, o);
+        if (lane < PW) dr[wv][lane] = acc;
+        __syncthreads();
+        if (tid < PW) {
+          double h = 0.0;
+#pragma unroll
+          for (int w = 0; w < GS_WAVES; ++w) h += dr[w][tid];
+          hh[tid] = h;
         }
-        if (k0 == j) hj = hb[0];
-        const int first = k0 == j ? 1 : 0;
-        // z_k -= (h_k / h_j) z_j for the tile's k
-        for (int q = first; q < kt; q += 64) {
-          const int kk = q + lane;
-          const double cf = kk < kt ? hb[kk] / hj : 0.0;
-          ovl = fmax(ovl, fabs(cf));
-          if (kk < kt)
-            for (int i = wv; i < nr; i += GS_WAVES) Z[i * ldz + k0 + kk] -= cf * Z[i * ldz + j];
+        __syncthreads();
+        // phase 2: z_k -= (h_k / h_j) z_j for k > j
+        const double hj = hh[j];
+        for (int e = tid; e < nr * PW; e += GS_THREADS) {
+          const int i = e / PW, k = e - i * PW;
+          if (k > j && k < pw) Pl[e] -= (hh[k] / hj) * Pl[i * PW + j];
+        }
+        for (int k = j + 1; k < pw; ++k) ovl = fmax(ovl, fabs(hh[k] / hj));
+        inv = 1.0 / sqrt(hj);
+        __syncthreads();
+      }
+      for (int i = tid; i < nr; i += GS_THREADS) Pl[i * PW + pw - 1] *= inv;
+      __syncthreads();
+      for (int e = tid; e < nr * PW; e += GS_THREADS) {
+        const int i = e / PW, q = e - i * PW;
+        if (q < pw) Z[(int64_t)i * ldz + p0 + q] = Pl[e];
+      }
+      // (b) project the panel out of the cluster's later vectors
+      for (int k0 = p1; k0 < c; k0 += 64) {
+        const int k = k0 + lane;
+        const bool valid = k < c;
+        double acc[PW];
+#pragma unroll
+        for (int q = 0; q < PW; ++q) acc[q] = 0.0;
+        for (int i = wv; i < nr; i += GS_WAVES) {
+          const double z = valid ? Z[(int64_t)i * ldz + k] : 0.0;
+#pragma unroll
+          for (int q = 0; q < PW; ++q) acc[q] = fma(Pl[i * PW + q], z, acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < PW; ++q) part[(wv * PW + q) * 64 + lane] = acc[q];
+        __syncthreads();
+        for (int e = tid; e < PW * 64; e += GS_THREADS) {
+          const int q = e >> 6, l = e & 63;
+          double v = 0.0;
+#pragma unroll
+          for (int w = 0; w < GS_WAVES; ++w) v += part[(w * PW + q) * 64 + l];
+          Cc[e] = (q < pw && k0 + l < c) ? v : 0.0;
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < PW; ++q) ovl = fmax(ovl, fabs(Cc[q * 64 + lane]));
+          for (int i = wv; i < nr; i += GS_WAVES) {
+            double z = Z[(int64_t)i * ldz + k];
+#pragma unroll
+            for (int q = 0; q < PW; ++q) z = fma(-Pl[i * PW + q], Cc[q * 64 + lane], z);
+            Z[(int64_t)i * ldz + k] = z;
+          }
         }
         __syncthreads();
       }
-      inv = 1.0 / sqrt(hj);
     }
-    for (int i = tid; i < nr; i += GS_THREADS) Z[i * ldz + (c - 1)] *= inv;
-    __syncthreads();
     // a second sweep only where the first removed overlaps above 1e-6
     if (!(bmax<GS_WAVES>(ovl, red) > 1e-6)) break;
   }
 }
 
-void launch_tri_cluster_gs(const GsCluster* cl, int ncl, hipStream_t s) {
+// LDS of the block Gram-Schmidt: the panel, the wave partials and C
+static size_t gs_lds(int pw, int nr) {
+  return ((size_t)nr * pw + (size_t)GS_WAVES * pw * 64 + (size_t)pw * 64) * sizeof(double);
+}
+
+void launch_tri_cluster_gs(const GsCluster* cl, int ncl, int nrmax, hipStream_t s) {
   if (ncl <= 0) return;
-  hipLaunchKernelGGL(k_tri_cluster_gs, dim3(ncl), dim3(GS_THREADS), 0, s, cl);
+  NSS_REQUIRE(nrmax >= 1 && nrmax <= RHO_BIG_MAX, NSS_E_ARG, "cluster Gram-Schmidt: rows above the block limit");
+  static const bool attr = [] {  // dynamic LDS above the default limit
+    (void)hipFuncSetAttribute((const void*)k_tri_cluster_gs<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)gs_lds(8, 1024));
+    (void)hipFuncSetAttribute((const void*)k_tri_cluster_gs<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)gs_lds(4, RHO_BIG_MAX));
+    return true;
+  }();
+  (void)attr;
+  if (nrmax <= 1024)
+    hipLaunchKernelGGL(k_tri_cluster_gs<8>, dim3(ncl), dim3(GS_THREADS), gs_lds(8, nrmax), s, cl);
+  else hipLaunchKernelGGL(k_tri_cluster_gs<4>, dim3(ncl), dim3(GS_THREADS), gs_lds(4, nrmax), s, cl);
   NSS_HIP(hipGetLastError());
 }
 

@@ -173,12 +173,8 @@ void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
   // NSS_GEMM_TILE overrides the plan's mode (set by the autotuner), which
   // overrides the size-based choice
   int forced = gemm_default_tile();
-  int64_t nit = gemm_big_items();
-  if (!forced && mode != PLAN_AUTO) {
-    forced = mode == PLAN_T32 ? 32 : mode == PLAN_T64 ? 64 : 0;
-    if (mode == PLAN_BIG256) nit = 256;
-    if (mode == PLAN_BIG512) nit = 512;
-  }
+  const int64_t nit = gemm_big_items();
+  if (!forced && mode != PLAN_AUTO) forced = mode == PLAN_T32 ? 32 : mode == PLAN_T64 ? 64 : 0;
   const bool nobig = forced == 32 || forced == 64;
   const int bk = esize == 8 ? 16 : 32;  // K slice of the large-output path
   std::vector<char> isbig(outs.size(), 0);
@@ -255,14 +251,18 @@ void GemmPlan::upload(DevicePool* pool, hipStream_t s, int esize) {
 
 // ---------------------------------------------------------------- autotuner
 // The tiling mode of an H*v plan is measured, not guessed: the first plan of
-// a shape class is uploaded and launched in each candidate mode (each timed
-// twice with HIP events on the context stream, from flushed caches) and the
-// fastest mode is cached for the class; the size-based mode stays unless
-// another is more than 3% faster.  Shape class: phase (W or Y),
-// operand bytes, and the number of 32x32 tiles and the largest output edge,
-// both rounded up to powers of two.  Candidates: 32x32 tiles; 64x64 tiles
-// (launches of at least 256 tiles of 32x32); the large-output path with 256
-// or 512 work items (plans with an output it takes).  NSS_GEMM_AUTOTUNE=0
+// a shape class is uploaded and launched in each candidate mode (twice
+// untimed, then four times back to back, each launch timed with HIP events
+// on the context stream and preceded by a prelude: for Y its W launch) and
+// the fastest mode is cached for the class.  The size-based mode stays
+// unless another is more than 10% faster: isolated trial launches overstate
+// the large-output path (its split-K partials and counters start cold), by
+// up to 45% against the same plan inside a Lanczos run.  Shape class: phase
+// (W or Y), operand bytes, and the number of 32x32 tiles and the largest
+// output edge, both in quarter octaves (ceil(4 log2)).
+// NSS_GEMM_TUNE_TRACE=1 prints every measurement on stderr.  Candidates:
+// 32x32 tiles; 64x64 tiles (launches of at least 256 tiles of 32x32); the
+// large-output path (plans with an output it takes).  NSS_GEMM_AUTOTUNE=0
 // keeps the size-based mode; NSS_GEMM_TILE forces a tile for every plan.
 namespace {
 struct TuneKey {
@@ -274,10 +274,9 @@ struct TuneKey {
 std::mutex tune_mu;
 std::map<TuneKey, int> tune_cache;
 
-int ceil_log2(int64_t v) {
-  int r = 0;
-  while (((int64_t)1 << r) < v) ++r;
-  return r;
+// ceil(4 log2 v): quarter octaves
+int ceil_log2q(int64_t v) {
+  return v <= 1 ? 0 : (int)std::ceil(4.0 * std::log2((double)v) - 1e-9);
 }
 
 bool autotune_enabled() {
@@ -286,7 +285,8 @@ bool autotune_enabled() {
 }
 }  // namespace
 
-void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure) {
+void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool measure,
+                   const std::function<void()>& prelude) {
   hipStream_t s = ctx->stream;
   auto launch = [&]() {
     if (esize == 8) launch_gemm_f64(p, b, s, kind);
@@ -305,7 +305,7 @@ void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool
     mx = std::max(mx, std::max(o.M, o.N));
     anybig = anybig || (big_output(o) && big_offsets_fit(o, p.hsegs));
   }
-  const TuneKey key{kind, esize, ceil_log2(n32), ceil_log2(mx)};
+  const TuneKey key{kind, esize, ceil_log2q(n32), ceil_log2q(mx)};
   {
     std::lock_guard<std::mutex> lk(tune_mu);
     auto it = tune_cache.find(key);
@@ -317,39 +317,34 @@ void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool
   }
   std::vector<int> cand{PLAN_T32};
   if (n32 >= 256) cand.push_back(PLAN_T64);
-  if (anybig) {
-    cand.push_back(PLAN_BIG256);
-    cand.push_back(PLAN_BIG512);
-  }
-  // the size-based mode, kept unless another is measurably faster
-  const int dflt = anybig ? PLAN_BIG512 : n32 >= GEMM_T64_MIN_TILES ? PLAN_T64 : PLAN_T32;
+  if (anybig) cand.push_back(PLAN_BIG);
+  // the size-based mode, kept unless another is clearly faster
+  const int dflt = anybig ? PLAN_BIG : n32 >= GEMM_T64_MIN_TILES ? PLAN_T64 : PLAN_T32;
   int best = dflt;
   if (cand.size() > 1) {
-    // each candidate timed from cold caches, as an H*v inside a Lanczos
-    // iteration runs (the vector work between calls evicts its operands):
-    // a 512 MiB fill between the launches flushes L2 and the MALL
-    constexpr size_t FLUSH = (size_t)512 << 20;
-    DevArr<char> flush;
-    flush.resize(ctx->pool, FLUSH);
-    hipEvent_t e0, e1;
-    NSS_HIP(hipEventCreate(&e0));
-    NSS_HIP(hipEventCreate(&e1));
+    hipEvent_t ev[8];
+    for (hipEvent_t& e : ev) NSS_HIP(hipEventCreate(&e));
     std::vector<float> t(cand.size(), 0.f);
-    for (int rep = 0; rep < 2; ++rep)
-      for (size_t c = 0; c < cand.size(); ++c) {
-        p.mode = cand[c];
-        p.upload(ctx->pool, s, esize);
-        NSS_HIP(hipMemsetAsync(flush.p, rep, FLUSH, s));
-        NSS_HIP(hipEventRecord(e0, s));
+    for (size_t c = 0; c < cand.size(); ++c) {
+      p.mode = cand[c];
+      p.upload(ctx->pool, s, esize);
+      // two untimed calls (the plan's fresh tables and scratch reach the TLB
+      // and caches, the clocks leave the idle of the upload), then four
+      // back-to-back calls timed one by one
+      for (int rep = 0; rep < 6; ++rep) {
+        if (prelude) prelude();
+        if (rep >= 2) NSS_HIP(hipEventRecord(ev[2 * (rep - 2)], s));
         launch();
-        NSS_HIP(hipEventRecord(e1, s));
-        NSS_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        NSS_HIP(hipEventElapsedTime(&ms, e0, e1));
-        t[c] += ms;
+        if (rep >= 2) NSS_HIP(hipEventRecord(ev[2 * (rep - 2) + 1], s));
       }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+      NSS_HIP(hipEventSynchronize(ev[7]));
+      for (int r = 0; r < 4; ++r) {
+        float ms = 0.f;
+        NSS_HIP(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
+        t[c] += 0.5f * ms;
+      }
+    }
+    for (hipEvent_t& e : ev) (void)hipEventDestroy(e);
     float td = -1.f, tb = 0.f;
     int bi = 0;
     for (size_t c = 0; c < cand.size(); ++c) {
@@ -359,7 +354,13 @@ void plan_autotune(nss_ctx* ctx, GemmPlan& p, int kind, int esize, Bases b, bool
         bi = (int)c;
       }
     }
-    if (!(td >= 0.f && td <= 1.03f * tb)) best = cand[bi];
+    if (!(td >= 0.f && td <= 1.1f * tb)) best = cand[bi];
+    if (const char* tr = std::getenv("NSS_GEMM_TUNE_TRACE"); tr && tr[0] == '1') {
+      std::fprintf(stderr, "gemm tune kind %d bytes %d class (%d, %d) tiles32 %lld:", kind, esize, key.lt, key.lm,
+                   (long long)n32);
+      for (size_t c = 0; c < cand.size(); ++c) std::fprintf(stderr, " mode %d %.1f us", cand[c], 500.f * t[c]);
+      std::fprintf(stderr, " -> %d\n", best);
+    }
   }
   {
     std::lock_guard<std::mutex> lk(tune_mu);
@@ -527,14 +528,20 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, cons
   sb->W.resize(ctx->pool, std::max<int64_t>(1, sb->wsize));
   {
     // the autotuner's trial launches read x and W and write W and y: scratch
-    // vectors, zeroed (the operator blocks are the superblock's own)
+    // vectors (the operator blocks are the superblock's own).  A Y trial
+    // follows its W launch, as in every H*v.  (Trials from evicted caches
+    // ranked the modes differently from the H*v calls of a run, whose
+    // operator blocks stay partly cached; back-to-back trials rank them as
+    // the runs do.)
+    const int64_t nx = std::max<int64_t>(1, sb->n);
     DevArr<double> tx, ty;
-    tx.resize(ctx->pool, std::max<int64_t>(1, sb->n));
-    ty.resize(ctx->pool, std::max<int64_t>(1, sb->n));
-    NSS_HIP(hipMemsetAsync(tx.p, 0, std::max<int64_t>(1, sb->n) * sizeof(double), ctx->stream));
+    tx.resize(ctx->pool, nx);
+    ty.resize(ctx->pool, nx);
+    launch_fill_const(tx.p, nx, 0.5, ctx->stream);
     const Bases tb{tx.p, sb->W.p, ty.p};
-    plan_autotune(ctx, sb->W64, GEMM_HV_W, 8, tb, true);
-    plan_autotune(ctx, sb->Y64, GEMM_HV_Y, 8, tb, true);
+    plan_autotune(ctx, sb->W64, GEMM_HV_W, 8, tb, true, nullptr);
+    plan_autotune(ctx, sb->Y64, GEMM_HV_Y, 8, tb, true,
+                  [&]() { launch_gemm_f64(sb->W64, tb, ctx->stream, GEMM_HV_W); });
     NSS_HIP(hipStreamSynchronize(ctx->stream));
   }
   sb->has32 = false;
@@ -545,8 +552,10 @@ void sb_set_terms(nss_ctx* ctx, SbDevice* sb, const std::vector<OpRef>& HL, cons
     sb->Yf.resize(ctx->pool, std::max<int64_t>(1, sb->n));
     NSS_HIP(hipMemsetAsync(sb->Xf.p, 0, std::max<int64_t>(1, sb->n) * sizeof(float), ctx->stream));
     const Bases tb{sb->Xf.p, sb->Wf.p, sb->Yf.p};
-    plan_autotune(ctx, sb->W32, GEMM_HV_W, 4, tb, true);
-    plan_autotune(ctx, sb->Y32, GEMM_HV_Y, 4, tb, true);
+    plan_autotune(ctx, sb->W32, GEMM_HV_W, 4, tb, true, nullptr);
+    plan_autotune(ctx, sb->Y32, GEMM_HV_Y, 4, tb, true,
+                  [&]() { launch_gemm_f32(sb->W32, tb, ctx->stream, GEMM_HV_W); });
+    NSS_HIP(hipStreamSynchronize(ctx->stream));
     sb->has32 = true;
   }
 }
@@ -656,8 +665,8 @@ void sb_merge_hv_plans(const std::vector<SbDevice*>& sbs, const std::vector<int6
   // tiling modes from the autotuner's cache (a merged plan of one chain has
   // that chain's shape class), size-based for classes not measured yet
   (void)s;
-  plan_autotune(ctx, W, GEMM_HV_W, f32 ? 4 : 8, Bases{}, false);
-  plan_autotune(ctx, Y, GEMM_HV_Y, f32 ? 4 : 8, Bases{}, false);
+  plan_autotune(ctx, W, GEMM_HV_W, f32 ? 4 : 8, Bases{}, false, nullptr);
+  plan_autotune(ctx, Y, GEMM_HV_Y, f32 ? 4 : 8, Bases{}, false, nullptr);
 }
 
 // H*v of merged plans, timed like sb_hv.
@@ -952,7 +961,9 @@ void rho_materialize(nss_ctx* ctx, const std::vector<RhoVec*>& vecs, const std::
       ctx->timers["rdm.gs_cmax"] = std::max(ctx->timers["rdm.gs_cmax"], nv);
     }
   launch_tri_invit(B->dreq.p, nreq, B->fac.p, nmax, s);
-  launch_tri_cluster_gs(B->dcl.p, ncl, s);
+  int nrmax = 1;
+  for (const GsCluster& c : B->hcl) nrmax = std::max(nrmax, c.re - c.rs);
+  launch_tri_cluster_gs(B->dcl.p, ncl, nrmax, s);
   if (!lo.empty()) launch_tri_backtr(B->dbt.p, (int)lo.size(), 1024, s);
   if (!hi.empty()) launch_tri_backtr(B->dbt.p + lo.size(), (int)hi.size(), RHO_BIG_MAX, s);
   owner->done.push_back(std::move(B));
