@@ -40,7 +40,8 @@ in FP64 and with the FP32-MFMA H.v, whose |dE0/L| against FP64 is reported.
 --workload spin1_L100_m1024_hv_replay: the iDMRG growth of the default
 workload, then --steps FP64 H.v products on its last superblock (1.5M
 states, 13 sectors) and nothing else: the kernel replay that hardware
-counter passes (rocprofv3 --pmc) are taken on.
+counter passes (rocprofv3 --pmc) are taken on.  The _hv_replay32 variant
+grows with the FP32-MFMA solver and replays the FP32-MFMA H.v.
 """
 from __future__ import annotations
 
@@ -71,6 +72,8 @@ WORKLOADS = {
     "hubbard_L32_m128": dict(model="electron", L=32, m=128, U=4.0, seed=1, sweeps=2),
     # kernel replay for counter passes: the default workload's growth, then H.v only
     "spin1_L100_m1024_hv_replay": dict(model="spin1", L=100, m=1024, U=0.0, seed=2, sweeps=1, replay=True),
+    "spin1_L100_m1024_hv_replay32": dict(model="spin1", L=100, m=1024, U=0.0, seed=2, sweeps=1, replay=True,
+                                         fp32=True),
     # 64 independent Hubbard chains, L=32, per chain from the seed m in
     # [128, 512], U in [0, 8), N in {16, 14, 18} (N_up = N_dw = N/2)
     "hubbard_batch64_L32": dict(model="electron", L=32, batch=64, m_lo=128, m_hi=512, U_hi=8.0,
@@ -407,28 +410,32 @@ def main_pbc(N, ctx, w, a):
 
 
 def main_replay(N, ctx, w, a):
-    """The default workload's iDMRG growth, then a.steps FP64 H.v products on
-    its last superblock (kernel replay for counter passes)."""
-    d = make_engine(N, ctx, w, 1, "fp64", a.verbose)
+    """The default workload's iDMRG growth, then a.steps FP64 (or, for the
+    fp32 variant, FP32-MFMA) H.v products on its last superblock (kernel
+    replay for counter passes)."""
+    f32 = bool(w.get("fp32"))
+    d = make_engine(N, ctx, w, 1, "mixed" if f32 else "fp64", a.verbose)
     last = d.run("i")[-1]
     # the context keeps the engine's last superblock: ctx.hv applies its H_sb
     v = np.random.default_rng(0).uniform(-1.0, 1.0, int(last["m_sb"]))
     ctx.hv_stats(reset=True)
     t = time.perf_counter()
     for _ in range(a.steps):
-        ctx.hv(v)
+        ctx.hv(v, fp32=f32)
     wall = time.perf_counter() - t
     st = ctx.hv_stats(reset=True)
-    calls = max(1, st["calls"])
-    y_avg_s, y_flops = st["t_y"] / calls, st["flops_y"] / calls
+    tag, kn = ("32", "f32") if f32 else ("", "f64")
+    calls = max(1, st["calls" + tag])
+    y_avg_s, y_flops = st["t_y" + tag] / calls, st["flops_y" + tag] / calls
     sb = d.last_superblock()
-    out = {"workload": a.workload, "hv_calls": st["calls_all"], "wall_s": wall, "m_sb": int(v.size),
+    out = {"workload": a.workload, "hv_calls": st["calls_all" + tag], "wall_s": wall, "m_sb": int(v.size),
            "nsect": int(last["nsect"]), "nterms": int(last["nterms"]),
            "sectors": {"Dl": [int(x) for x in sb["Dl"]], "Dr": [int(x) for x in sb["Dr"]],
                        "term_dst": [int(x) for x in sb["dst"]], "term_src": [int(x) for x in sb["src"]]},
-           "k_hv_y_f64": {"avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
-                          "tflops": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0},
-           "k_hv_w_f64": {"avg_launch_us": 1e6 * st["t_w"] / calls, "flops_per_launch": st["flops_w"] / calls}}
+           f"k_hv_y_{kn}": {"avg_launch_us": 1e6 * y_avg_s, "flops_per_launch": y_flops,
+                            "tflops": y_flops / y_avg_s / 1e12 if y_avg_s > 0 else 0.0},
+           f"k_hv_w_{kn}": {"avg_launch_us": 1e6 * st["t_w" + tag] / calls,
+                            "flops_per_launch": st["flops_w" + tag] / calls}}
     d.close()
     ctx.close()
     if a.calib_copy:

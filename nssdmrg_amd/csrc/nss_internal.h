@@ -259,7 +259,7 @@ void launch_normalize(const double* w, double* v, int64_t n, const double* sq, d
 struct LzChainDev {
   double* V;        // (kmax + 1) x n Krylov basis
   double* w;        // H v output, then the orthogonalised residual
-  double* x;        // H v input: copy of the newest basis vector
+  double* x;        // H v input: copy of the newest basis vector (may be null: no copy)
   double* slots;    // iteration j at slots + j * sstride (single-chain slot layout, width 2 kb1 + 1)
   double* part;     // (kb1 + 2) x nch partial sums
   double* om;       // 3 x (kb1 + 1) orthogonality estimates: omega_{j,k} in row j % 3
@@ -267,6 +267,9 @@ struct LzChainDev {
   double* bpre;     // |w| after the three-term step
   int64_t n;
   int64_t sstride;
+  // rounding level of the H v products (unit of the orthogonality
+  // estimates' rounding terms; the full pass runs above sqrt(eps))
+  double eps;
   int32_t nch, blk0, active, kmax;
 };
 int64_t lz_nch(int64_t n);

@@ -1121,6 +1121,7 @@ __global__ __launch_bounds__(256) void k_orth_update(const double* __restrict__ 
     w[i] = x;
     ss += x * x;
   }
+  if (!sq) return;  // no sums of squares asked for (uniform over the grid)
   const double r = block_sum256(ss, red);
   if (threadIdx.x == 0) sq[blockIdx.x] = r;
 }
@@ -1245,7 +1246,8 @@ __global__ __launch_bounds__(256) void k_lzb_sub(const LzChainDev* __restrict__ 
 //                       + beta_{k-1} om_{j,k-1} - beta_{j-1} om_{j-1,k} +- 2 eps |T|
 // and asks for the full Gram-Schmidt pass when max_k |om_{j+1,k}| exceeds
 // sqrt(eps), then again in the next iteration; after a pass the estimates
-// restart at eps.  beta_j here is |w| after the three-term step.
+// restart at eps.  beta_j here is |w| after the three-term step; eps is the
+// chain's rounding level (FP64 or FP32 H v).
 __global__ __launch_bounds__(256) void k_lzb_omega(const LzChainDev* __restrict__ ch, int j, int kb1) {
   __shared__ double sh[4];
   __shared__ double red[256];
@@ -1277,7 +1279,7 @@ __global__ __launch_bounds__(256) void k_lzb_omega(const LzChainDev* __restrict_
   }
   tn = red[0];
   __syncthreads();
-  const double eps = 2.220446049250313e-16;
+  const double eps = C.eps;
   double mx = 0.0;
   for (int k = threadIdx.x; k < j; k += 256) {
     const double* sk = C.slots + (size_t)k * C.sstride;
@@ -1394,7 +1396,7 @@ __global__ __launch_bounds__(256) void k_lzb_normalize(const LzChainDev* __restr
   for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
     const double y = C.w[i] * inv;
     v[i] = y;
-    C.x[i] = y;
+    if (C.x) C.x[i] = y;
   }
 }
 

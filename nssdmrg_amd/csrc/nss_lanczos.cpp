@@ -323,6 +323,28 @@ static double lz_mixed_window() {
   return v;
 }
 
+// Rounding level of the FP32-MFMA H*v for the orthogonality estimates of
+// the mixed solver's correction space (NSS_MIXED_PRO_EPS, default the FP32
+// machine epsilon 2^-23); its full Gram-Schmidt pass runs above sqrt of it.
+static double lz_mixed_eps() {
+  static const double v = [] {
+    const char* e = std::getenv("NSS_MIXED_PRO_EPS");
+    const double x = e ? std::atof(e) : 0.0;
+    return (x > 0.0 && x < 1e-2) ? x : 1.1920928955078125e-07;
+  }();
+  return v;
+}
+
+// NSS_MIXED_FULL_REORTH=1: the correction space's Gram-Schmidt pass in every
+// iteration instead of partial reorthogonalisation.
+static int lz_mixed_full_reorth() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_MIXED_FULL_REORTH");
+    return e && e[0] == '1' ? 1 : 0;
+  }();
+  return v;
+}
+
 // Lowest eigenpair in mixed precision: FP32-MFMA H*v for the bulk of the
 // Krylov work, FP64 for everything that decides the answer.  Each cycle:
 //  1. FP64 block step: W = H X for the block X of p <= 3 orthonormal vectors
@@ -331,8 +353,11 @@ static double lz_mixed_window() {
 //     when |R_1| <= tol max(1, |theta_1|), the FP64 criterion of the
 //     thick-restart solver.
 //  2. A Lanczos space V[p..) built with FP32-MFMA H*v from R_1 (FP64
-//     vectors, recurrence and a Gram-Schmidt pass against X and V every
-//     iteration), together with the exact couplings C = R^T V (X^T H v =
+//     vectors and recurrence; every H v projected out of X, then the
+//     partial-reorthogonalisation iteration of the batched solver, its
+//     estimates at the FP32 rounding level lz_mixed_eps(), so that the
+//     pass against V runs only when semi-orthogonality at sqrt of that level
+//     is at risk), together with the exact couplings C = R^T V (X^T H v =
 //     R^T v for v orthogonal to X).
 //  3. Rayleigh-Ritz of H over [X V]: M = [[Theta, C], [C^T, T]], with the
 //     FP32 rounding confined to the tridiagonal T of the correction space.
@@ -373,6 +398,38 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
   launch_norm(v0, n, L.part.p, L.scal.p, s);
   launch_div(v0, L.vec(0), n, L.scal.p, s);
   const double inner = lz_mixed_inner(), window = lz_mixed_window();
+  // the correction space's chain: V[p..), orthogonality estimates at the
+  // FP32-MFMA rounding level
+  const int kb1 = kmax + 1;
+  const int64_t nchk = lz_nch(n);
+  const int full = lz_mixed_full_reorth();
+  DevArr<double> cpart, om, cbp, hx;
+  DevArr<int32_t> cst, dmap;
+  DevArr<LzChainDev> dch;
+  cpart.resize(ctx->pool, (size_t)(kb1 + 2) * nchk);
+  om.resize(ctx->pool, (size_t)3 * (kb1 + 1));
+  cbp.resize(ctx->pool, 1);
+  hx.resize(ctx->pool, PMAX);
+  cst.resize(ctx->pool, 2);
+  dmap.resize(ctx->pool, nchk);
+  dch.resize(ctx->pool, 1);
+  std::vector<double> om0((size_t)3 * (kb1 + 1), 0.0);
+  om0[0] = 1.0;
+  LzChainDev ch{};
+  ch.w = L.w.p;
+  ch.x = nullptr;
+  ch.slots = L.slots.p;
+  ch.part = cpart.p;
+  ch.om = om.p;
+  ch.st = cst.p;
+  ch.bpre = cbp.p;
+  ch.n = n;
+  ch.sstride = S;
+  ch.eps = lz_mixed_eps();
+  ch.nch = (int32_t)nchk;
+  ch.blk0 = 0;
+  ch.active = 1;
+  NSS_HIP(hipMemsetAsync(dmap.p, 0, (size_t)nchk * sizeof(int32_t), s));
   int p = 1, prev_p = 0;
   double prev_res = -1.0;
   std::vector<double> G, th, nrm(PMAX), sl, cc, M, ev;
@@ -429,31 +486,43 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
     launch_normalize(L.w.p, L.vec(p), n, L.sq.p, L.scal.p + 1, s);
     launch_gemv_t(R.p, n, p, L.vec(p), n, gpart.p, Cd.p, s);  // C[k*p + i] = R_i . v_{p+k}
     NSS_HIP(hipMemsetAsync(L.slots.p, 0, (size_t)kmax * S * sizeof(double), s));
-    L.restart_at = p;
+    // the correction space V[p..) as one chain of the partial-reorthogonalisation
+    // kernels (inner iteration k in slot k), its estimates restarted
+    ch.V = L.vec(p);
+    ch.kmax = kmax - p;
+    NSS_HIP(hipMemcpyAsync(dch.p, &ch, sizeof(LzChainDev), hipMemcpyHostToDevice, s));
+    NSS_HIP(hipMemcpyAsync(om.p, om0.data(), om0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    NSS_HIP(hipMemsetAsync(cst.p, 0, 2 * sizeof(int32_t), s));
     const double target = std::max(0.5 * lim, inner * res1);
     std::vector<double> dV(kmax + 1, 0.0), eV(kmax + 1, 0.0), bV(kmax + 1, 0.0);
-    int j = p, read_upto = p, next_check = p + 1, m = 0;
+    // checks (host round trips) at jj = 4, then where the residual estimate's
+    // geometric decay since the previous check predicts the target
+    int jj = 0, read_upto = 0, next_check = std::min(4, kmax - p), m = 0, m_prev = 0;
+    double est_prev = 0.0;
     bool breakdown = false;
     for (;;) {
-      L.iterate(j, true);
-      ++j;
-      launch_gemv_t(R.p, n, p, L.vec(j), n, gpart.p, Cd.p + (size_t)(j - p) * p, s);
+      sb_hv_fp32(ctx, sb, L.vec(p + jj), L.w.p);
+      ++L.numop32;
+      launch_orth_pass(L.V.p, n, p, L.w.p, n, L.part.p, hx.p, nullptr, s);  // w orthogonal to X
+      launch_lzb_iteration(dch.p, dmap.p, (int)nchk, 1, jj, kb1, full, s);
+      ++jj;
+      const int j = p + jj;
+      launch_gemv_t(R.p, n, p, L.vec(j), n, gpart.p, Cd.p + (size_t)jj * p, s);
       const bool at_end = j == kmax || j == n;
-      if (j < next_check && !at_end) continue;
-      const int cnt = j - read_upto;
+      if (jj < next_check && !at_end) continue;
+      const int cnt = jj - read_upto;
       sl.resize((size_t)cnt * S);
       NSS_HIP(hipMemcpyAsync(sl.data(), L.slots.p + (size_t)read_upto * S, sl.size() * sizeof(double),
                              hipMemcpyDeviceToHost, s));
-      cc.resize((size_t)(j - p + 1) * p);
+      cc.resize((size_t)(jj + 1) * p);
       NSS_HIP(hipMemcpyAsync(cc.data(), Cd.p, cc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       NSS_HIP(hipStreamSynchronize(s));
       double hn = std::fabs(th[0]);
-      for (int it = read_upto; it < j; ++it) {
-        const double* q = sl.data() + (size_t)(it - read_upto) * S;
-        const int k = it - p;
-        dV[k] = q[it] + q[(kmax + 1) + it];
-        if (k > 0) eV[k - 1] = q[it - 1] + q[(kmax + 1) + it - 1];
-        bV[k] = q[2 * (kmax + 1)];
+      for (int k = read_upto; k < jj; ++k) {
+        const double* q = sl.data() + (size_t)(k - read_upto) * S;
+        dV[k] = q[k] + q[kb1 + k];
+        if (k > 0) eV[k - 1] = q[k - 1] + q[kb1 + k - 1];
+        bV[k] = q[2 * kb1];
         hn = std::max(hn, std::fabs(dV[k]));
         m = k + 1;
         if (bV[k] <= 1e-14 * std::max(1.0, hn) || !std::isfinite(bV[k])) {
@@ -461,7 +530,7 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
           break;
         }
       }
-      read_upto = j;
+      read_upto = jj;
       // 3. Rayleigh-Ritz over [X, V[p..p+m)]
       const int D = p + m;
       M.assign((size_t)D * D, 0.0);
@@ -475,7 +544,14 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
       sym_eig(D, M.data(), ev.data());
       const double est = std::fabs(bV[m - 1] * M[(size_t)(D - 1) * D + 0]);
       if (!(est <= target || at_end || breakdown)) {
-        next_check = std::min(kmax, j + (m < 8 ? 1 : 4));
+        int step = 4;
+        if (m_prev > 0 && est > 0.0 && est < est_prev) {
+          const double rate = std::log(est / est_prev) / (m - m_prev);  // < 0
+          step = (int)std::clamp(std::ceil(1.1 * std::log(target / est) / rate), 1.0, 16.0);
+        }
+        m_prev = m;
+        est_prev = est;
+        next_check = std::min(kmax - p, jj + step);
         continue;
       }
       // the next block: the lowest Ritz vector and those within the window
@@ -696,6 +772,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     C.bpre = BP.p + c;
     C.n = n[c];
     C.sstride = (int64_t)K * S;
+    C.eps = 2.220446049250313e-16;  // FP64 H v
     C.nch = nch[c];
     C.blk0 = blk0[c];
     C.active = 1;
