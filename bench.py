@@ -21,8 +21,9 @@ value = H_sb.psi GFLOP/s of the FP64 path over the timed steps: algorithmic
 FLOPs of every H.v (dense sector blocks, reference evaluation order) over
 their GPU time measured with HIP events on the context stream.  The same
 workload is then run with the FP32-MFMA H.v (FP64 Lanczos recurrence, FP64
-Rayleigh-Ritz blocks, FP64 rho) and reported under "fp32_mixed", with its
-sweep times and |dE0/L| against the FP64 run.
+Rayleigh-Ritz blocks, FP64 rho; superblocks below 2^17 states solve in
+FP64) and reported under "fp32_mixed", with its sweep times and |dE0/L|
+against the FP64 run.
 
 --workload hubbard_L64_m256: the Hubbard chain t=1, U=4, half filling, OBC,
 L=64, m=256, 3 finite sweeps, the same definitions.
@@ -126,8 +127,11 @@ def run_batch(N, ctx, w, warmup, steps, verbose, sequential=False):
     t0 = time.perf_counter()
     B.finite_begin()
     t_idmrg = time.perf_counter() - t0
+    print(f"[bench] growth of {len(engines)} chains: {t_idmrg:.1f} s", flush=True)
     for im in range(warmup):
+        t = time.perf_counter()
         B.sweep(im)
+        print(f"[bench] warmup sweep {im}: {time.perf_counter() - t:.1f} s", flush=True)
     n0 = [len(e.steps()) for e in engines]
     ctx.hv_stats(reset=True)
     walls = []
@@ -135,6 +139,7 @@ def run_batch(N, ctx, w, warmup, steps, verbose, sequential=False):
         t = time.perf_counter()
         B.sweep(im)
         walls.append(time.perf_counter() - t)
+        print(f"[bench] timed sweep {im}: {walls[-1]:.1f} s", flush=True)
     st = ctx.hv_stats(reset=True)
     timed = [e.steps()[k:] for e, k in zip(engines, n0)]
     return engines, dict(t_idmrg=t_idmrg, walls=walls, stats=st, steps=timed)
@@ -558,6 +563,10 @@ def main():
         t32 = s32["t_w32"] + s32["t_y32"]
         n32 = int(sum(x["numop_fp32"] for x in f32))
         n64 = int(sum(x["numop"] - x["numop_fp32"] for x in f32))
+        # steps that ran the mixed solver (superblocks of 2^17 states and more;
+        # smaller ones solve in FP64)
+        fm = [x for x in f32 if x["numop_fp32"] > 0]
+        n64m = int(sum(x["numop"] - x["numop_fp32"] for x in fm))
         out["fp32_mixed"] = {
             "ms_per_step": 1e3 * r32["timed"] / a.steps,
             "finite_sweep_wall_s": r32["sweep_walls"],
@@ -565,7 +574,9 @@ def main():
             "hv_fp32_gflops": s32["flops32"] / t32 / 1e9 if t32 > 0 else 0.0,
             "roofline_y": {"kernel": "k_hv_y_f32", "frac": (s32["flops_y32"] / max(1e-30, s32["t_y32"])) /
                            PEAK_FP32_MFMA, "peak_tflops": PEAK_FP32_MFMA / 1e12},
-            "finite_hv_calls": {"fp32": n32, "fp64": n64, "fp64_fraction": n64 / max(1, n32 + n64)},
+            "finite_hv_calls": {"fp32": n32, "fp64": n64, "fp64_fraction": n64 / max(1, n32 + n64),
+                                "mixed_solver_steps": len(fm), "finite_steps": len(f32),
+                                "fp64_fraction_in_mixed_solver_steps": n64m / max(1, n32 + n64m)},
             "e0_per_site": f32[-1]["e_per_site"][0],
             "max_abs_dE0_per_site_vs_fp64": de,
             "max_abs_dS_vs_fp64": max(abs(x["entropy_left"] - y["entropy_left"]) for x, y in zip(allx, ally)),
@@ -573,7 +584,9 @@ def main():
         }
         print(f"[bench] mixed done: {out['fp32_mixed']['ms_per_step']:.1f} ms/step, sweeps "
               f"{[round(x, 2) for x in r32['sweep_walls']]} s, max|dE0/L| {de:.2e}, FP64 H.v "
-              f"{100 * out['fp32_mixed']['finite_hv_calls']['fp64_fraction']:.1f}%", flush=True)
+              f"{100 * out['fp32_mixed']['finite_hv_calls']['fp64_fraction']:.1f}% "
+              f"({100 * out['fp32_mixed']['finite_hv_calls']['fp64_fraction_in_mixed_solver_steps']:.1f}% "
+              f"in the {len(fm)} of {len(f32)} steps the mixed solver ran)", flush=True)
         d32.close()
     if cpu is not None:
         out["cpu_baseline"] = cpu

@@ -1401,13 +1401,21 @@ struct Engine {
     return S;
   }
 
+  // The H*v precision of a step: mixed-precision engines solve superblocks
+  // below lz_mixed_min_n() states in FP64 (their H*v is latency-bound, so
+  // FP32 saves nothing, while the mixed solver's extra cycles cost more).
+  int step_precision(const StepState& S) const {
+    return par.precision == NSS_FP32_MIXED && S.msb >= lz_mixed_min_n() ? NSS_FP32_MIXED : NSS_FP64;
+  }
+
   void diag(StepState& S) {
     PhaseTimer pt(ctx, "step.diag");
     nss_step_info& I = S.st.info;
+    const int prec = step_precision(S);
     if (S.dense) {
       S.lo = dense_diag(ctx, sbd(), S.neig, psi.p);
       I.numop = 0;
-    } else if (S.neig == 1 && par.precision == NSS_FP64 && S.msb >= lz_pro_min_n()) {
+    } else if (S.neig == 1 && prec == NSS_FP64 && S.msb >= lz_pro_min_n()) {
       // large superblocks: Lanczos with partial reorthogonalisation (a batch
       // of one chain; the thick-restart solver past kmax), where the full
       // Gram-Schmidt pass would cost more HBM traffic than H*v itself
@@ -1422,7 +1430,7 @@ struct Engine {
       I.numop_fp32 = 0;
     } else {
       S.lo = lanczos_device(ctx, sbd(), S.neig, par.lanc_kmax, par.lanc_maxiter, par.lanc_tol, par.seed, S.v0,
-                            par.precision, psi.p);
+                            prec, psi.p);
       I.numop = S.lo.numop + S.lo.numop_fp32;
       I.numop_fp32 = S.lo.numop_fp32;
     }
@@ -1662,7 +1670,8 @@ struct Engine {
 // up its superblock, one batched Lanczos solves all ground states (the H*v
 // of all chains as one grouped launch per phase), then every engine
 // truncates.  Steps on the dense path, with LANC_NEIGEN > 1 or in mixed
-// precision use the engine's own solver inside the same lockstep.
+// precision (superblocks from lz_mixed_min_n() states) use the engine's own
+// solver inside the same lockstep.
 struct Batch {
   nss_ctx* ctx = nullptr;
   std::vector<Engine*> es;
@@ -1675,7 +1684,7 @@ struct Batch {
     std::vector<size_t> who;
     for (size_t i = 0; i < es.size(); ++i) {
       Engine* e = es[i];
-      if (S[i].dense || S[i].neig != 1 || e->par.precision != NSS_FP64) {
+      if (S[i].dense || S[i].neig != 1 || e->step_precision(S[i]) != NSS_FP64) {
         e->diag(S[i]);
         continue;
       }

@@ -521,6 +521,9 @@ double lz_mixed_inner();
 // thick-restart solver with a Gram-Schmidt pass every iteration
 // (NSS_LZ_PRO_MIN_N; 0 for every size).
 int64_t lz_pro_min_n();
+// Smallest superblock a mixed-precision engine solves with FP32-MFMA H*v
+// (NSS_MIXED_MIN_N, default 2^17, read on every call).
+int64_t lz_mixed_min_n();
 // FP64 thick-restart iterations take alpha_j from the Y launch's partial
 // sums of v_j . H v_j (NSS_LZ_ALPHA_FROM_Y; 0 for a separate dot pass).
 bool lz_alpha_from_y();

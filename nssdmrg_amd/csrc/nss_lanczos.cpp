@@ -680,6 +680,11 @@ int64_t lz_pro_min_n() {
   return v;
 }
 
+int64_t lz_mixed_min_n() {
+  const char* e = std::getenv("NSS_MIXED_MIN_N");
+  return e ? (int64_t)std::max(0LL, std::atoll(e)) : (int64_t)1 << 17;
+}
+
 // Full Gram-Schmidt pass in every batched iteration (NSS_LZB_FULL_REORTH=1)
 // instead of partial reorthogonalisation.
 static int lzb_full_reorth() {

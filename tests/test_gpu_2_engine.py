@@ -264,7 +264,10 @@ def test_restart_round_trip(ctx, tmp_path, model):
     assert st2[0]["m_sb"] == st1[-1]["m_sb"]
 
 
-def test_mixed_precision_refined_matches_fp64(ctx):
+def test_mixed_precision_refined_matches_fp64(ctx, monkeypatch):
+    # every superblock through the mixed solver (the engine's default sends
+    # superblocks below 2^17 states to the FP64 solver)
+    monkeypatch.setenv("NSS_MIXED_MIN_N", "0")
     kw = dict(site=N.spin_site(3), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=48, msweep=[48],
               target_qn=(0.0, 0.0), neigen=1, lanc_dim_threshold=128)
     _, s64 = run(ctx, "f", **kw)

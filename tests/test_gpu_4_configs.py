@@ -161,11 +161,14 @@ def test_config2_hubbard_L64_vs_oracle(ctx, golden_dir):
         d.close()
 
 
-def test_config2_hubbard_L64_m256_full(ctx):
+def test_config2_hubbard_L64_m256_full(ctx, monkeypatch):
     print("case config2 Hubbard U=4 L=64 m=256, FP64 and FP32-MFMA", flush=True)
     d64, s64 = _run(ctx, **_hub64(256))
     _invariants(d64, s64)
     _sweeps_do_not_rise(s64)
+    # every superblock through the mixed solver, also those below the
+    # engine's 2^17-state threshold for it
+    monkeypatch.setenv("NSS_MIXED_MIN_N", "0")
     d32, s32 = _run(ctx, **_hub64(256, precision="mixed"))
     n32, n64 = _fp64_vs_mixed(d64, s64, d32, s32)
     assert n32 > 4 * n64, (n32, n64)
@@ -187,7 +190,12 @@ def test_config3_spin1_L100_m1024_fp64_vs_mixed(ctx):
     d32, s32 = _run(ctx, precision="mixed", **kw)
     n32, n64 = _fp64_vs_mixed(d64, s64, d32, s32)
     print(f"  mixed: {n32} FP32 and {n64} FP64 H*v", flush=True)
-    fin = [b for b in s32 if b["type"] == "f" and b["numop"] > 0]
+    # superblocks of 2^17 states and more run the mixed solver, smaller ones FP64
+    # only; the mixed solver's FP64 share of its finite steps' H*v stays small
+    big = [b for b in s32 if b["numop"] > 0 and b["m_sb"] >= 1 << 17]
+    assert all(b["numop_fp32"] == 0 for b in s32 if b["m_sb"] < 1 << 17)
+    assert sum(b["numop_fp32"] > 0 for b in big) >= 0.9 * len(big), len(big)
+    fin = [b for b in s32 if b["type"] == "f" and b["numop_fp32"] > 0]
     n32f = sum(b["numop_fp32"] for b in fin)
     n64f = sum(b["numop"] - b["numop_fp32"] for b in fin)
     assert n64f <= 0.2 * (n32f + n64f), (n32f, n64f)
