@@ -119,6 +119,29 @@ def test_finite_sweeps_vs_oracle(ctx, model):
                 assert s["m_left"] == r.m_left, i
 
 
+def test_finite_msweep_order_pinned_by_ed(ctx):
+    # the engine's finite driver with Msweep = [4, 4096] on 10 sites against
+    # exact diagonalisation (as the oracle's test_finite_msweep_order_pinned_by_ed):
+    # sweep 1 (13 steps) keeps 4 states and stays visibly above the exact
+    # energy, sweep 2 (12 steps) ends exact, and the last step of sweep 1
+    # (left length Ldmrg+1, label 1) is not renormalised (8 = 4 x 2 states)
+    print("case finite Msweep [4, 4096] vs ED", flush=True)
+    d, st = run(ctx, "f", site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=4, mdmrg=4096, msweep=[4, 4096],
+                target_qn=(0.0, 0.0), neigen=1, lanc_dim_threshold=16)
+    ed10 = O.ed_ground(O.spin_site(2), 10, SPIN_HIJ, (0.0,))[0]
+    fin = [s for s in st if s["type"] == "f"]
+    assert len(fin) == 25
+    s1, s2 = fin[:13], fin[13:]
+    assert [s["left_len"] for s in s1] == [5, 6, 7, 8, 7, 6, 5, 4, 3, 2, 3, 4, 5]
+    assert [s["left_len"] for s in s2] == [6, 7, 8, 7, 6, 5, 4, 3, 2, 3, 4, 5]
+    assert all(s["energies"][0] >= ed10 - 1e-10 for s in s1)
+    assert min(s["energies"][0] - ed10 for s in s1[2:]) > 1e-4
+    assert not s1[-1]["renormalized"] and s1[-1]["m_left"] == 8
+    for s in s2[-2:]:
+        assert abs(s["energies"][0] - ed10) < 1e-10, (s["left_len"], s["energies"][0] - ed10)
+    d.close()
+
+
 @pytest.mark.parametrize("model,neigen", [("spin12", 1), ("spin12", 2), ("hubbard", 1)])
 def test_finite_prediction_matches_oracle(ctx, model, neigen):
     # Lanczos started from the transformed previous ground state converges to

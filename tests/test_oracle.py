@@ -123,6 +123,46 @@ def test_hubbard_U2_published_trajectory(golden_dir):
     assert abs(d.records[0].energies[0] - e4) < 1e-12
 
 
+def test_hubbard_U0_published_and_tight_binding_exact(golden_dir):
+    # .plot/Hubbard/U0 (U=0, M=40, HFMODE=F) and .plot/Hubbard/TBexact (exact
+    # E/L of the half-filled open tight-binding chain of 2j sites).  Left
+    # lengths 2..4 are untruncated and equal the published run; from 5 on
+    # the degeneracy window of the current rule keeps more than 40 states of
+    # the highly degenerate U=0 spectra, and every step lies between the
+    # exact energy (variational bound) and the published one.  First 19 rows.
+    ref = {int(r[0]): r[1] for r in _load(golden_dir, "hubbard_U0_L100_M40_hf0_energy.dmrg")}
+    ex = {int(r[0]): r[1] for r in _load(golden_dir, "tb_exact_obc_eVSj.txt")}
+    p = O.Params(ldmrg=20, mdmrg=40, target_qn=(0.5, 0.5), hij=HUB_HIJ, neigen=1, lanc_dim_threshold=128)
+    d = O.DMRG(O.electron_site(0.0, False), p)
+    d.infinite()
+    assert [r.left_len for r in d.records] == list(range(2, 21))
+    for r in d.records:
+        e = r.e_per_site[0]
+        if r.left_len <= 4:
+            assert abs(e - ref[r.left_len]) < 1e-12
+        assert ex[r.left_len] - 1e-12 <= e <= ref[r.left_len] + 1e-12, r.left_len
+    assert max(r.m_left for r in d.records[3:]) > 40
+
+
+def test_spin12_L300_published_first_rows(golden_dir):
+    # .plot/HeisenbergSU2.3 (spin-1/2, M=40, LANC_NEIGEN=2): left lengths 2..8
+    # to rounding (E0/L, E1/L, S); at left length 8 the degeneracy window keeps
+    # 43 states and the runs part, the current rule's energies staying at or
+    # below the published ones.  First 29 rows.
+    ref = _load(golden_dir, "spin12_L300_M40_energy.dmrg")
+    refs = _load(golden_dir, "spin12_L300_M40_Sentropy.dmrg")
+    p = O.Params(ldmrg=30, mdmrg=40, target_qn=(0.0,), hij=SPIN_HIJ, neigen=2, lanc_dim_threshold=128)
+    d = O.DMRG(O.spin_site(2), p)
+    d.infinite()
+    assert len(d.records) == 29
+    for r, e, sv in zip(d.records, ref, refs):
+        assert r.left_len == int(e[0])
+        if r.left_len <= 8:
+            assert abs(r.e_per_site[0] - e[1]) < 1e-12 and abs(r.e_per_site[1] - e[2]) < 1e-12
+            assert abs(r.entropy_left - sv[1]) < 1e-10
+        assert r.e_per_site[0] <= e[1] + 1e-12 and r.e_per_site[1] <= e[2] + 1e-10, r.left_len
+
+
 # ------------------------------------------------------------------ exact checks
 @pytest.mark.parametrize("sun,L", [(2, 10), (3, 6)])
 def test_spin_untruncated_equals_ed(sun, L):
@@ -192,6 +232,34 @@ def test_finite_sweeps_reach_ed_when_untruncated():
     assert fin and all(r.L == 10 for r in fin)
     for r in fin:
         assert abs(r.energies[0] - ed10) < 1e-10
+
+
+def test_finite_msweep_order_pinned_by_ed():
+    # Msweep = [4, 4096] on 10 sites (finite_DMRG, src/DMRG_MAIN.f90:208-239,
+    # fNlat = 2 Ldmrg = 8).  Sweep 1 runs 13 steps (left 4 -> 8, right 2 -> 8,
+    # left 2 -> 5) and sweep 2, which starts from the unrenormalised left block
+    # of length 5, runs 12.  Sweep 1 keeps 4 states: variational and visibly
+    # above the exact energy.  Sweep 2 keeps everything: its last two steps
+    # (both blocks rebuilt untruncated) are exact.  Ignoring Msweep, or taking
+    # its entries in another order, fails one of these.  The last step of each
+    # sweep (left length Ldmrg+1, label 1) is not renormalised
+    # (src/DMRG_MAIN.f90:401-404): its left block keeps all 4 x 2 enlarged
+    # states in sweep 1.
+    p = O.Params(ldmrg=4, mdmrg=4096, target_qn=(0.0,), hij=SPIN_HIJ, neigen=1,
+                 lanc_dim_threshold=16, msweep=[4, 4096])
+    d = O.DMRG(O.spin_site(2), p)
+    d.finite()
+    ed10 = O.ed_ground(O.spin_site(2), 10, SPIN_HIJ, (0.0,))[0]
+    fin = [r for r in d.records if r.kind == "f"]
+    assert len(fin) == 25
+    s1, s2 = fin[:13], fin[13:]
+    assert [r.left_len for r in s1] == [5, 6, 7, 8, 7, 6, 5, 4, 3, 2, 3, 4, 5]
+    assert [r.left_len for r in s2] == [6, 7, 8, 7, 6, 5, 4, 3, 2, 3, 4, 5]
+    assert all(r.energies[0] >= ed10 - 1e-10 for r in s1)
+    assert min(r.energies[0] - ed10 for r in s1[2:]) > 1e-4
+    assert s1[-1].m_left == 8
+    for r in s2[-2:]:
+        assert abs(r.energies[0] - ed10) < 1e-10, (r.left_len, r.energies[0] - ed10)
 
 
 # ------------------------------------------------------- superblock H, H*v
