@@ -270,6 +270,9 @@ struct LzChainDev {
   // rounding level of the H v products (unit of the orthogonality
   // estimates' rounding terms; the full pass runs above sqrt(eps))
   double eps;
+  // estimate of the new vector's loss against the previous one
+  // (omega_{j+1,j}) at each step
+  double psi;
   int32_t nch, blk0, active, kmax;
 };
 int64_t lz_nch(int64_t n);

@@ -1247,7 +1247,8 @@ __global__ __launch_bounds__(256) void k_lzb_sub(const LzChainDev* __restrict__ 
 // and asks for the full Gram-Schmidt pass when max_k |om_{j+1,k}| exceeds
 // sqrt(eps), then again in the next iteration; after a pass the estimates
 // restart at eps.  beta_j here is |w| after the three-term step; eps is the
-// chain's rounding level (FP64 or FP32 H v).
+// chain's rounding level (FP64 or FP32 H v) and psi its estimate of
+// omega_{j+1,j}.
 __global__ __launch_bounds__(256) void k_lzb_omega(const LzChainDev* __restrict__ ch, int j, int kb1) {
   __shared__ double sh[4];
   __shared__ double red[256];
@@ -1290,7 +1291,7 @@ __global__ __launch_bounds__(256) void k_lzb_omega(const LzChainDev* __restrict_
     nxt[k] = v;
     mx = fmax(mx, fabs(v));
   }
-  const double psi = eps * sqrt((double)C.n);
+  const double psi = C.psi;
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {

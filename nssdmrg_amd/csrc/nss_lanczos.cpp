@@ -426,6 +426,11 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
   ch.n = n;
   ch.sstride = S;
   ch.eps = lz_mixed_eps();
+  // the FP32 rounding of H v enters every step, while alpha comes from the
+  // same w it is subtracted from: the new vector's loss against the previous
+  // one is at the FP32 rounding level (the FP64 path's eps sqrt(n) models the
+  // dot products' rounding, which is far below it here)
+  ch.psi = ch.eps;
   ch.nch = (int32_t)nchk;
   ch.blk0 = 0;
   ch.active = 1;
@@ -567,9 +572,24 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
       NSS_HIP(hipMemcpyAsync(hd.p, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, s));
       for (int i = 0; i < np; ++i) launch_gemv_n(L.V.p, n, D, hd.p + (size_t)i * D, Tmp.p + (size_t)i * n, n, 1, s);
       NSS_HIP(hipMemcpyAsync(L.V.p, Tmp.p, (size_t)np * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      for (int i = 0; i < np; ++i) {
+      launch_norm(L.vec(0), n, L.part.p, L.scal.p, s);
+      launch_div(L.vec(0), L.vec(0), n, L.scal.p, s);
+      // the block orthonormalised in FP64; a vector mostly inside the span of
+      // the ones before it (a copy of a converged Ritz vector that the FP32
+      // space's rounding left in it, not a partner state) ends the block
+      for (int i = 1; i < np; ++i) {
         launch_norm(L.vec(i), n, L.part.p, L.scal.p, s);
         launch_div(L.vec(i), L.vec(i), n, L.scal.p, s);
+        launch_orth_pass(L.V.p, n, i, L.vec(i), n, L.part.p, hx.p, L.sq.p, s);
+        launch_normalize(L.vec(i), L.vec(i), n, L.sq.p, L.scal.p + 2, s);
+        double rest = 0.0;
+        NSS_HIP(hipMemcpyAsync(&rest, L.scal.p + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        NSS_HIP(hipStreamSynchronize(s));
+        if (!(rest >= 0.1)) {
+          if (lz_mixed_trace()) std::fprintf(stderr, "mixed   block vector %d dropped (%.3e outside the rest)\n", i, rest);
+          np = i;
+          break;
+        }
       }
       NSS_HIP(hipStreamSynchronize(s));  // y lives on the host
       p = np;
@@ -778,6 +798,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     C.n = n[c];
     C.sstride = (int64_t)K * S;
     C.eps = 2.220446049250313e-16;  // FP64 H v
+    C.psi = C.eps * std::sqrt((double)n[c]);  // rounding of the length-n dot products
     C.nch = nch[c];
     C.blk0 = blk0[c];
     C.active = 1;
