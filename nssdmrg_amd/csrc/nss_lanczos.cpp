@@ -335,6 +335,17 @@ static double lz_mixed_eps() {
   return v;
 }
 
+// Largest correction space of one cycle of the mixed solver
+// (NSS_MIXED_MAXM, read per call; default 64): a cycle that reaches it ends
+// with the Rayleigh-Ritz of what it has, and the next cycle starts from that
+// (longer FP32 spaces grown from random starts degraded into Ritz values far
+// outside the spectrum and forced FP64 hand-overs).
+static int lz_mixed_maxm() {
+  const char* e = std::getenv("NSS_MIXED_MAXM");
+  const int v = e ? std::atoi(e) : 0;
+  return v >= 8 ? v : 64;
+}
+
 // NSS_MIXED_FULL_REORTH=1: the correction space's Gram-Schmidt pass in every
 // iteration instead of partial reorthogonalisation.
 static int lz_mixed_full_reorth() {
@@ -362,8 +373,9 @@ static int lz_mixed_full_reorth() {
 //  3. Rayleigh-Ritz of H over [X V]: M = [[Theta, C], [C^T, T]], with the
 //     FP32 rounding confined to the tridiagonal T of the correction space.
 //     The cycle ends when the lowest pair's residual estimate has fallen by
-//     lz_mixed_inner() below |R_1|; the Ritz vectors of M whose values lie
-//     within lz_mixed_window() |theta_1| of the lowest become the next block.
+//     lz_mixed_inner() below |R_1|, or at lz_mixed_maxm() vectors; the Ritz
+//     vectors of M whose values lie within lz_mixed_window() |theta_1| of the
+//     lowest become the next block, orthonormalised in FP64.
 // A near-degenerate partner (a state closer to the ground state than the
 // FP32 rounding of H*v, which an FP32 Krylov space mixes with it at O(1))
 // thereby enters the block and is separated exactly by the FP64 Rayleigh-Ritz.
@@ -426,11 +438,18 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
   ch.n = n;
   ch.sstride = S;
   ch.eps = lz_mixed_eps();
-  // the FP32 rounding of H v enters every step, while alpha comes from the
-  // same w it is subtracted from: the new vector's loss against the previous
-  // one is at the FP32 rounding level (the FP64 path's eps sqrt(n) models the
-  // dot products' rounding, which is far below it here)
-  ch.psi = ch.eps;
+  // the new vector's loss against the previous one: an eighth of the
+  // semi-orthogonality threshold (NSS_MIXED_PSI overrides).  alpha comes from
+  // the same w it is subtracted from, so the FP64 path's eps sqrt(n) (the
+  // rounding of length-n dot products) does not model it: at 12.8M states it
+  // exceeds sqrt(eps) and forced a full pass every iteration; one FP32
+  // rounding unit let spaces grown from random starts degrade (measured on
+  // the spin-1 m=1024 and periodic-ring workloads)
+  ch.psi = std::sqrt(ch.eps) / 8.0;
+  if (const char* e = std::getenv("NSS_MIXED_PSI")) {
+    const double v = std::atof(e);
+    if (v > 0.0 && v < 1.0) ch.psi = v;
+  }
   ch.nch = (int32_t)nchk;
   ch.blk0 = 0;
   ch.active = 1;
@@ -513,7 +532,7 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
       ++jj;
       const int j = p + jj;
       launch_gemv_t(R.p, n, p, L.vec(j), n, gpart.p, Cd.p + (size_t)jj * p, s);
-      const bool at_end = j == kmax || j == n;
+      const bool at_end = j == kmax || j == n || jj >= lz_mixed_maxm();
       if (jj < next_check && !at_end) continue;
       const int cnt = jj - read_upto;
       sl.resize((size_t)cnt * S);
