@@ -336,7 +336,7 @@ static double lz_mixed_eps() {
 }
 
 // Largest correction space of one cycle of the mixed solver
-// (NSS_MIXED_MAXM, read per call; default 64): a cycle that reaches it ends
+// (NSS_MIXED_MAXM, read per solve; default 64): a cycle that reaches it ends
 // with the Rayleigh-Ritz of what it has, and the next cycle starts from that
 // (longer FP32 spaces grown from random starts degraded into Ritz values far
 // outside the spectrum and forced FP64 hand-overs).
@@ -415,6 +415,7 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
   const int kb1 = kmax + 1;
   const int64_t nchk = lz_nch(n);
   const int full = lz_mixed_full_reorth();
+  const int maxm = lz_mixed_maxm();
   DevArr<double> cpart, om, cbp, hx;
   DevArr<int32_t> cst, dmap;
   DevArr<LzChainDev> dch;
@@ -532,7 +533,7 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
       ++jj;
       const int j = p + jj;
       launch_gemv_t(R.p, n, p, L.vec(j), n, gpart.p, Cd.p + (size_t)jj * p, s);
-      const bool at_end = j == kmax || j == n || jj >= lz_mixed_maxm();
+      const bool at_end = j == kmax || j == n || jj >= maxm;
       if (jj < next_check && !at_end) continue;
       const int cnt = jj - read_upto;
       sl.resize((size_t)cnt * S);
