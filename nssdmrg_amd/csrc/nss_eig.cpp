@@ -109,11 +109,12 @@ void tridiag_eig(int n, const double* d, const double* e, double* w, int nr, con
 // iteration: two solves with T - theta I, factored by Gaussian elimination
 // with partial pivoting (the tridiagonal LU of LAPACK's dgttrf, U with two
 // superdiagonals), zero pivots replaced by eps ||T||.
-void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast) {
+void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast, double* zfull) {
   if (n <= 0) return;
   if (n == 1) {
     *theta = d[0];
     *zlast = 1.0;
+    if (zfull) zfull[0] = 1.0;
     return;
   }
   const double eps = std::numeric_limits<double>::epsilon();
@@ -192,6 +193,7 @@ void tridiag_lowest(int n, const double* d, const double* e, double* theta, doub
     for (int i = 0; i < n; ++i) x[i] /= nrm;
   }
   *zlast = x[n - 1];
+  if (zfull) std::copy(x.begin(), x.end(), zfull);
 }
 
 void tridiag_eig_full(int n, const double* d, const double* e, double* w, double* z) {

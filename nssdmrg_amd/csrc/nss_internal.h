@@ -489,8 +489,10 @@ void sym_eig(int n, double* a, double* w);
 // zrows (nr x n, row-major) must enter holding the same rows of the identity.
 // Lowest eigenvalue theta of the tridiagonal (d, e) and the last component of
 // its unit eigenvector (sign arbitrary): the Lanczos residual estimate
-// |beta_m z_last| in O(n) per bisection step instead of a full QL.
-void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast);
+// |beta_m z_last| in O(n) per bisection step instead of a full QL.  With
+// zfull, the whole eigenvector (n entries) from the same inverse iteration.
+void tridiag_lowest(int n, const double* d, const double* e, double* theta, double* zlast,
+                    double* zfull = nullptr);
 void tridiag_eig(int n, const double* d, const double* e, double* w, int nr, const int* rows,
                  double* zrows);
 // Full eigenvectors of the tridiagonal (z n x n row-major, columns = vectors).

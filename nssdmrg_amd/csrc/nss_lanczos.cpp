@@ -789,7 +789,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
   P.resize(ctx->pool, ptot);
   SL.resize(ctx->pool, (size_t)kb * K * S);
   scal.resize(ctx->pool, K);
-  yd.resize(ctx->pool, kb1);
+  yd.resize(ctx->pool, (size_t)K * kb1);  // a slice per chain
   NSS_HIP(hipMemsetAsync(SL.p, 0, (size_t)kb * K * S * sizeof(double), s));
   // orthogonality estimates: omega_{0,0} = 1, everything else 0
   OM.resize(ctx->pool, (size_t)K * 3 * kb1);
@@ -853,12 +853,15 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     launch_div(v0, v0, n[c], scal.p + c, s);
     NSS_HIP(hipMemcpyAsync(hc[c].x, v0, n[c] * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  LzClock clk(ctx);
   GemmPlan Wp, Yp;
   sb_merge_hv_plans(sbs, xoff, woff, Wp, Yp, ctx, s);
+  clk.lap("lzb.merge");
   // host side of every chain: the projected tridiagonal
   std::vector<std::vector<double>> d(K, std::vector<double>(kb1, 0.0)), e(K, std::vector<double>(kb1, 0.0)),
       beta(K, std::vector<double>(kb1, 0.0));
   std::vector<double> hn(K, 0.0);
+  std::vector<std::vector<double>> ybuf(K);  // each chain's Ritz coefficients, alive until the end
   std::vector<int> numop(K, 0);
   std::vector<char> active(K, 1), fallback(K, 0);
   int nact = K, j = 0, read_upto = 0;
@@ -867,6 +870,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
   while (nact > 0) {
     hv_plans(ctx, Wp, Yp, bases);
     launch_lzb_iteration(dch.p, dmap.p, btot, K, j, kb1, full, s);
+    clk.lap("lzb.enqueue");
     ++j;
     bool at_end = false;
     for (int c = 0; c < K; ++c)
@@ -880,6 +884,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
     NSS_HIP(hipMemcpyAsync(hs.data(), SL.p + (size_t)read_upto * K * S, hs.size() * sizeof(double),
                            hipMemcpyDeviceToHost, s));
     NSS_HIP(hipStreamSynchronize(s));
+    clk.lap("lzb.wait");
     bool changed = false;
     for (int c = 0; c < K; ++c) {
       if (!active[c]) continue;
@@ -897,21 +902,23 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
           break;
         }
       }
-      std::vector<double> th(m), zl(1);
-      tridiag_lowest(m, d[c].data(), e[c].data(), &th[0], &zl[0]);
-      const double res = std::fabs(beta[c][m - 1] * zl[0]);
-      const bool conv = res <= tol * std::max(1.0, std::fabs(th[0]));
+      // the lowest pair of the tridiagonal: theta, and its eigenvector y from
+      // the same inverse iteration (its last component gives the residual)
+      double th0 = 0.0, zl = 0.0;
+      std::vector<double>& y = ybuf[c];
+      y.assign(m, 0.0);
+      tridiag_lowest(m, d[c].data(), e[c].data(), &th0, &zl, y.data());
+      const double res = std::fabs(beta[c][m - 1] * zl);
+      const bool conv = res <= tol * std::max(1.0, std::fabs(th0));
       if (!(conv || breakdown || m >= kmc[c])) continue;
-      // Ritz vector x = V[0..m) y, y the lowest eigenvector of the tridiagonal
-      std::vector<double> Yv((size_t)m * m), y(m);
-      tridiag_eig_full(m, d[c].data(), e[c].data(), th.data(), Yv.data());
-      for (int a = 0; a < m; ++a) y[a] = Yv[(size_t)a * m];
-      NSS_HIP(hipMemcpyAsync(yd.p, y.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
-      launch_gemv_n(hc[c].V, n[c], m, yd.p, jobs[c].xdev, n[c], 1, s);
+      // Ritz vector x = V[0..m) y (y stays alive in ybuf until the final
+      // synchronisation: no wait per chain)
+      double* ydc = yd.p + (size_t)c * kb1;
+      NSS_HIP(hipMemcpyAsync(ydc, y.data(), m * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_gemv_n(hc[c].V, n[c], m, ydc, jobs[c].xdev, n[c], 1, s);
       launch_norm(jobs[c].xdev, n[c], hc[c].part, scal.p + c, s);
       launch_div(jobs[c].xdev, jobs[c].xdev, n[c], scal.p + c, s);
-      NSS_HIP(hipStreamSynchronize(s));  // y lives on the host stack
-      jobs[c].out.evals.assign(1, th[0]);
+      jobs[c].out.evals.assign(1, th0);
       jobs[c].out.numop = numop[c];
       fallback[c] = !(conv || breakdown);
       active[c] = 0;
@@ -920,6 +927,7 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
       changed = true;
     }
     read_upto = j;
+    clk.lap("lzb.host");
     if (changed && nact > 0) {
       // drop the finished chains from the merged H*v and the vector launches
       NSS_HIP(hipMemcpyAsync(dch.p, hc.data(), K * sizeof(LzChainDev), hipMemcpyHostToDevice, s));
@@ -932,6 +940,8 @@ void lanczos_batch(nss_ctx* ctx, std::vector<LanczosJob>& jobs, int kmax_req, in
           aw.push_back(woff[c]);
         }
       sb_merge_hv_plans(asb, ax, aw, Wp, Yp, ctx, s);
+      clk.lap("lzb.merge");
+      clk.count("lzb.merges");
     }
   }
   NSS_HIP(hipStreamSynchronize(s));
