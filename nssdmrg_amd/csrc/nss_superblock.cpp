@@ -770,6 +770,16 @@ static int rho_big_limit() {
   return v;
 }
 
+// Number of large blocks in one rho call from which blocks of up to 768 go to
+// rocSOLVER instead of the large-block path (NSS_RHO_BIG_MANY, default 64).
+static int rho_big_many() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_RHO_BIG_MANY");
+    return e ? std::max(1, std::atoi(e)) : 64;
+  }();
+  return v;
+}
+
 static int rho_tri_limit() {
   static const int v = [] {
     const char* e = std::getenv("NSS_RHO_TRI_MAX");
@@ -1070,7 +1080,20 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
   if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
   const int tmax = std::min(rho_tri_max_dim(ctx->device), rho_tri_limit());
-  const int bmax = rho_big_limit();
+  int bmax = rho_big_limit();
+  {
+    // many mid-sized blocks in one call (the rho of a batch of chains): the
+    // batched rocSOLVER classes beat one tridiagonalisation launch per column
+    // (64 Hubbard chains: 2 s less per sweep); few large blocks (one chain at
+    // m=1024) take the large-block path
+    int nbig = 0, dbig = 0;
+    for (const Item& it : items)
+      if (it.d > tmax && it.d <= bmax) {
+        ++nbig;
+        dbig = std::max(dbig, it.d);
+      }
+    if (nbig >= rho_big_many() && dbig <= 768) bmax = tmax;
+  }
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff, toff, teoff;
   std::vector<int32_t> jdim, tdim;
