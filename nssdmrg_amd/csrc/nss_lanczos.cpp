@@ -227,11 +227,19 @@ static std::vector<double> trlan(Lz& L, const double* v0, int neigen, int maxite
       }
       if (breakdown) converged = true;
       if (converged || at_end || m == n) {
-        if (k == 0) {  // full eigenvectors of the tridiagonal
+        if (k == 0) {
           std::vector<double> d(m), e(m, 0.0);
           for (int i = 0; i < m; ++i) d[i] = L.t(i, i);
           for (int i = 0; i + 1 < m; ++i) e[i] = L.t(i + 1, i);
-          tridiag_eig_full(m, d.data(), e.data(), th.data(), Y.data());
+          if (neigen == 1 && (converged || m == n || (soft && cycles >= maxiter))) {
+            // only the lowest Ritz vector is needed: its inverse iteration
+            std::vector<double> z(m);
+            double zl = 0.0;
+            tridiag_lowest(m, d.data(), e.data(), &th[0], &zl, z.data());
+            for (int a = 0; a < m; ++a) Y[(size_t)a * m] = z[a];
+          } else {  // full eigenvectors of the tridiagonal (also for a thick restart)
+            tridiag_eig_full(m, d.data(), e.data(), th.data(), Y.data());
+          }
         }
         break;
       }
