@@ -980,6 +980,7 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
 // sum (strided over 256 threads, then wave shuffles and LDS) whose order does
 // not depend on timing, so every run is bit-reproducible.
 constexpr int64_t LZ_CHUNK = 2048;  // elements per partial-sum chunk
+constexpr int LZ_GEMVT_VG = 8;      // basis vectors per block of k_lz_gemvt
 
 __device__ __forceinline__ double coop_sum(const double* __restrict__ p, int64_t cnt, double* sh) {
   double a = 0.0;
@@ -1037,15 +1038,17 @@ __global__ __launch_bounds__(256) void k_lz_scalars(const double* __restrict__ a
 }
 
 // partial[j*nch + c] = <V_j, w'> over chunk c, w' = w - alpha v_a - beta v_b
-// (alpha = ascal[0] when ascal, beta = bprev[0] when vb)
+// (alpha = ascal[0] when ascal, beta = bprev[0] when vb); block (c, g) takes
+// the chunk's products with the LZ_GEMVT_VG vectors of group g
 __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, int64_t ldv, int m,
                                                   const double* __restrict__ w, int64_t n,
                                                   double* __restrict__ partial, int64_t nch,
                                                   const double* __restrict__ ascal, const double* __restrict__ va,
                                                   const double* __restrict__ vb, const double* __restrict__ bprev) {
   constexpr int PER = LZ_CHUNK / 256;
-  __shared__ double red[LZ_MAX_BASIS][4];
+  __shared__ double red[LZ_GEMVT_VG][4];
   const int64_t c = blockIdx.x;
+  const int j0 = blockIdx.y * LZ_GEMVT_VG;  // this block's basis vectors j0 .. j0 + VG - 1
   const int64_t i0 = c * LZ_CHUNK;
   const double a = ascal ? ascal[0] : 0.0;
   const double b = vb ? bprev[0] : 0.0;
@@ -1061,20 +1064,30 @@ __global__ __launch_bounds__(256) void k_lz_gemvt(const double* __restrict__ V, 
     }
     x[u] = wi;
   }
-  const int wave = threadIdx.x >> 6;
-  for (int j = 0; j < m; ++j) {
-    const double* v = V + (size_t)j * ldv;
-    double acc = 0.0;
+  // every load of the block's vectors issued together (indices past m read
+  // vector m-1 and are not stored), then one wave sum per vector
+  double acc[LZ_GEMVT_VG];
+#pragma unroll
+  for (int q = 0; q < LZ_GEMVT_VG; ++q) {
+    const double* v = V + (size_t)min(j0 + q, m - 1) * ldv;
+    double s = 0.0;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int64_t i = i0 + threadIdx.x + 256 * u;
-      if (i < n) acc += v[i] * x[u];
+      if (i < n) s += v[i] * x[u];
     }
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[j][wave] = acc;
+    acc[q] = s;
+  }
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < LZ_GEMVT_VG; ++q) {
+    const double r = wave_sum(acc[q]);
+    if ((threadIdx.x & 63) == 0) red[q][wave] = r;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < m; j += 256) partial[(size_t)j * nch + c] = (red[j][0] + red[j][1]) + (red[j][2] + red[j][3]);
+  const int q = threadIdx.x;
+  if (q < LZ_GEMVT_VG && j0 + q < m)
+    partial[(size_t)(j0 + q) * nch + c] = (red[q][0] + red[q][1]) + (red[q][2] + red[q][3]);
 }
 
 // h[j] = sum_c partial[j*nch + c], one block per j
@@ -1165,8 +1178,8 @@ void launch_orth_pass(const double* V, int64_t ldv, int m, double* w, int64_t n,
     hipLaunchKernelGGL(k_lz_scalars, dim3(1), dim3(256), 0, s, apart, na, aout, vb ? bprev : nullptr, bout);
     ascal = aout;
   }
-  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch), dim3(256), 0, s, V, ldv, m, w, n, partial, nch, ascal, va, vb,
-                     bprev);
+  hipLaunchKernelGGL(k_lz_gemvt, dim3((unsigned)nch, (unsigned)((m + LZ_GEMVT_VG - 1) / LZ_GEMVT_VG)), dim3(256), 0, s,
+                     V, ldv, m, w, n, partial, nch, ascal, va, vb, bprev);
   hipLaunchKernelGGL(k_lz_hsum, dim3((unsigned)m), dim3(256), 0, s, partial, nch, hout);
   hipLaunchKernelGGL(k_orth_update, dim3((unsigned)orth_blocks(n)), dim3(256), sizeof(double) * (size_t)m, s, V,
                      ldv, m, hout, w, n, sq, ascal, va, vb, bprev);
