@@ -980,7 +980,9 @@ void launch_gemv_n(const double* V, int64_t ldv, int ncols, const double* h, dou
 // sum (strided over 256 threads, then wave shuffles and LDS) whose order does
 // not depend on timing, so every run is bit-reproducible.
 constexpr int64_t LZ_CHUNK = 2048;  // elements per partial-sum chunk
-constexpr int LZ_GEMVT_VG = 8;      // basis vectors per block of k_lz_gemvt
+// basis vectors per block of k_lz_gemvt: one gives the most blocks in flight
+// (Hubbard L=64 m=256 steps 5.1 / 5.2 / 5.4 / 5.9 / 6.9 ms with 1 / 2 / 4 / 8 / 16)
+constexpr int LZ_GEMVT_VG = 1;
 
 __device__ __forceinline__ double coop_sum(const double* __restrict__ p, int64_t cnt, double* sh) {
   double a = 0.0;
