@@ -387,8 +387,9 @@ static int lz_mixed_full_reorth() {
 // A near-degenerate partner (a state closer to the ground state than the
 // FP32 rounding of H*v, which an FP32 Krylov space mixes with it at O(1))
 // thereby enters the block and is separated exactly by the FP64 Rayleigh-Ritz.
-// A cycle whose |R_1| does not fall below a quarter of the previous one (with
-// an unchanged block size) hands over to the FP64 thick-restart solver from X_1.
+// A cycle whose |R_1| does not fall below a quarter of the previous one and
+// whose theta_1 does not fall by more than 1% of the previous |R_1| (with an
+// unchanged block size) hands over to the FP64 thick-restart solver from X_1.
 // NSS_MIXED_TRACE=1: one line per cycle of the mixed solver on stderr.
 static bool lz_mixed_trace() {
   static const bool v = [] {
@@ -464,7 +465,7 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
   ch.active = 1;
   NSS_HIP(hipMemsetAsync(dmap.p, 0, (size_t)nchk * sizeof(int32_t), s));
   int p = 1, prev_p = 0;
-  double prev_res = -1.0;
+  double prev_res = -1.0, prev_theta = 0.0;
   std::vector<double> G, th, nrm(PMAX), sl, cc, M, ev;
   for (int cycle = 0; cycle < std::max(8, maxiter); ++cycle) {
     // 1. FP64 block step
@@ -508,10 +509,14 @@ static double lanczos_mixed_block(Lz& L, const double* v0, int maxiter, double t
       NSS_HIP(hipStreamSynchronize(s));
       return th[0];
     }
-    // no progress from the FP32 space: FP64 from here (a cycle that grew the
-    // block has only just separated a near-degenerate pair, and is exempt)
-    if (prev_res > 0.0 && p == prev_p && res1 > 0.25 * prev_res) break;
+    // no progress from the FP32 space: FP64 from here.  Progress is a
+    // residual down by 4, or (early cycles from a random start, whose
+    // residual need not fall yet) theta down by more than 1% of the previous
+    // residual; a cycle that grew the block has only just separated a
+    // near-degenerate pair, and is exempt.
+    if (prev_res > 0.0 && p == prev_p && res1 > 0.25 * prev_res && !(prev_theta - th[0] > 0.01 * prev_res)) break;
     prev_res = res1;
+    prev_theta = th[0];
     prev_p = p;
     // 2. FP32 Lanczos space from R_1, orthogonal to X
     NSS_HIP(hipMemcpyAsync(L.w.p, R.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
