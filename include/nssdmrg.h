@@ -328,8 +328,12 @@ int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int npos, const i
  * block_left/right.restart, src/DMRG_MAIN.f90:151-152); operator keys of
  * both generations are accepted (S_z or S_z_n, C_l1_s1 or C_l1_s1_n, ...).
  * save: writes the two blocks that entered the last step (SAVE_BLOCK at
- * length Ldmrg-1, src/DMRG_MAIN.f90:308-317) with the current keys and an
- * empty rotation-matrix list. */
+ * length Ldmrg-1, src/DMRG_MAIN.f90:308-317) with the current keys.  Both
+ * carry the rotation matrices of every length (omatrices "1".."length",
+ * each over the previous length's basis in the order its file lists it);
+ * with all of them a loaded block keeps its history and a continued run can
+ * be measured, as in the reference.  A file with only "1" still continues
+ * a run. */
 int nss_dmrg_load_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
 int nss_dmrg_save_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
 int nss_dmrg_destroy(nss_dmrg* d);

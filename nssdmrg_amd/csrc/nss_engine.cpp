@@ -1003,7 +1003,35 @@ struct Engine {
     }
   };
 
-  Block load_block(const std::string& path) {
+  // Position of every state of the enlarged basis of `parent` (sector q, index
+  // within it) by its index in the reference's Kronecker order: block (x) site
+  // on the left (a d + s), site (x) block on the right (s dim + a), with a the
+  // parent state's index in the order a file lists the parent's basis (forder:
+  // file index -> (sector, index); empty: the parent's own order).
+  std::vector<std::pair<int, int>> kron_positions(const Block& parent, bool left, std::vector<Sector>& esec,
+                                                   const std::vector<std::pair<int, int>>& forder = {}) const {
+    std::vector<std::vector<Piece>> pieces;
+    enlarged_basis(parent, esec, pieces);
+    std::vector<int> aoff(parent.sec.size() + 1, 0);
+    for (size_t a = 0; a < parent.sec.size(); ++a) aoff[a + 1] = aoff[a] + parent.sec[a].dim;
+    const int dp = aoff.back(), d = site.dim;
+    std::vector<int> fidx(dp);  // own index -> file index
+    for (int g = 0; g < dp; ++g) fidx[g] = g;
+    if (!forder.empty()) {
+      NSS_REQUIRE((int)forder.size() == dp, NSS_E_ARG, "read_block: rotation matrix does not match the block below");
+      for (int g = 0; g < dp; ++g) fidx[aoff[forder[g].first] + forder[g].second] = g;
+    }
+    std::vector<std::pair<int, int>> pos((size_t)dp * d, {-1, -1});
+    for (size_t q = 0; q < pieces.size(); ++q)
+      for (const Piece& pc : pieces[q])
+        for (int ia = 0; ia < parent.sec[pc.a].dim; ++ia) {
+          const int g = fidx[aoff[pc.a] + ia];
+          pos[left ? (size_t)g * d + pc.s : (size_t)pc.s * dp + g] = {(int)q, pc.off + ia};
+        }
+    return pos;
+  }
+
+  Block load_block(const std::string& path, bool left) {
     std::ifstream in(path);
     NSS_REQUIRE(in.good(), NSS_E_ARG, ("read_block: cannot open " + path).c_str());
     Tokens tk;
@@ -1081,11 +1109,127 @@ struct Engine {
     }
     for (const auto& kv : site.ops)
       NSS_REQUIRE(b.ops.count(kv.first), NSS_E_ARG, ("read_block: operator missing: " + kv.first).c_str());
+    // rotation matrices (omatrices, keys "1".."length"): with all of them the
+    // block's history is rebuilt, which measurement walks; with fewer (older
+    // files keep only "1") the block continues a run but is not measured,
+    // as in the reference (src/BLOCKS/BLOCKS.f90:526-528)
+    struct Sp { long nr = 0, nc = 0; std::vector<long> r, c; std::vector<double> v; };
+    std::map<long, Sp> om;
+    if (tk.i < tk.t.size()) {
+      const long nom = tk.inum();
+      for (long o = 0; o < nom; ++o) {
+        const long key = std::strtol(tk.next().c_str(), nullptr, 10);
+        (void)tk.next();  // type
+        Sp u;
+        u.nr = tk.inum();
+        u.nc = tk.inum();
+        for (long r = 0; r < u.nr; ++r) {
+          const long nz = tk.inum();
+          for (long z = 0; z < nz; ++z) {
+            const long c = tk.inum() - 1;
+            const double v = tk.rnum();
+            NSS_REQUIRE(c >= 0 && c < u.nc, NSS_E_ARG, "read_block: rotation matrix column out of range");
+            if (v != 0.0) {
+              u.r.push_back(r);
+              u.c.push_back(c);
+              u.v.push_back(v);
+            }
+          }
+        }
+        om[key] = std::move(u);
+      }
+    }
+    bool history = b.length >= 1;
+    for (long k = 2; k <= b.length; ++k) history = history && om.count(k);
+    if (history && b.length >= 2) {
+      std::shared_ptr<const Block> par = std::make_shared<Block>(block_from_site());
+      // the one-site block's file order: column c of "1" holds site state s
+      // (the reference writes the identity: the site's own order)
+      std::vector<std::pair<int, int>> forder(site.dim);
+      {
+        std::vector<int> rank(site.dim, 0), cnt(par->sec.size(), 0);
+        for (int st = 0; st < site.dim; ++st) rank[st] = cnt[par->find(site.qn[st])]++;
+        std::vector<int> state_of(site.dim);
+        for (int c = 0; c < site.dim; ++c) state_of[c] = c;
+        if (om.count(1)) {
+          const Sp& u1 = om.at(1);
+          NSS_REQUIRE(u1.nr == site.dim && u1.nc == site.dim, NSS_E_ARG, "read_block: rotation matrix 1 is not the site's");
+          for (size_t z = 0; z < u1.v.size(); ++z) state_of[u1.c[z]] = (int)u1.r[z];
+        }
+        for (int c = 0; c < site.dim; ++c) forder[c] = {par->find(site.qn[state_of[c]]), rank[state_of[c]]};
+      }
+      for (int k = 2; k <= b.length; ++k) {
+        const Sp& u = om.at(k);
+        std::vector<Sector> esec;
+        const auto kp = kron_positions(*par, left, esec, forder);
+        NSS_REQUIRE(u.nr == (long)kp.size(), NSS_E_ARG, "read_block: rotation matrix rows differ from the enlarged block");
+        // each column's enlarged sector
+        std::vector<int> colq(u.nc, -1);
+        for (size_t z = 0; z < u.v.size(); ++z) {
+          const int q = kp[u.r[z]].first;
+          NSS_REQUIRE(colq[u.c[z]] < 0 || colq[u.c[z]] == q, NSS_E_ARG, "read_block: rotation matrix mixes QN sectors");
+          colq[u.c[z]] = q;
+        }
+        // the new basis: the stored block's own sector map at its length,
+        // sectors in order of first appearance (columns in file order) below
+        Block nb;
+        nb.length = k;
+        std::vector<std::pair<int, int>> cpos(u.nc);
+        if (k == b.length) {
+          NSS_REQUIRE(u.nc == dim, NSS_E_ARG, "read_block: last rotation matrix does not match Dim");
+          nb.sec = b.sec;
+          for (long c = 0; c < u.nc; ++c) cpos[c] = pos[c];
+        } else {
+          std::map<int, int> secof;
+          for (long c = 0; c < u.nc; ++c) {
+            NSS_REQUIRE(colq[c] >= 0, NSS_E_ARG, "read_block: empty rotation matrix column");
+            auto it = secof.find(colq[c]);
+            if (it == secof.end()) {
+              it = secof.emplace(colq[c], (int)nb.sec.size()).first;
+              nb.sec.push_back({esec[colq[c]].q, 0});
+            }
+            cpos[c] = {it->second, nb.sec[it->second].dim++};
+          }
+        }
+        auto B = std::make_shared<Basis>();
+        int64_t tot = 0;
+        for (size_t j = 0; j < nb.sec.size(); ++j) {
+          int q = -1;
+          for (long c = 0; c < u.nc; ++c)
+            if (cpos[c].first == (int)j && colq[c] >= 0) { q = colq[c]; break; }
+          NSS_REQUIRE(q >= 0 && esec[q].q == nb.sec[j].q, NSS_E_ARG, "read_block: rotation matrix and sector map disagree");
+          B->src.push_back(q);
+          B->dim.push_back(esec[q].dim);
+          B->m.push_back(nb.sec[j].dim);
+          B->uoff.push_back(tot);
+          tot += (int64_t)esec[q].dim * nb.sec[j].dim;
+        }
+        std::vector<double> h(std::max<int64_t>(1, tot), 0.0);
+        for (size_t z = 0; z < u.v.size(); ++z) {
+          const auto [q, loc] = kp[u.r[z]];
+          const auto [j, ic] = cpos[u.c[z]];
+          NSS_REQUIRE(B->src[j] == q, NSS_E_ARG, "read_block: rotation matrix and sector map disagree");
+          h[B->uoff[j] + (int64_t)loc * B->m[j] + ic] = u.v[z];
+        }
+        B->U.resize(ctx->pool, h.size());
+        NSS_HIP(hipMemcpyAsync(B->U.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        NSS_HIP(hipStreamSynchronize(ctx->stream));
+        forder = cpos;  // this block's file order, for the rows of the next matrix
+        if (k == b.length) {
+          b.from = B;
+          b.parent = par;
+        } else {
+          nb.from = B;
+          nb.parent = par;
+          par = std::make_shared<Block>(std::move(nb));
+        }
+      }
+    }
     NSS_HIP(hipStreamSynchronize(ctx->stream));
-    return b;  // rotation matrices are not needed to continue a run (measurement needs the history)
+    return b;
   }
 
-  void save_block(const Block& b, const std::string& path) {
+  void save_block(const Block& b, const std::string& path, bool left) {
     std::ofstream out(path);
     NSS_REQUIRE(out.good(), NSS_E_ARG, ("write_block: cannot open " + path).c_str());
     out.precision(17);
@@ -1137,7 +1281,61 @@ struct Engine {
           if (dense[(size_t)r * dim + c] != 0.0) out << c + 1 << " " << dense[(size_t)r * dim + c] << "\n";
       }
     }
-    out << 0 << "\n";  // rotation matrices: not written (read_block then warns that no measurement is possible)
+    // rotation matrices (omatrices, STORE_UMAT='b', src/DMRG_RDM.f90:223-228):
+    // "1" the one-site block's basis, "k" the kept states of the block of
+    // length k as columns over the enlarged block of length k-1 (rows in the
+    // Kronecker order of block (x) site, or site (x) block on the right),
+    // when the block carries its whole history
+    std::vector<const Block*> chain(b.length + 1, nullptr);
+    chain[b.length] = &b;
+    for (int k = b.length; k > 1 && chain[k]; --k) chain[k - 1] = chain[k]->parent.get();
+    bool full = chain[1] != nullptr && chain[1]->length == 1;
+    for (int k = 2; k <= b.length && full; ++k) full = chain[k] && chain[k]->from && chain[k]->length == k;
+    auto write_sparse = [&](const std::string& key, int nr, int nc, const std::vector<double>& dense) {
+      out << key << "\nnone\n" << nr << " " << nc << "\n";
+      for (int r = 0; r < nr; ++r) {
+        int nz = 0;
+        for (int c = 0; c < nc; ++c) nz += dense[(size_t)r * nc + c] != 0.0;
+        out << nz << "\n";
+        for (int c = 0; c < nc; ++c)
+          if (dense[(size_t)r * nc + c] != 0.0) out << c + 1 << " " << dense[(size_t)r * nc + c] << "\n";
+      }
+    };
+    out << (full ? b.length : 1) << "\n";
+    {  // "1": site state s (rows, natural order) -> the one-site block's state
+      const Block b1 = full ? *chain[1] : Block{};
+      const int d = site.dim;
+      std::vector<double> u((size_t)d * d, 0.0);
+      std::vector<int> cnt(std::max<size_t>(1, b1.sec.size()), 0), soff(b1.sec.size() + 1, 0);
+      for (size_t k = 0; k < b1.sec.size(); ++k) soff[k + 1] = soff[k] + b1.sec[k].dim;
+      for (int st = 0; st < d; ++st) {
+        const int k = full ? b1.find(site.qn[st]) : -1;
+        u[(size_t)st * d + (k >= 0 ? soff[k] + cnt[k]++ : st)] = 1.0;
+      }
+      write_sparse("1", d, d, u);
+    }
+    for (int k = 2; full && k <= b.length; ++k) {
+      const Block& bk = *chain[k];
+      const Basis& B = *bk.from;
+      std::vector<Sector> esec;
+      const auto kp = kron_positions(*chain[k - 1], left, esec);
+      const int nr = (int)kp.size(), nc = bk.dim();
+      std::vector<int> coff(bk.sec.size() + 1, 0);
+      for (size_t j = 0; j < bk.sec.size(); ++j) coff[j + 1] = coff[j] + bk.sec[j].dim;
+      int64_t tot = 0;
+      for (size_t j = 0; j < B.src.size(); ++j) tot += (int64_t)B.dim[j] * B.m[j];
+      std::vector<double> h(std::max<int64_t>(1, tot));
+      NSS_HIP(hipMemcpy(h.data(), B.U.p, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost));
+      std::vector<double> u((size_t)nr * nc, 0.0);
+      for (int r = 0; r < nr; ++r) {
+        const auto [q, loc] = kp[r];
+        for (size_t j = 0; j < B.src.size(); ++j) {
+          if (B.src[j] != q) continue;
+          for (int c = 0; c < B.m[j]; ++c) u[(size_t)r * nc + coff[j] + c] = h[B.uoff[j] + (int64_t)loc * B.m[j] + c];
+        }
+      }
+      write_sparse(std::to_string(k), nr, nc, u);
+    }
     NSS_REQUIRE(out.good(), NSS_E_HIP, ("write_block: write failed for " + path).c_str());
   }
 
@@ -2079,7 +2277,7 @@ extern "C" int nss_dmrg_load_blocks(nss_dmrg* d, const char* left_path, const ch
   NSS_REQUIRE(d && d->e && left_path && right_path, NSS_E_ARG, "nss_dmrg_load_blocks: bad arguments");
   nss::Engine* E = d->e;
   NSS_HIP(hipSetDevice(E->ctx->device));
-  nss::Block l = E->load_block(left_path), r = E->load_block(right_path);
+  nss::Block l = E->load_block(left_path, true), r = E->load_block(right_path, false);
   NSS_REQUIRE(l.length == r.length, NSS_E_ARG, "infinite_DMRG error: L.length != R.length");
   E->left = std::move(l);
   E->right = std::move(r);
@@ -2092,8 +2290,8 @@ extern "C" int nss_dmrg_save_blocks(nss_dmrg* d, const char* left_path, const ch
   nss::Engine* E = d->e;
   NSS_REQUIRE(E->lastL.parent && E->lastR.parent, NSS_E_STATE, "nss_dmrg_save_blocks: run a step first");
   NSS_HIP(hipSetDevice(E->ctx->device));
-  E->save_block(*E->lastL.parent, left_path);
-  E->save_block(*E->lastR.parent, right_path);
+  E->save_block(*E->lastL.parent, left_path, true);
+  E->save_block(*E->lastR.parent, right_path, false);
   NSS_API_END
 }
 

@@ -263,6 +263,15 @@ def test_restart_fixture_last_step(ctx, golden_dir, model):
     assert len(st) == 1 and st[0]["left_len"] == 40
     got = st[0]["e_per_site"][:len(ref)]
     assert np.max(np.abs(got - ref)) < 1e-10, (got, ref)
+    # the files hold the rotation matrices of every length in the reference's
+    # own basis order: the continued run measures the reference's profiles
+    if model == "spin12":
+        checks = {"sz": [SZ12]}
+    else:
+        checks = {"n": [NUP + NDW], "d": [NUP @ NDW], "s2z": [(NUP - NDW) @ (NUP - NDW)]}
+    for name, ops in checks.items():
+        prof = np.loadtxt(os.path.join(golden_dir, f"{tag}_{name}.check"))
+        assert np.max(np.abs(d.measure(ops, pos=np.arange(1, 41))[0] - prof)) < 1e-6, name
 
 
 @pytest.mark.parametrize("model", ["spin12", "hubbard"])
@@ -285,6 +294,22 @@ def test_restart_round_trip(ctx, tmp_path, model):
     assert abs(st2[0]["energies"][0] - st1[-1]["energies"][0]) < 1e-11
     assert abs(st2[0]["entropy_left"] - st1[-1]["entropy_left"]) < 1e-10
     assert st2[0]["m_sb"] == st1[-1]["m_sb"]
+    # the files carry the rotation matrices of every length (omatrices), so
+    # the restarted run measures: the same profile as the original run
+    if model == "spin12":
+        ops = [np.diag([0.5, -0.5])]
+    else:
+        ops = [np.diag([0.0, 1.0, 1.0, 2.0]), np.diag([0.0, 0.0, 0.0, 1.0])]
+    m1 = d1.measure(ops)
+    m2 = d2.measure(ops)
+    assert m1.shape == m2.shape
+    assert np.abs(m1 - m2).max() < 1e-9, np.abs(m1 - m2).max()
+    # written again from the restarted run, the history survives a second trip
+    d2.save_blocks(lp, rp)
+    d3 = N.DMRG(ctx, N.DMRGConfig(**kw))
+    d3.load_blocks(lp, rp)
+    d3.run("i")
+    assert np.abs(d3.measure(ops) - m1).max() < 1e-9
 
 
 def test_mixed_precision_refined_matches_fp64(ctx, monkeypatch):
