@@ -1700,6 +1700,9 @@ constexpr int TRI_THREADS = 1024;
 constexpr int TRI_WAVES = TRI_THREADS / 64;
 constexpr int TRI_MAX = 128;
 constexpr int TRI_BIS = 4;  // lanes (and interior points) per eigenvalue in the multisection
+constexpr int TRI_PB = 8;   // vectors per panel of the cluster Gram-Schmidt
+static_assert(TRI_MAX * TRI_BIS <= TRI_THREADS, "one group of TRI_BIS lanes per eigenvalue");
+static_assert(TRI_MAX <= 128, "a Gram-Schmidt panel: two rows per lane of one wave");
 constexpr int64_t TRI_WS = (int64_t)6 * TRI_MAX * TRI_MAX;  // workspace doubles per block
 
 int64_t rho_tri_workspace(int nblk) { return (int64_t)nblk * TRI_WS; }
@@ -1786,12 +1789,14 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
   __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
   __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX], yb[TRI_MAX];
   __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
+  __shared__ double cb[TRI_PB * TRI_MAX];  // C of a Gram-Schmidt panel
   const int b = blockIdx.x;
   const int n = dim[b];
   const int ld = n + 1;
   const int tid = threadIdx.x;
   const int g8 = tid >> 3, l8 = tid & 7;    // groups of 8 lanes
   const int r32 = tid >> 5, c32 = tid & 31;  // 32 x 32 thread grid
+  const int wv = tid >> 6, lane = tid & 63;  // waves
   double* A = sm;
   const double* src = a + off[b];
   double* hv = ws + (size_t)b * TRI_WS;           // reflector k at hv[k*TRI_MAX + r], r < n-k-1
@@ -2069,49 +2074,103 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
   }
   __syncthreads();
   // 4. clusters: consecutive eigenvalues of one block within 1e-3 ||T|| of
-  //    each other (dstein's reorthogonalisation criterion); right-looking modified
-  //    Gram-Schmidt, up to two sweeps.  Step j: h_k = z_j . z_k for k >= j (eight
-  //    lanes per dot), then z_k -= (h_k / h_j) z_j for k > j; z_j /= |z_j|
-  //    during step j+1, which reads neither z_j nor its scale.
+  //    each other (dstein's reorthogonalisation criterion); blocked modified
+  //    Gram-Schmidt, up to two sweeps.  A panel of TRI_PB consecutive vectors
+  //    is orthonormalised within itself by one wave holding it in registers
+  //    (two rows per lane; step j: h_k = z_j . z_k for the panel's k >= j,
+  //    z_k -= (h_k / h_j) z_j, z_j /= |z_j|), then projected out of every
+  //    later vector of the cluster at once: C = P^T Z (eight lanes per column
+  //    of Z), Z -= P C.  A cluster's vectors are read once per panel instead
+  //    of once per vector, and a panel costs three barriers.
   const double ctol = 1e-3 * tnorm;
   for (int s0 = 0; s0 < n;) {
     int e0 = s0 + 1;
     while (e0 < n && eg[e0 - 1] != 0.0 && lam[e0] - lam[e0 - 1] <= ctol) ++e0;  // within one block
     if (e0 - s0 > 1) {
       for (int sweep = 0; sweep < 2; ++sweep) {
-        double inv = 1.0;   // scale of z_{j-1}
-        double ovl = 0.0;   // largest |h_k| / h_j this thread applied
-        for (int j = s0; j < e0; ++j) {
-          const int cnt = e0 - j;  // columns j .. e0-1
-          if (j > s0 && tid < n) Z[tid * ld + (j - 1)] *= inv;
-          for (int kk = g8; kk < cnt; kk += TRI_THREADS / 8) {
-            double h = 0.0;
-#pragma unroll 4
-            for (int i = l8; i < n; i += 8) h += Z[i * ld + j] * Z[i * ld + (j + kk)];
-            h = group8_sum(h);
-            if (l8 == 0) hb[kk] = h;
-          }
-          __syncthreads();
-          const double hj = hb[0];
-          inv = 1.0 / sqrt(hj);
-          double cq[4];  // this thread's coefficients h_k / h_j, k = j + 1 + c32 + 32q
+        double ovl = 0.0;  // largest overlap coefficient this thread applied
+        for (int j0 = s0; j0 < e0; j0 += TRI_PB) {
+          const int pe = min(j0 + TRI_PB, e0);
+          if (wv == 0) {  // wave-uniform: every lane of wave 0 takes part in the sums
+            const int np = pe - j0;
+            double z0[TRI_PB], z1[TRI_PB];  // rows lane and lane + 64
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int kk = 1 + c32 + 32 * q;
-            cq[q] = kk < cnt ? hb[kk] / hj : 0.0;
-            ovl = fmax(ovl, fabs(cq[q]));
-          }
-          for (int i = r32; i < n; i += 32) {
-            const double zj = Z[i * ld + j];
+            for (int q = 0; q < TRI_PB; ++q) {
+              z0[q] = (q < np && lane < n) ? Z[lane * ld + j0 + q] : 0.0;
+              z1[q] = (q < np && lane + 64 < n) ? Z[(lane + 64) * ld + j0 + q] : 0.0;
+            }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int kk = 1 + c32 + 32 * q;
-              if (kk < cnt) Z[i * ld + (j + kk)] -= cq[q] * zj;
+            for (int j = 0; j < TRI_PB; ++j) {
+              if (j < np) {
+                double h[TRI_PB];
+#pragma unroll
+                for (int k = j; k < TRI_PB; ++k) h[k] = k < np ? tri_wave_sum(z0[j] * z0[k] + z1[j] * z1[k]) : 0.0;
+                const double hj = h[j];
+#pragma unroll
+                for (int k = j + 1; k < TRI_PB; ++k)
+                  if (k < np) {
+                    const double c = h[k] / hj;
+                    ovl = fmax(ovl, fabs(c));
+                    z0[k] -= c * z0[j];
+                    z1[k] -= c * z1[j];
+                  }
+                const double inv = 1.0 / sqrt(hj);
+                z0[j] *= inv;
+                z1[j] *= inv;
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < TRI_PB; ++q) {
+              if (q < np && lane < n) Z[lane * ld + j0 + q] = z0[q];
+              if (q < np && lane + 64 < n) Z[(lane + 64) * ld + j0 + q] = z1[q];
             }
           }
           __syncthreads();
+          const int pb = pe - j0, rest = e0 - pe;
+          if (rest > 0) {
+            for (int kk = g8; kk < rest; kk += TRI_THREADS / 8) {
+              double acc[TRI_PB];
+#pragma unroll
+              for (int q = 0; q < TRI_PB; ++q) acc[q] = 0.0;
+              for (int i = l8; i < n; i += 8) {
+                const double z = Z[i * ld + pe + kk];
+#pragma unroll
+                for (int q = 0; q < TRI_PB; ++q)
+                  if (q < pb) acc[q] += Z[i * ld + j0 + q] * z;
+              }
+#pragma unroll
+              for (int q = 0; q < TRI_PB; ++q) acc[q] = group8_sum(acc[q]);
+              if (l8 == 0)
+                for (int q = 0; q < pb; ++q) cb[q * TRI_MAX + kk] = acc[q];
+            }
+            __syncthreads();
+            double cq[4][TRI_PB];  // this thread's columns c32 + 32s of C
+#pragma unroll
+            for (int sc = 0; sc < 4; ++sc)
+#pragma unroll
+              for (int q = 0; q < TRI_PB; ++q) {
+                const int k = c32 + 32 * sc;
+                cq[sc][q] = (k < rest && q < pb) ? cb[q * TRI_MAX + k] : 0.0;
+                ovl = fmax(ovl, fabs(cq[sc][q]));
+              }
+            for (int i = r32; i < n; i += 32) {
+              double pz[TRI_PB];
+#pragma unroll
+              for (int q = 0; q < TRI_PB; ++q) pz[q] = q < pb ? Z[i * ld + j0 + q] : 0.0;
+#pragma unroll
+              for (int sc = 0; sc < 4; ++sc) {
+                const int k = c32 + 32 * sc;
+                if (k < rest) {
+                  double d = 0.0;
+#pragma unroll
+                  for (int q = 0; q < TRI_PB; ++q) d += pz[q] * cq[sc][q];
+                  Z[i * ld + pe + k] -= d;
+                }
+              }
+            }
+            __syncthreads();
+          }
         }
-        if (tid < n) Z[tid * ld + (e0 - 1)] *= inv;
         // a second sweep only where the first removed overlaps above 1e-6
         // (near-degenerate iterates); below that one sweep leaves the
         // cluster orthogonal to working precision
@@ -2206,7 +2265,7 @@ int rho_tri_max_dim(int device) {
   if (cached >= 0) return cached;
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
-  const int budget = lds - 16384;  // static LDS of the kernel (about 10 KB)
+  const int budget = lds - 24576;  // static LDS of the kernel (about 18 KB)
   int nmax = 0;
   for (int n = 3; n <= TRI_MAX; ++n)
     if (n * (n + 1) * (int)sizeof(double) <= budget) nmax = n;
