@@ -1779,36 +1779,21 @@ __device__ __forceinline__ double tri_rcp(double q) {
   return fma(fma(-q, r, 1.0), r, r);
 }
 
-__global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __restrict__ a,
-                                                             const int64_t* __restrict__ off,
-                                                             const int32_t* __restrict__ dim,
-                                                             const int64_t* __restrict__ eoff,
-                                                             double* __restrict__ vout, double* __restrict__ wout,
-                                                             double* __restrict__ ws) {
-  extern __shared__ double sm[];  // n x (n+1): A, then Z
-  __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
-  __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX], yb[TRI_MAX];
-  __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
-  __shared__ double cb[TRI_PB * TRI_MAX];  // C of a Gram-Schmidt panel
-  const int b = blockIdx.x;
-  const int n = dim[b];
-  const int ld = n + 1;
+// Householder tridiagonalisation (dsytd2's lower form) of the n x n block A
+// in LDS (row stride ld, 3 <= n <= TRI_MAX) by the whole 1024-thread
+// workgroup.  Reflector k (v[0] = 1 at row k+1, n-k-1 entries) goes to
+// rout[k rk + r]; T to dg (n), eg (n-1) and tau (k < n-2).  A is overwritten.
+__device__ __forceinline__ void tri_reduce_lds(double* A, int ld, int n, double* rout, int64_t rk, double* dg,
+                                               double* eg, double* tau, double* vb, double* pb, double* red,
+                                               double* red2, double* red3) {
   const int tid = threadIdx.x;
   const int g8 = tid >> 3, l8 = tid & 7;    // groups of 8 lanes
   const int r32 = tid >> 5, c32 = tid & 31;  // 32 x 32 thread grid
-  const int wv = tid >> 6, lane = tid & 63;  // waves
-  double* A = sm;
-  const double* src = a + off[b];
-  double* hv = ws + (size_t)b * TRI_WS;           // reflector k at hv[k*TRI_MAX + r], r < n-k-1
-  double* lu = hv + (size_t)TRI_MAX * TRI_MAX;    // factor f, row i, vector t at lu[(f*TRI_MAX + i)*TRI_MAX + t]
-  for (int i = r32; i < n; i += 32)
-    for (int j = c32; j < n; j += 32) A[i * ld + j] = src[(size_t)i * n + j];
-  __syncthreads();
-  // 1. tridiagonalisation (every branch below is uniform: its inputs are
-  //    LDS values or block sums that every thread reads alike)
-  //    Three barriers per reflector: the sums of squares of the next column
-  //    and p.v are reduced per wave before the barrier that follows their
-  //    producing phase, and summed over the waves after it.
+  // Every branch below is uniform: its inputs are LDS values or block sums
+  // that every thread reads alike.  Three barriers per reflector: the sums
+  // of squares of the next column and p.v are reduced per wave before the
+  // barrier that follows their producing phase, and summed over the waves
+  // after it.
   bool have_s2 = false;  // red2 holds the wave partials of the next column's squares
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;  // x = A[k+1..n-1][k]
@@ -1834,7 +1819,7 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
     if (tid < m) {
       const double v = (tid == 0) ? 1.0 : A[(k + 1 + tid) * ld + k] * scale;
       vb[tid] = v;
-      hv[(size_t)k * TRI_MAX + tid] = v;
+      rout[(int64_t)k * rk + tid] = v;
     }
     if (tid == 0) {
       tau[k] = t;
@@ -1901,6 +1886,35 @@ __global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __res
     eg[n - 2] = A[(n - 1) * ld + (n - 2)];
   }
   __syncthreads();
+}
+
+__global__ __launch_bounds__(TRI_THREADS) void k_rho_eig_tri(const double* __restrict__ a,
+                                                             const int64_t* __restrict__ off,
+                                                             const int32_t* __restrict__ dim,
+                                                             const int64_t* __restrict__ eoff,
+                                                             double* __restrict__ vout, double* __restrict__ wout,
+                                                             double* __restrict__ ws) {
+  extern __shared__ double sm[];  // n x (n+1): A, then Z
+  __shared__ double dg[TRI_MAX], eg[TRI_MAX], e2g[TRI_MAX], tau[TRI_MAX], lam[TRI_MAX];
+  __shared__ double vb[TRI_MAX], pb[TRI_MAX], hb[TRI_MAX], yb[TRI_MAX];
+  __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
+  __shared__ double cb[TRI_PB * TRI_MAX];  // C of a Gram-Schmidt panel
+  const int b = blockIdx.x;
+  const int n = dim[b];
+  const int ld = n + 1;
+  const int tid = threadIdx.x;
+  const int g8 = tid >> 3, l8 = tid & 7;    // groups of 8 lanes
+  const int r32 = tid >> 5, c32 = tid & 31;  // 32 x 32 thread grid
+  const int wv = tid >> 6, lane = tid & 63;  // waves
+  double* A = sm;
+  const double* src = a + off[b];
+  double* hv = ws + (size_t)b * TRI_WS;           // reflector k at hv[k*TRI_MAX + r], r < n-k-1
+  double* lu = hv + (size_t)TRI_MAX * TRI_MAX;    // factor f, row i, vector t at lu[(f*TRI_MAX + i)*TRI_MAX + t]
+  for (int i = r32; i < n; i += 32)
+    for (int j = c32; j < n; j += 32) A[i * ld + j] = src[(size_t)i * n + j];
+  __syncthreads();
+  // 1. tridiagonalisation; the reflectors to the global workspace
+  tri_reduce_lds(A, ld, n, hv, TRI_MAX, dg, eg, tau, vb, pb, red, red2, red3);
   // Gershgorin interval and ||T|| (every thread alike)
   double lo0 = HUGE_VAL, hi0 = -HUGE_VAL, tnorm = 0.0, emax2 = 0.0;
   for (int i = 0; i < n; ++i) {
