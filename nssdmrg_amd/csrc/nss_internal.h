@@ -312,11 +312,17 @@ struct TredBlk {
   double* a;    // n x n row-major symmetric block, reduced in place: afterwards
                 // row k holds reflector k at columns k+1..n-1 (v[k+1] = 1)
   double* ws;   // tred_ws_size(n) doubles: v[2][n], p[2][n], d[n], e[n], tau[n]
-  int32_t n, pad;
+  int32_t n;
+  int32_t jt;   // columns 0..jt-1 by the per-column launches, the rest by the tail
 };
 int64_t tred_ws_size(int n);
+// The tail of the per-column tridiagonalisation, one workgroup per block in
+// LDS: the pending update of reflector jt-1, then columns jt..n-1
+// (1 <= jt, 3 <= n - jt <= nmax <= 128).
+void launch_tred_tail(const TredBlk* blks, int nblk, int nmax, hipStream_t s);
 // Column j of the Householder tridiagonalisation of every block (launch j = 0
-// .. nmax-1 in order; nmax = the largest n of the batch).
+// .. (the largest jt) - 1 in order; nmax = the largest n of the batch; a
+// block is left alone from its column jt on).
 void launch_tred_col(const TredBlk* blks, int nblk, int j, int nmax, hipStream_t s);
 // The whole tridiagonalisation in one cooperative launch with the trailing
 // matrices in LDS (k_tred_lds): the plan gives each block that fits a

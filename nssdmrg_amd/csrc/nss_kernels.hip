@@ -2298,6 +2298,71 @@ void launch_rho_eig_tri(const double* a, const int64_t* off, const int32_t* dim,
   NSS_HIP(hipGetLastError());
 }
 
+// The tail of the large-block tridiagonalisation.  Once the per-column
+// launches have reduced columns 0..jt-1 of a block, the rows and columns
+// jt..n-1 still owe the update of reflector jt-1 (v in v[(jt-1)&1], p in
+// p[(jt-1)&1] over rows jt.., tau[jt-1]); one workgroup applies it,
+// A -= v w^T + w v^T with w = p - (tau/2)(p.v) v, while loading the trailing
+// block into LDS, stores reflector jt-1 to row jt-1, and reduces the
+// trailing block there.  Output in the per-column path's layout.
+__global__ __launch_bounds__(TRI_THREADS) void k_tred_tail(const TredBlk* __restrict__ blks) {
+  extern __shared__ double sm[];  // m0 x (m0+1)
+  __shared__ double dg[TRI_MAX], eg[TRI_MAX], tau[TRI_MAX], vb[TRI_MAX], pb[TRI_MAX];
+  __shared__ double red[TRI_WAVES], red2[TRI_WAVES], red3[TRI_WAVES];
+  const TredBlk B = blks[blockIdx.x];
+  const int n = B.n, jt = B.jt, m0 = n - jt;
+  if (jt < 1 || m0 < 3) return;  // not a tail (the host never launches one)
+  const int ld = m0 + 1;
+  const int tid = threadIdx.x;
+  const int r32 = tid >> 5, c32 = tid & 31;
+  double* A = B.a;
+  double* dd = B.ws + 4 * (int64_t)n;
+  double* ee = B.ws + 5 * (int64_t)n;
+  double* tt = B.ws + 6 * (int64_t)n;
+  const double* vq = B.ws + (int64_t)((jt - 1) & 1) * n;
+  const double* pq = B.ws + 2 * (int64_t)n + (int64_t)((jt - 1) & 1) * n;
+  double pv = 0.0;
+  for (int i = tid; i < m0; i += TRI_THREADS) {
+    const double v = vq[jt + i], p = pq[jt + i];
+    vb[i] = v;
+    pb[i] = p;
+    pv += p * v;
+  }
+  pv = tri_block_sum(pv, red);  // its barriers also publish vb, pb
+  const double K = 0.5 * tt[jt - 1] * pv;
+  for (int i = tid; i < m0; i += TRI_THREADS) {
+    const double w = pb[i] - K * vb[i];
+    A[(int64_t)(jt - 1) * n + jt + i] = vb[i];  // reflector jt-1 to its row
+    pb[i] = w;  // only this thread reads and writes entry i here
+  }
+  __syncthreads();
+  for (int i = r32; i < m0; i += 32)
+    for (int j = c32; j < m0; j += 32)
+      sm[i * ld + j] = A[(int64_t)(jt + i) * n + jt + j] - (vb[i] * pb[j] + pb[i] * vb[j]);
+  __syncthreads();
+  tri_reduce_lds(sm, ld, m0, A + (int64_t)jt * n + jt + 1, n + 1, dg, eg, tau, vb, pb, red, red2, red3);
+  for (int k = tid; k < m0; k += TRI_THREADS) {
+    dd[jt + k] = dg[k];
+    ee[jt + k] = k + 1 < m0 ? eg[k] : 0.0;
+    tt[jt + k] = k + 2 < m0 ? tau[k] : 0.0;
+  }
+  if (tid == 0) A[(int64_t)(n - 2) * n + (n - 1)] = 1.0;  // reflector n-2: the identity
+}
+
+void launch_tred_tail(const TredBlk* blks, int nblk, int nmax, hipStream_t s) {
+  if (nblk <= 0) return;
+  NSS_REQUIRE(nmax >= 3 && nmax <= TRI_MAX, NSS_E_ARG, "tridiagonalisation tail: size out of range");
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute((const void*)k_tred_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              TRI_MAX * (TRI_MAX + 1) * (int)sizeof(double));
+    return true;
+  }();
+  (void)attr;
+  const size_t lds = (size_t)nmax * (nmax + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_tred_tail, dim3(nblk), dim3(TRI_THREADS), lds, s, blks);
+  NSS_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------ batched dense eigensolve staging
 // Block b (dim[b] x dim[b] at src+off[b]) is copied into the b-th ncls x ncls
 // slab of dst, the rest of the slab holding -1 on the diagonal (below any

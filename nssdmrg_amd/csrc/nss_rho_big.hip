@@ -121,7 +121,7 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
   const int b = blockIdx.x % nblk, t = blockIdx.x / nblk;
   const TredBlk B = blks[b];
   const int n = B.n;
-  if (j >= n) return;
+  if (j >= B.jt) return;
   const int r0 = j + 1 + t * ROWS;
   if (t > 0 && r0 >= n) return;
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;

@@ -772,6 +772,17 @@ static int rho_big_limit() {
 
 // Number of large blocks in one rho call from which blocks of up to 768 go to
 // rocSOLVER instead of the large-block path (NSS_RHO_BIG_MANY, default 64).
+// Trailing size below which the large-block tridiagonalisation finishes in
+// one workgroup per block (NSS_TRED_TAIL, default 128, capped by the LDS of
+// the one-workgroup kernel; below 3: every column by its own launch)
+static int tred_tail_size() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_TRED_TAIL");
+    return e ? std::atoi(e) : 128;
+  }();
+  return v;
+}
+
 static int rho_big_many() {
   static const int v = [] {
     const char* e = std::getenv("NSS_RHO_BIG_MANY");
@@ -1213,6 +1224,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
       t.a = rho.p + it.roff;
       t.ws = def->ws.p + def->hwo[k];
       t.n = it.d;
+      t.jt = it.d;
       def->blk.push_back(t);
       hbeo.push_back(it.eoff);
     }
@@ -1235,17 +1247,27 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
                                st) != 0;
       if (ctx->prof) ctx->timers["rdm.count_tred_lds"] += lds_ok;
     }
+    // the rest: one launch per column until the trailing block fits one
+    // workgroup's LDS, then that tail in one launch per block set
+    const int tail = std::min(tred_tail_size(), tmax);
     std::vector<TredBlk> rest;
-    int rmax = 0;
+    int rmax = 0, jtmax = 0, tmx = 0;
     for (int k = 0; k < nb; ++k)
       if (!lds_ok || !plan.covered[k]) {
-        rest.push_back(def->blk[k]);
-        rmax = std::max(rmax, def->blk[k].n);
+        TredBlk t = def->blk[k];
+        if (tail >= 3 && t.n > tail) {
+          t.jt = t.n - tail;
+          tmx = tail;
+        }
+        rest.push_back(t);
+        rmax = std::max(rmax, t.n);
+        jtmax = std::max(jtmax, t.jt);
       }
     if (!rest.empty()) {
       dtbr.resize(ctx->pool, rest.size());
       NSS_HIP(hipMemcpyAsync(dtbr.p, rest.data(), rest.size() * sizeof(TredBlk), hipMemcpyHostToDevice, st));
-      for (int j = 0; j < rmax; ++j) launch_tred_col(dtbr.p, (int)rest.size(), j, rmax, st);
+      for (int j = 0; j < jtmax; ++j) launch_tred_col(dtbr.p, (int)rest.size(), j, rmax, st);
+      if (tmx) launch_tred_tail(dtbr.p, (int)rest.size(), tmx, st);
     }
     launch_tri_eigvals(dtb.p, dbeo.p, dD.p, nb, nbmax, st);
     def->hws.resize(wtot + nbar_d);
