@@ -900,9 +900,10 @@ void launch_tri_cluster_gs(const GsCluster* cl, int ncl, int nrmax, hipStream_t 
 }
 
 // x = H_0 (H_1 (... (H_{n-3} z))) for V vectors per wave, row lane + 64 t of
-// vector v in register z[t][v] (NPL rows per lane: n <= 64 NPL).  The next
-// reflector's entries are loaded before the current one is applied.
-template <int NPL, int V>
+// vector v in register z[t][v] (NPL rows per lane: n <= 64 NPL).  Reflectors
+// (and their tau) are loaded PF steps ahead into a ring of registers, so a
+// wave waits for memory about once per PF reflectors.
+template <int NPL, int V, int PF>
 __global__ __launch_bounds__(256) void k_tri_backtr(const BtTask* __restrict__ tasks, int ntask) {
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (w >= ntask) return;  // wave-uniform
@@ -916,45 +917,43 @@ __global__ __launch_bounds__(256) void k_tri_backtr(const BtTask* __restrict__ t
 #pragma unroll
     for (int v = 0; v < V; ++v) z[t][v] = (i < n && v < T.ns) ? T.z[(int64_t)i * T.ldz + T.s0 + v] : 0.0;
   }
-  double vr[NPL];
-  int k = n - 3;
-  if (k >= 0) {
-    const double* vk = T.a + (int64_t)k * n;
+  // ring slot u holds reflector k0 - u - PF q at step q (zero for k < 0)
+  double vr[PF][NPL], tq[PF];
+  const int k0 = n - 3;
+  auto load = [&](int k, double (&dst)[NPL], double& tk) {
+    const double* vk = T.a + (int64_t)(k > 0 ? k : 0) * n;
 #pragma unroll
     for (int t = 0; t < NPL; ++t) {
       const int i = lane + 64 * t;
-      vr[t] = (i > k && i < n) ? vk[i] : 0.0;
+      dst[t] = (k >= 0 && i > k && i < n) ? vk[i] : 0.0;
     }
-  }
-  for (; k >= 0; --k) {
-    const double tk = T.tau[k];
-    double vn[NPL];  // reflector k-1, loaded ahead
-    if (k > 0) {
-      const double* vk = T.a + (int64_t)(k - 1) * n;
+    tk = k >= 0 ? T.tau[k] : 0.0;
+  };
 #pragma unroll
-      for (int t = 0; t < NPL; ++t) {
-        const int i = lane + 64 * t;
-        vn[t] = (i > k - 1 && i < n) ? vk[i] : 0.0;
+  for (int u = 0; u < PF; ++u) load(k0 - u, vr[u], tq[u]);
+  for (int kb = k0; kb >= 0; kb -= PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int k = kb - u;
+      if (k >= 0) {  // uniform
+        const double tk = tq[u];
+        if (tk != 0.0) {
+          double d[V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) d[v] = 0.0;
+#pragma unroll
+          for (int t = 0; t < NPL; ++t)
+#pragma unroll
+            for (int v = 0; v < V; ++v) d[v] = fma(vr[u][t], z[t][v], d[v]);
+#pragma unroll
+          for (int v = 0; v < V; ++v) d[v] = tk * wsum(d[v]);
+#pragma unroll
+          for (int t = 0; t < NPL; ++t)
+#pragma unroll
+            for (int v = 0; v < V; ++v) z[t][v] = fma(-vr[u][t], d[v], z[t][v]);
+        }
+        load(k - PF, vr[u], tq[u]);
       }
-    }
-    if (tk != 0.0) {
-      double d[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) d[v] = 0.0;
-#pragma unroll
-      for (int t = 0; t < NPL; ++t)
-#pragma unroll
-        for (int v = 0; v < V; ++v) d[v] = fma(vr[t], z[t][v], d[v]);
-#pragma unroll
-      for (int v = 0; v < V; ++v) d[v] = tk * wsum(d[v]);
-#pragma unroll
-      for (int t = 0; t < NPL; ++t)
-#pragma unroll
-        for (int v = 0; v < V; ++v) z[t][v] = fma(-vr[t], d[v], z[t][v]);
-    }
-    if (k > 0) {
-#pragma unroll
-      for (int t = 0; t < NPL; ++t) vr[t] = vn[t];
     }
   }
 #pragma unroll
@@ -966,14 +965,14 @@ __global__ __launch_bounds__(256) void k_tri_backtr(const BtTask* __restrict__ t
   }
 }
 
-int backtr_vectors_per_task(int n) { return n <= 1024 ? 4 : 2; }
+int backtr_vectors_per_task(int n) { return n <= 1024 ? 1 : 1; }
 
 void launch_tri_backtr(const BtTask* tasks, int ntask, int nmax, hipStream_t s) {
   if (ntask <= 0) return;
   NSS_REQUIRE(nmax <= RHO_BIG_MAX, NSS_E_ARG, "back-transformation: block above the large-block limit");
   const dim3 grid((ntask + 3) / 4);
-  if (nmax <= 1024) hipLaunchKernelGGL((k_tri_backtr<16, 4>), grid, dim3(256), 0, s, tasks, ntask);
-  else hipLaunchKernelGGL((k_tri_backtr<32, 2>), grid, dim3(256), 0, s, tasks, ntask);
+  if (nmax <= 1024) hipLaunchKernelGGL((k_tri_backtr<16, 1, 4>), grid, dim3(256), 0, s, tasks, ntask);
+  else hipLaunchKernelGGL((k_tri_backtr<32, 1, 4>), grid, dim3(256), 0, s, tasks, ntask);
   NSS_HIP(hipGetLastError());
 }
 
