@@ -84,6 +84,18 @@ __device__ __forceinline__ double bsum(double v, double* red) {
   return r;
 }
 
+// the same with one barrier, for a red that the workgroup uses once
+template <int NW>
+__device__ __forceinline__ double bsum_once(double v, double* red) {
+  v = wsum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) r += red[w];
+  return r;
+}
+
 template <int NW>
 __device__ __forceinline__ double bmax(double v, double* red) {
   v = wmax(v);
@@ -112,12 +124,14 @@ int64_t tred_ws_size(int n) { return (int64_t)7 * n; }
 // previous v and p and row j (into registers, then LDS) and this wave's own
 // row (into registers, RV doubles per lane: n <= 64 RV), so the launch
 // waits for memory once rather than once per dependent stage.  NT threads,
-// one trailing-matrix row per wave.
+// one trailing-matrix row per wave.  Two workgroup barriers per launch: the
+// block sums of p.v and of the column's squares reduce into LDS slots of
+// their own, and v_j is scaled where it is read rather than in place.
 template <int NT, int RV>
 __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blks, int nblk, int j, int nmax) {
   constexpr int NW = NT / 64, ROWS = NW, NU = (RHO_BIG_MAX + NT - 1) / NT;
   extern __shared__ double sm[];  // vp, wp over [j, n); vj over [j, n)
-  __shared__ double red[NW];
+  __shared__ double red[NW], red2[NW];
   const int b = blockIdx.x % nblk, t = blockIdx.x / nblk;
   const TredBlk B = blks[b];
   const int n = B.n;
@@ -142,6 +156,7 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
   const double* pq = pbuf + ((j - 1) & 1) * n;
   const double* rowj = A + (int64_t)j * n + j;
   double lv[NU], lp[NU], lr[NU];
+  const double gv0 = j > 0 ? vq[j] : 0.0, gp0 = j > 0 ? pq[j] : 0.0;  // v, p at row j
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     const int i = tid + u * NT;
@@ -171,31 +186,27 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
     const int i = tid + u * NT;
     if (i < m) {
       vp[i] = lv[u];
-      wp[i] = lp[u];
       pv += lp[u] * lv[u];
     }
   }
-  pv = bsum<NW>(pv, red);  // (its barriers also publish vp, wp)
+  pv = bsum_once<NW>(pv, red);  // (its barrier also publishes vp)
   const double K = 0.5 * taup * pv;
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int i = tid + u * NT;
-    if (i < m) wp[i] = lp[u] - K * lv[u];
-  }
-  __syncthreads();
-  // row j after the previous update (the column j by symmetry), into vj
-  const double v0 = vp[0], w0 = wp[0];
+  // w over [j, n) to LDS, and row j after the previous update (the column j
+  // by symmetry) into vj; w and v at row j itself come from the loads
+  const double w0 = gp0 - K * gv0;
   double s2 = 0.0;
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
     const int i = tid + u * NT;
     if (i < m) {
-      const double r = lr[u] - v0 * wp[i] - w0 * lv[u];
+      const double w = lp[u] - K * lv[u];
+      wp[i] = w;
+      const double r = lr[u] - gv0 * w - w0 * lv[u];
       vj[i] = r;
       if (i >= 2) s2 += r * r;
     }
   }
-  s2 = bsum<NW>(s2, red);
+  s2 = bsum_once<NW>(s2, red2);  // (its barrier also publishes wp, vj)
   const double dj = vj[0];
   const double alpha = m > 1 ? vj[1] : 0.0;
   double tj = 0.0, beta = alpha, scale = 0.0;
@@ -204,9 +215,8 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
     tj = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-  __syncthreads();  // every thread has read vj[0], vj[1]
-  for (int i = tid + 1; i < m; i += NT) vj[i] = i == 1 ? 1.0 : vj[i] * scale;
-  __syncthreads();
+  // v_j at local index i >= 1 (vj holds the unscaled column)
+  auto vjs = [&](int i) { return i == 1 ? 1.0 : vj[i] * scale; };
   if (t == 0) {
     if (tid == 0) {
       dd[j] = dj;
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
       tau[j] = tj;
     }
     double* vo = vbuf + (j & 1) * n;
-    for (int i = tid + 1; i < m; i += NT) vo[j + i] = vj[i];
+    for (int i = tid + 1; i < m; i += NT) vo[j + i] = vjs(i);
     if (j > 0) {  // reflector j-1 to its row (columns j..n-1)
       double* keep = A + (int64_t)(j - 1) * n;
       for (int i = tid; i < m; i += NT) keep[j + i] = vp[i];
@@ -235,7 +245,7 @@ __global__ __launch_bounds__(NT) void k_tred_col(const TredBlk* __restrict__ blk
           a -= vi * wp[lc] + wi * vp[lc];
           row[c] = a;
         }
-        acc += a * vj[lc];
+        acc += a * vjs(lc);
       }
     }
     acc = wsum(acc);
