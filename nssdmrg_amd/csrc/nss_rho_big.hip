@@ -667,7 +667,7 @@ void launch_tri_eigvals(const TredBlk* blks, const int64_t* eoff, double* w, int
 // of z (rows outside stay as the caller zeroed them).
 __global__ __launch_bounds__(256) void k_tri_invit(const InvReq* __restrict__ rq, int nreq,
                                                    double* __restrict__ fac, int nmax) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nreq) return;
   const InvReq R = rq[r];
   const int nb = R.re - R.rs;
@@ -767,7 +767,9 @@ __global__ __launch_bounds__(256) void k_tri_invit(const InvReq* __restrict__ rq
 
 void launch_tri_invit(const InvReq* req, int nreq, double* fac, int nmax, hipStream_t s) {
   if (nreq <= 0) return;
-  hipLaunchKernelGGL(k_tri_invit, dim3((nreq + 255) / 256), dim3(256), 0, s, req, nreq, fac, nmax);
+  // one wave per workgroup: each lane streams its own factors through
+  // memory, so the vectors' waves are spread over as many CUs as they fill
+  hipLaunchKernelGGL(k_tri_invit, dim3((nreq + 63) / 64), dim3(64), 0, s, req, nreq, fac, nmax);
   NSS_HIP(hipGetLastError());
 }
 
