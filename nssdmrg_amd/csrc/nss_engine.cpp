@@ -1112,7 +1112,11 @@ struct Engine {
     // rotation matrices (omatrices, keys "1".."length"): with all of them the
     // block's history is rebuilt, which measurement walks; with fewer (older
     // files keep only "1") the block continues a run but is not measured,
-    // as in the reference (src/BLOCKS/BLOCKS.f90:526-528)
+    // as in the reference (src/BLOCKS/BLOCKS.f90:526-528).  Row r of matrix
+    // k is the state (a, s) of the enlarged block of length k-1, a d + s on
+    // the left and s dim + a on the right, a in the order that file lists
+    // the block of length k-1 (the columns of matrix k-1; the site's own
+    // order under matrix "1")
     struct Sp { long nr = 0, nc = 0; std::vector<long> r, c; std::vector<double> v; };
     std::map<long, Sp> om;
     if (tk.i < tk.t.size()) {
@@ -1141,7 +1145,9 @@ struct Engine {
     }
     bool history = b.length >= 1;
     for (long k = 2; k <= b.length; ++k) history = history && om.count(k);
-    if (history && b.length >= 2) {
+    // best effort: rotation matrices that do not fit the block (another
+    // writer's convention) leave it without history, as a file with "1" only
+    if (history && b.length >= 2) try {
       std::shared_ptr<const Block> par = std::make_shared<Block>(block_from_site());
       // the one-site block's file order: column c of "1" holds site state s
       // (the reference writes the identity: the site's own order)
@@ -1224,6 +1230,11 @@ struct Engine {
           par = std::make_shared<Block>(std::move(nb));
         }
       }
+    } catch (const Error& e) {
+      if (e.code != NSS_E_ARG) throw;
+      NSS_HIP(hipStreamSynchronize(ctx->stream));
+      b.from.reset();
+      b.parent.reset();
     }
     NSS_HIP(hipStreamSynchronize(ctx->stream));
     return b;

@@ -314,6 +314,34 @@ def test_restart_round_trip(ctx, tmp_path, model):
     assert np.abs(d3.measure(ops) - m1).max() < 1e-9
 
 
+def test_restart_file_with_unfitting_rotation_matrices(ctx, tmp_path):
+    # rotation matrices that do not fit the block (every row of matrix "2"
+    # of the left file emptied, so its kept states have no components): the
+    # block loads without its history, continues the run as before, and
+    # cannot be measured on that side
+    kw = dict(site=N.spin_site(2), hij=np.array(SPIN_HIJ), ldmrg=10, mdmrg=16, target_qn=(0.0, 0.0), neigen=1,
+              lanc_dim_threshold=64)
+    d1, st1 = run(ctx, "i", **kw)
+    lp, rp = str(tmp_path / "block_left.restart"), str(tmp_path / "block_right.restart")
+    d1.save_blocks(lp, rp)
+    lines = open(lp).read().split("\n")
+    k = next(i for i in range(len(lines) - 1) if lines[i].strip() == "2" and lines[i + 1].strip() == "none")
+    nr = int(lines[k + 2].split()[0])
+    r, body = k + 3, []  # each row: its nonzero count, then one line per entry
+    for _ in range(nr):
+        r += 1 + int(lines[r])
+        body.append("0")
+    lines[k + 3:r] = body
+    open(lp, "w").write("\n".join(lines))
+    d2 = N.DMRG(ctx, N.DMRGConfig(**kw))
+    d2.load_blocks(lp, rp)
+    st2 = d2.run("i")
+    assert abs(st2[0]["energies"][0] - st1[-1]["energies"][0]) < 1e-11
+    with pytest.raises(N.NssError):
+        d2.measure(np.diag([0.5, -0.5]), pos=[1])
+    assert np.isfinite(d2.measure(np.diag([0.5, -0.5]), pos=[st2[0]["L"]])).all()
+
+
 def test_mixed_precision_refined_matches_fp64(ctx, monkeypatch):
     # every superblock through the mixed solver (the engine's default sends
     # superblocks below 2^17 states to the FP64 solver)
