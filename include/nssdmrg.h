@@ -332,8 +332,8 @@ int nss_dmrg_measure(nss_dmrg* d, int nops, const double* ops, int npos, const i
  * carry the rotation matrices of every length (omatrices "1".."length",
  * each over the previous length's basis in the order its file lists it);
  * with all of them a loaded block keeps its history and a continued run can
- * be measured, as in the reference.  A file with only "1" still continues
- * a run. */
+ * be measured, as in the reference.  A file with only "1", or with matrices
+ * that do not fit the block, still continues a run (without measurement). */
 int nss_dmrg_load_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
 int nss_dmrg_save_blocks(nss_dmrg* d, const char* left_path, const char* right_path);
 int nss_dmrg_destroy(nss_dmrg* d);
