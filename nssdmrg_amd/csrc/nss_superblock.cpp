@@ -1091,19 +1091,17 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   const int jmax = std::min(jacobi_max_dim(ctx->device), jacobi_limit());
   if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
   const int tmax = std::min(rho_tri_max_dim(ctx->device), rho_tri_limit());
-  int bmax = rho_big_limit();
+  const int bmax = rho_big_limit();
+  // many mid-sized blocks in one call (the rho of a batch of chains): those
+  // up to 768 go to the batched rocSOLVER classes, which beat one
+  // tridiagonalisation launch per column over all of them; the larger ones
+  // (and the few large blocks of one chain at m=1024) take the large-block
+  // path, whose per-column launches then carry only them
+  int mid = 0;
   {
-    // many mid-sized blocks in one call (the rho of a batch of chains): the
-    // batched rocSOLVER classes beat one tridiagonalisation launch per column
-    // (64 Hubbard chains: 2 s less per sweep); few large blocks (one chain at
-    // m=1024) take the large-block path
-    int nbig = 0, dbig = 0;
-    for (const Item& it : items)
-      if (it.d > tmax && it.d <= bmax) {
-        ++nbig;
-        dbig = std::max(dbig, it.d);
-      }
-    if (nbig >= rho_big_many() && dbig <= 768) bmax = tmax;
+    int nbig = 0;
+    for (const Item& it : items) nbig += it.d > tmax && it.d <= bmax;
+    if (nbig >= rho_big_many()) mid = 768;
   }
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff, toff, teoff;
@@ -1112,7 +1110,7 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   for (Item& it : items) {
     it.eoff = etot;
     etot += it.d;
-    it.kind = it.d <= jmax ? 1 : it.d <= tmax ? 3 : it.d <= bmax ? 4 : 2;
+    it.kind = it.d <= jmax ? 1 : it.d <= tmax ? 3 : it.d <= mid ? 2 : it.d <= bmax ? 4 : 2;
     if (ctx->prof) {
       ctx->timers[it.kind == 1   ? "rdm.count_jacobi"
                   : it.kind == 3 ? "rdm.count_tri"
