@@ -783,6 +783,16 @@ static int tred_tail_size() {
   return v;
 }
 
+// Largest block that a call with many large blocks sends to rocSOLVER
+// (NSS_RHO_MANY_MID, default RHO_BIG_MAX: all of them; read once)
+static int rho_many_mid() {
+  static const int v = [] {
+    const char* e = std::getenv("NSS_RHO_MANY_MID");
+    return e ? std::atoi(e) : RHO_BIG_MAX;
+  }();
+  return v;
+}
+
 static int rho_big_many() {
   static const int v = [] {
     const char* e = std::getenv("NSS_RHO_BIG_MANY");
@@ -1092,16 +1102,16 @@ void sb_rdm_eigh_multi(nss_ctx* ctx, std::vector<RdmJob>& jobs, DevArr<double>& 
   if (jmax < 2) throw Error(NSS_E_HIP, "rho eigensolve: the device's LDS cannot hold a Jacobi block");
   const int tmax = std::min(rho_tri_max_dim(ctx->device), rho_tri_limit());
   const int bmax = rho_big_limit();
-  // many mid-sized blocks in one call (the rho of a batch of chains): those
-  // up to 768 go to the batched rocSOLVER classes, which beat one
-  // tridiagonalisation launch per column over all of them; the larger ones
-  // (and the few large blocks of one chain at m=1024) take the large-block
-  // path, whose per-column launches then carry only them
+  // many large blocks in one call (the rho of a batch of chains): those up
+  // to rho_many_mid() (by default all) go to the batched rocSOLVER classes,
+  // which beat one tridiagonalisation launch per column carrying all of
+  // them; the few large blocks of one chain (m=1024) take the large-block
+  // path
   int mid = 0;
   {
     int nbig = 0;
     for (const Item& it : items) nbig += it.d > tmax && it.d <= bmax;
-    if (nbig >= rho_big_many()) mid = 768;
+    if (nbig >= rho_big_many()) mid = rho_many_mid();
   }
   int64_t etot = 0;
   std::vector<int64_t> joff, jeoff, toff, teoff;
